@@ -1,0 +1,70 @@
+// Host-side launchers of the gfx950 Harvest kernels (kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+
+namespace hv {
+
+struct Quirk {  // WORLD's two overwritten bins, per (utterance, band)
+  double d1re, d1im, d2, pad;
+};
+
+// Decimation (exact chunked scan of WORLD's zero-phase IIR).
+void launch_decimate(hipStream_t s, const Config& c, const Utt* utt,
+                     const long long* c_off, long long total_chunks,
+                     const double* x, double* pass_buffer, double* states,
+                     double* carries, double* y);
+void launch_copy_input(hipStream_t s, const Config& c, const Utt* utt,
+                       const long long* y_off, long long total, const double* x,
+                       double* y);
+// y -= mean(y) per utterance, deterministic fixed-order reduction.
+void launch_remove_mean(hipStream_t s, const Config& c, const Utt* utt,
+                        const long long* p_off, long long total_blocks,
+                        const long long* y_off, long long total_y, double* y,
+                        double* partial, double* mean);
+// Two-bin correction terms of every (utterance, band).
+void launch_quirk(hipStream_t s, const Config& c, const Utt* utt,
+                  const long long* p_off, long long total_blocks, const double* y,
+                  const double* coef, const int* coef_off, const int* half,
+                  double* partial3, double* sums3, Quirk* quirk);
+void launch_fill_int(hipStream_t s, int* p, long long n, int value);
+
+// Filterbank + crossing detection for bands [band0, band0 + group).
+void launch_filter_events(hipStream_t s, const Config& c, const Utt* utt,
+                          const long long* t_off, long long total_tiles,
+                          int band0, int group, int max_half, const double* y,
+                          const double* coef, const int* coef_off,
+                          const int* half, const Quirk* quirk, double* slab,
+                          int* counts);
+void launch_tile_scan(hipStream_t s, const Config& c, const Utt* utt,
+                      long long total_tiles, int group, const int* counts,
+                      int* offs, int* totals);
+void launch_interp(hipStream_t s, const Config& c, const Utt* utt,
+                   const long long* t_off, long long total_tiles,
+                   long long total_frames, int band0, int group,
+                   const double* boundaries, const double* slab,
+                   const int* counts, const int* offs, const int* totals,
+                   double* raw);
+void launch_official(hipStream_t s, const Config& c, const long long* f_off,
+                     long long total_frames, const double* raw, double* official,
+                     int* count);
+void launch_refine(hipStream_t s, const Config& c, const Utt* utt,
+                   const long long* f_off, long long total_frames,
+                   const double* y, const double* official, const int* count,
+                   const double* twiddle, double* rf0, double* rscore);
+void launch_prune_best(hipStream_t s, const Config& c, const Utt* utt,
+                       const long long* f_off, long long total_frames,
+                       const int* count, const double* rf0,
+                       const double* rscore, double* pf0, double* pscore,
+                       double* best, int* row_nnz);
+// Exclusive scan of n ints into out[0..n] (out[n] = total).
+void launch_exclusive_scan(hipStream_t s, const int* in, int* out, long long n,
+                           int* block_tmp, long long block_tmp_len);
+long long scan_block_count(long long n);
+void launch_compact(hipStream_t s, const Config& c, const Utt* utt,
+                    const long long* f_off, long long total_frames,
+                    const int* count, const double* pf0, const double* pscore,
+                    const int* offsets, double* out_f0, double* out_score);
+
+}  // namespace hv

@@ -1,0 +1,1049 @@
+// Hand-written gfx950 kernels of the Harvest hot path.  All arithmetic is FP64
+// and the file is compiled with -ffp-contract=off: wherever WORLD's expression
+// order matters for a discrete decision it is kept, and fma() appears only in
+// accumulations whose order WORLD's FFTs do not define anyway.
+//
+// Every kernel processes a batch of utterances laid out by hv::Utt; a plain
+// call is a batch of one, and every per-utterance reduction has a fixed order
+// that does not depend on the batch, so batched results are bitwise equal to
+// plain ones.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+
+#include "kernels.h"
+
+namespace hv {
+namespace {
+
+void check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    throw std::runtime_error(std::string("HIP kernel launch failed (") + what +
+                             "): " + hipGetErrorString(e));
+}
+
+unsigned grid_for(long long n) {
+  return (unsigned)((n + kThreads - 1) / kThreads);
+}
+
+// Fixed-order tree reduction of one double per thread (result in thread 0).
+__device__ double block_sum(double v, double* lds) {
+  lds[threadIdx.x] = v;
+  __syncthreads();
+  for (int stride = kThreads / 2; stride > 0; stride >>= 1) {
+    if ((int)threadIdx.x < stride) lds[threadIdx.x] += lds[threadIdx.x + stride];
+    __syncthreads();
+  }
+  double r = lds[0];
+  __syncthreads();
+  return r;
+}
+
+// Exclusive scan over the workgroup of an unsigned 64-bit value; `total`
+// receives the sum.  Wave64 shuffles, then one LDS pass over the 4 waves.
+__device__ unsigned long long block_scan_u64(unsigned long long v,
+                                             unsigned long long* wave_sums,
+                                             unsigned long long& total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  unsigned long long incl = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    unsigned long long o = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += o;
+  }
+  if (lane == 63) wave_sums[wave] = incl;
+  __syncthreads();
+  unsigned long long before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < kThreads / 64; ++w) {
+    unsigned long long s = wave_sums[w];
+    if (w < wave) before += s;
+    all += s;
+  }
+  total = all;
+  __syncthreads();
+  return before + incl - v;
+}
+
+// ---------------------------------------------------------------------------
+// Decimation.  WORLD runs a 3rd-order IIR forward over the padded signal,
+// then again over the reversed output, and keeps every r-th sample
+// (matlabfunctions.cpp:178-204 with the padding of harvest.cpp:50-62).  The
+// IIR state s = (w_{t-1}, w_{t-2}, w_{t-3}) evolves as s' = A s + x e1, so a
+// chunk of kChunk samples maps s to P s + z with P = A^kChunk.  Phase 1
+// computes z per chunk, phase 2 scans the chunk carries per utterance, phase
+// 3 replays every chunk from its exact carried state with WORLD's expression.
+
+// Pass 0 reads the replicate-padded input: WORLD pads x by `lag` copies of its
+// end samples and decimate() adds 9 odd reflections that, on a constant run,
+// equal the end samples again (2a - a = a exactly).  Pass 1 reads pass 0's
+// output backwards.
+template <int PASS>
+__device__ __forceinline__ double iir_input(const Config& c, const Utt& u,
+                                            const double* x, const double* w,
+                                            long long i) {
+  if (PASS == 0) {
+    long long j = i - 9 - c.lag;
+    j = j < 0 ? 0 : (j >= u.length ? u.length - 1 : j);
+    return x[u.x_off + j];
+  }
+  return w[u.m_off + (u.m - 1 - i)];
+}
+
+template <int PASS>
+__global__ __launch_bounds__(kThreads) void k_iir_chunk_state(
+    Config c, const Utt* utt, const long long* c_off, long long total,
+    const double* x, const double* w, double* states) {
+  long long cg = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (cg >= total) return;
+  const Utt u = utt[find_segment(c_off, c.count, cg)];
+  long long start = (cg - u.c_off) * kChunk;
+  long long end = min(start + (long long)kChunk, u.m);
+  double w0 = 0.0, w1 = 0.0, w2 = 0.0;
+  for (long long i = start; i < end; ++i) {
+    double wt = iir_input<PASS>(c, u, x, w, i) + c.a0 * w0 + c.a1 * w1 + c.a2 * w2;
+    w2 = w1;
+    w1 = w0;
+    w0 = wt;
+  }
+  states[cg * 3 + 0] = w0;
+  states[cg * 3 + 1] = w1;
+  states[cg * 3 + 2] = w2;
+}
+
+__device__ __forceinline__ void mat3_mul(const double* a, const double* b, double* r) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      r[i * 3 + j] = a[i * 3 + 0] * b[0 * 3 + j] + a[i * 3 + 1] * b[1 * 3 + j] +
+                     a[i * 3 + 2] * b[2 * 3 + j];
+}
+
+__device__ __forceinline__ void mat3_apply(const double* m, const double* v,
+                                           const double* add, double* r) {
+  double t0 = m[0] * v[0] + m[1] * v[1] + m[2] * v[2] + add[0];
+  double t1 = m[3] * v[0] + m[4] * v[1] + m[5] * v[2] + add[1];
+  double t2 = m[6] * v[0] + m[7] * v[1] + m[8] * v[2] + add[2];
+  r[0] = t0;
+  r[1] = t1;
+  r[2] = t2;
+}
+
+// One workgroup per utterance: each thread owns a run of K consecutive chunks,
+// runs are combined with a Kogge-Stone scan using (P^K)^(2^j), then every run
+// replays its chunks to emit the carry-in state of each chunk.
+__global__ __launch_bounds__(kThreads) void k_iir_carry(Config c, const Utt* utt,
+                                                       const double* states,
+                                                       double* carries) {
+  __shared__ double qpow[8][9];
+  __shared__ double scan[kThreads][3];
+  const Utt u = utt[blockIdx.x];
+  const int C = u.chunks;
+  const int K = (C + kThreads - 1) / kThreads;
+  const double* P = c.chunk_pow;
+  if (threadIdx.x == 0) {
+    double q[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, base[9], tmp[9];
+    for (int k = 0; k < 9; ++k) base[k] = P[k];
+    for (int e = K; e > 0; e >>= 1) {
+      if (e & 1) {
+        mat3_mul(q, base, tmp);
+        for (int k = 0; k < 9; ++k) q[k] = tmp[k];
+      }
+      mat3_mul(base, base, tmp);
+      for (int k = 0; k < 9; ++k) base[k] = tmp[k];
+    }
+    for (int k = 0; k < 9; ++k) qpow[0][k] = q[k];
+    for (int j = 1; j < 8; ++j) mat3_mul(qpow[j - 1], qpow[j - 1], qpow[j]);
+  }
+  const long long first = (long long)threadIdx.x * K;
+  const long long last = min(first + (long long)K, (long long)C);
+  const double* z = states + u.c_off * 3;
+  double agg[3] = {0.0, 0.0, 0.0};
+  for (long long k = first; k < last; ++k) mat3_apply(P, agg, z + k * 3, agg);
+  scan[threadIdx.x][0] = agg[0];
+  scan[threadIdx.x][1] = agg[1];
+  scan[threadIdx.x][2] = agg[2];
+  __syncthreads();
+  for (int j = 0; j < 8; ++j) {
+    int d = 1 << j;
+    double nv[3] = {scan[threadIdx.x][0], scan[threadIdx.x][1], scan[threadIdx.x][2]};
+    if ((int)threadIdx.x >= d) {
+      double prev[3] = {scan[threadIdx.x - d][0], scan[threadIdx.x - d][1],
+                        scan[threadIdx.x - d][2]};
+      mat3_apply(qpow[j], prev, nv, nv);
+    }
+    __syncthreads();
+    scan[threadIdx.x][0] = nv[0];
+    scan[threadIdx.x][1] = nv[1];
+    scan[threadIdx.x][2] = nv[2];
+    __syncthreads();
+  }
+  double carry[3] = {0.0, 0.0, 0.0};
+  if (threadIdx.x > 0) {
+    carry[0] = scan[threadIdx.x - 1][0];
+    carry[1] = scan[threadIdx.x - 1][1];
+    carry[2] = scan[threadIdx.x - 1][2];
+  }
+  double* out = carries + u.c_off * 3;
+  for (long long k = first; k < last; ++k) {
+    out[k * 3 + 0] = carry[0];
+    out[k * 3 + 1] = carry[1];
+    out[k * 3 + 2] = carry[2];
+    mat3_apply(P, carry, z + k * 3, carry);
+  }
+}
+
+template <int PASS>
+__global__ __launch_bounds__(kThreads) void k_iir_replay(
+    Config c, const Utt* utt, const long long* c_off, long long total,
+    const double* x, double* w, const double* carries, double* y) {
+  long long cg = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (cg >= total) return;
+  const Utt u = utt[find_segment(c_off, c.count, cg)];
+  long long start = (cg - u.c_off) * kChunk;
+  long long end = min(start + (long long)kChunk, u.m);
+  double w0 = carries[cg * 3 + 0], w1 = carries[cg * 3 + 1], w2 = carries[cg * 3 + 2];
+  // decimate(): y[count] = tmp1[nbeg + count r + 8] of the twice-reversed
+  // output, i.e. pass-1 index m - 9 - nbeg - count r; harvest.cpp:62 keeps
+  // count = lag / r + j.
+  const long long out_base =
+      u.m - 9 - u.nbeg - (long long)(c.lag / c.ratio) * c.ratio;
+  for (long long i = start; i < end; ++i) {
+    double wt = iir_input<PASS>(c, u, x, w, i) + c.a0 * w0 + c.a1 * w1 + c.a2 * w2;
+    double out = c.b0 * wt + c.b1 * w0 + c.b1 * w1 + c.b0 * w2;
+    w2 = w1;
+    w1 = w0;
+    w0 = wt;
+    if (PASS == 0) {
+      w[u.m_off + i] = out;
+    } else {
+      long long d = out_base - i;
+      if (d >= 0 && d % c.ratio == 0) {
+        long long j = d / c.ratio;
+        if (j < u.n) y[u.y_off + j] = out;
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_copy_input(
+    Config c, const Utt* utt, const long long* y_off, long long total,
+    const double* x, double* y) {
+  long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= total) return;
+  const Utt u = utt[find_segment(y_off, c.count, i)];
+  y[i] = x[u.x_off + (i - u.y_off)];
+}
+
+// ---------------------------------------------------------------------------
+// Mean removal (harvest.cpp:81-85) with a deterministic two-level sum.
+__global__ __launch_bounds__(kThreads) void k_partial_sum(
+    Config c, const Utt* utt, const long long* p_off, const double* y,
+    double* partial) {
+  __shared__ double lds[kThreads];
+  long long pb = blockIdx.x;
+  const Utt u = utt[find_segment(p_off, c.count, pb)];
+  long long start = (pb - u.p_off) * kMeanBlock;
+  long long end = min(start + (long long)kMeanBlock, (long long)u.n);
+  double s = 0.0;
+  for (long long i = start + threadIdx.x; i < end; i += kThreads) s += y[u.y_off + i];
+  s = block_sum(s, lds);
+  if (threadIdx.x == 0) partial[pb] = s;
+}
+
+__global__ __launch_bounds__(kThreads) void k_mean_final(Config c, const Utt* utt,
+                                                        const double* partial,
+                                                        double* mean) {
+  __shared__ double lds[kThreads];
+  const Utt u = utt[blockIdx.x];
+  double s = 0.0;
+  for (int k = threadIdx.x; k < u.pblocks; k += kThreads) s += partial[u.p_off + k];
+  s = block_sum(s, lds);
+  if (threadIdx.x == 0) mean[blockIdx.x] = s / u.n;
+}
+
+__global__ __launch_bounds__(kThreads) void k_subtract(Config c, const long long* y_off,
+                                                      long long total, double* y,
+                                                      const double* mean) {
+  long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= total) return;
+  y[i] -= mean[find_segment(y_off, c.count, i)];
+}
+
+// ---------------------------------------------------------------------------
+// WORLD overwrites bins n/2-1 and n/2 of every band's product spectrum with
+// S[n/2] * P[n/2-1] (harvest.cpp:122-133).  The filtered signal is therefore
+// the linear convolution plus (-1)^k [2 Re(D1 e^{-2 pi i k / n}) + D2] / n
+// with D1 = (S[n/2] - 1) P[n/2-1] and D2 = S[n/2] Re P[n/2-1] - P[n/2].
+__global__ __launch_bounds__(kThreads) void k_quirk_partial(
+    Config c, const Utt* utt, const long long* p_off, const double* y,
+    double* partial3) {
+  __shared__ double lds[kThreads];
+  long long pb = blockIdx.x;
+  const Utt u = utt[find_segment(p_off, c.count, pb)];
+  long long start = (pb - u.p_off) * kMeanBlock;
+  long long end = min(start + (long long)kMeanBlock, (long long)u.n);
+  double sh = 0.0, sr = 0.0, si = 0.0;
+  for (long long i = start + threadIdx.x; i < end; i += kThreads) {
+    double v = y[u.y_off + i];
+    if (i & 1) v = -v;
+    double sn, cs;
+    sincospi(2.0 * (double)i / u.fft, &sn, &cs);
+    sh += v;
+    sr = fma(v, cs, sr);
+    si = fma(v, sn, si);
+  }
+  sh = block_sum(sh, lds);
+  sr = block_sum(sr, lds);
+  si = block_sum(si, lds);
+  if (threadIdx.x == 0) {
+    partial3[pb * 3 + 0] = sh;
+    partial3[pb * 3 + 1] = sr;
+    partial3[pb * 3 + 2] = si;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_quirk_final(Config c, const Utt* utt,
+                                                         const double* partial3,
+                                                         double* sums3) {
+  __shared__ double lds[kThreads];
+  const Utt u = utt[blockIdx.x];
+  double a[3] = {0.0, 0.0, 0.0};
+  for (int k = threadIdx.x; k < u.pblocks; k += kThreads)
+    for (int q = 0; q < 3; ++q) a[q] += partial3[(u.p_off + k) * 3 + q];
+  for (int q = 0; q < 3; ++q) {
+    double s = block_sum(a[q], lds);
+    if (threadIdx.x == 0) sums3[blockIdx.x * 3 + q] = s;
+  }
+}
+
+// One thread per (utterance, band): filter spectrum at the two bins, direct
+// sums over the 2h+1 taps (coefficients are stored reversed: cr[mm] = c[2h-mm]).
+__global__ __launch_bounds__(kThreads) void k_quirk_bands(
+    Config c, const Utt* utt, const double* sums3, const double* coef,
+    const int* coef_off, const int* half, Quirk* quirk) {
+  long long id = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (id >= (long long)c.count * c.bands) return;
+  int ui = (int)(id / c.bands), b = (int)(id % c.bands);
+  const Utt u = utt[ui];
+  const double* cr = coef + coef_off[b];
+  int h = half[b];
+  double hh = 0.0, hr = 0.0, hi = 0.0;
+  for (int m = 0; m <= 2 * h; ++m) {
+    double v = cr[2 * h - m];
+    if (m & 1) v = -v;
+    double sn, cs;
+    sincospi(2.0 * (double)m / u.fft, &sn, &cs);
+    hh += v;
+    hr = fma(v, cs, hr);
+    hi = fma(v, sn, hi);
+  }
+  double shalf = sums3[ui * 3 + 0], s1r = sums3[ui * 3 + 1], s1i = sums3[ui * 3 + 2];
+  double p1r = s1r * hr - s1i * hi;
+  double p1i = s1r * hi + s1i * hr;
+  double p2 = shalf * hh;
+  Quirk q;
+  q.d1re = shalf * p1r - p1r;
+  q.d1im = shalf * p1i - p1i;
+  q.d2 = shalf * p1r - p2;
+  q.pad = 0.0;
+  quirk[id] = q;
+}
+
+__global__ __launch_bounds__(kThreads) void k_fill_int(int* p, long long n, int v) {
+  long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Filterbank + crossings.  One workgroup = one (tile of kTile samples, band).
+// f[i] = sum_{m=0}^{2h} c[m] y[i + h + 1 - m] + Q(i + h + 1) / n reproduces
+// WORLD's filtered_signal[i] / n (harvest.cpp:115-142; c2r scaling is a
+// power of two and cancels in every crossing).  Each thread produces 8
+// consecutive outputs with a register-blocked FIR from LDS, then the four
+// crossing types of harvest.cpp:206-238 are detected per sample pair, their
+// sub-sample edges computed with ZeroCrossingEngine's formula
+// (harvest.cpp:184-186), compacted in order and written to the tile's slab.
+template <bool kLds>
+__global__ __launch_bounds__(kThreads) void k_filter_events(
+    Config c, const Utt* utt, const long long* t_off, long long total_tiles,
+    int band0, const double* y, const double* coef, const int* coef_off,
+    const int* half, const Quirk* quirk, double* slab, int* counts) {
+  extern __shared__ double smem[];
+  __shared__ unsigned long long wave_sums[kThreads / 64];
+  const long long g = blockIdx.x;
+  const int bg = blockIdx.y;
+  const int b = band0 + bg;
+  const int ui = find_segment(t_off, c.count, g);
+  const Utt u = utt[ui];
+  const long long i0 = (g - u.t_off) * kTile;
+  const int N = u.n;
+  const int h = half[b];
+  const int taps8 = (2 * h + 1 + 7) & ~7;
+  const double* cr = coef + coef_off[b];
+  const double* yu = y + u.y_off;
+  const long long base = i0 - h + 1;
+  const int span = kTile + taps8 + 10;
+  if (kLds) {
+    for (int s = threadIdx.x; s < span; s += kThreads) {
+      long long idx = base + s;
+      smem[s] = (idx >= 0 && idx < N) ? yu[idx] : 0.0;
+    }
+    __syncthreads();
+  }
+  auto Y = [&](int s) -> double {
+    if (kLds) return smem[s];
+    long long idx = base + s;
+    return (idx >= 0 && idx < N) ? yu[idx] : 0.0;
+  };
+
+  const int j0 = threadIdx.x * 8;
+  double acc[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) acc[r] = 0.0;
+  for (int mm = 0; mm < taps8; mm += 8) {
+    double v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = Y(j0 + mm + q);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const double cc = cr[mm + k];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) acc[r] = fma(cc, v[r + k], acc[r]);
+    }
+  }
+  // Two halo outputs (kTile, kTile + 1) for the derivative crossings.
+  double halo = 0.0;
+  if (threadIdx.x < 2) {
+    const int j = kTile + threadIdx.x;
+    for (int mm = 0; mm < taps8; ++mm) halo = fma(cr[mm], Y(j + mm), halo);
+  }
+  // Two-bin correction and scale.
+  const Quirk qk = quirk[(long long)ui * c.bands + b];
+  const double inv_n = 1.0 / u.fft;
+  const long long nmask = u.fft - 1;
+  auto correction = [&](long long k) -> double {
+    double sn, cs;
+    sincospi(2.0 * (double)(k & nmask) / u.fft, &sn, &cs);
+    double q = 2.0 * (qk.d1re * cs + qk.d1im * sn) + qk.d2;
+    return (k & 1) ? -q : q;
+  };
+  if (kLds) __syncthreads();
+  double* f = smem;
+#pragma unroll
+  for (int r = 0; r < 8; ++r)
+    f[j0 + r] = acc[r] + correction(i0 + j0 + r + h + 1) * inv_n;
+  if (threadIdx.x < 2) {
+    const int j = kTile + threadIdx.x;
+    f[j] = halo + correction(i0 + j + h + 1) * inv_n;
+  }
+  __syncthreads();
+
+  // Crossing flags: bit (type * 8 + r).  Types: 0 negative-going, 1
+  // positive-going, 2 peak, 3 dip; WORLD's order (harvest.cpp:218-237).
+  unsigned flags = 0;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int j = j0 + r;
+    const long long i = i0 + j;
+    if (i <= N - 2) {
+      const double a = f[j], bb = f[j + 1];
+      if (0.0 < a && bb <= 0.0) flags |= 1u << (0 * 8 + r);
+      if (a < 0.0 && bb >= 0.0) flags |= 1u << (1 * 8 + r);
+    }
+    if (i <= N - 3) {
+      const double d0 = f[j + 1] - f[j], d1 = f[j + 2] - f[j + 1];
+      if (0.0 < d0 && d1 <= 0.0) flags |= 1u << (2 * 8 + r);
+      if (d0 < 0.0 && d1 >= 0.0) flags |= 1u << (3 * 8 + r);
+    }
+  }
+  unsigned long long packed = 0;
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+    packed |= (unsigned long long)__popc((flags >> (t * 8)) & 0xffu) << (16 * t);
+  unsigned long long total;
+  unsigned long long excl = block_scan_u64(packed, wave_sums, total);
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    unsigned bits = (flags >> (t * 8)) & 0xffu;
+    int pos = (int)((excl >> (16 * t)) & 0xffffu);
+    double* out = slab + ((long long)(bg * 4 + t) * total_tiles + g) * kSlabCap;
+    while (bits) {
+      const int r = __ffs(bits) - 1;
+      bits &= bits - 1;
+      const int j = j0 + r;
+      const double e = (double)(i0 + j + 1);
+      double fe;
+      if (t < 2) {
+        fe = e - f[j] / (f[j + 1] - f[j]);
+      } else {
+        const double d0 = f[j + 1] - f[j], d1 = f[j + 2] - f[j + 1];
+        fe = e - d0 / (d1 - d0);
+      }
+      out[pos++] = fe;
+    }
+  }
+  if (threadIdx.x < 4)
+    counts[(long long)(bg * 4 + threadIdx.x) * total_tiles + g] =
+        (int)((total >> (16 * threadIdx.x)) & 0xffffu);
+}
+
+// Exclusive scan of per-tile crossing counts within each utterance, for every
+// (band in group, type).  One workgroup per (utterance, band*4+type).
+__global__ __launch_bounds__(kThreads) void k_tile_scan(Config c, const Utt* utt,
+                                                       long long total_tiles,
+                                                       const int* counts, int* offs,
+                                                       int* totals) {
+  __shared__ unsigned long long wave_sums[kThreads / 64];
+  const int ui = blockIdx.x;
+  const long long q = blockIdx.y;
+  const Utt u = utt[ui];
+  const int* cnt = counts + q * total_tiles + u.t_off;
+  int* out = offs + q * total_tiles + u.t_off;
+  const int T = u.tiles;
+  const int K = (T + kThreads - 1) / kThreads;
+  const int first = threadIdx.x * K;
+  const int last = min(first + K, T);
+  unsigned long long s = 0;
+  for (int k = first; k < last; ++k) s += (unsigned long long)cnt[k];
+  unsigned long long total;
+  unsigned long long run = block_scan_u64(s, wave_sums, total);
+  for (int k = first; k < last; ++k) {
+    out[k] = (int)run;
+    run += (unsigned long long)cnt[k];
+  }
+  if (threadIdx.x == 0) totals[q * c.count + ui] = (int)total;
+}
+
+// Raw candidate contour (harvest.cpp:260-293): for each 1 ms frame whose
+// sample position falls in this tile, interp1 with histc's end extrapolation
+// (matlabfunctions.cpp:136-176) over the interval list of each crossing type,
+// average, and gate (harvest.cpp:240-254).  Frame i of a tile only needs
+// crossings of ranks [R_lo - 3, R_hi + 2] (R: ranks inside the tile, same-type
+// edges are more than one sample apart), so each type's window of ranks
+// [R_lo - 4, R_hi + 4] is staged in LDS.
+__global__ __launch_bounds__(kThreads) void k_interp(
+    Config c, const Utt* utt, const long long* t_off, long long total_tiles,
+    long long total_frames, int band0, const double* boundaries,
+    const double* slab, const int* counts, const int* offs, const int* totals,
+    double* raw) {
+  constexpr int kWin = kSlabCap + 2 * kWindowMargin + 2;
+  __shared__ double win[4][kWin];
+  __shared__ int wlo_s[4], whi_s[4], E_s[4];
+  const long long g = blockIdx.x;
+  const int bg = blockIdx.y;
+  const int b = band0 + bg;
+  const int ui = find_segment(t_off, c.count, g);
+  const Utt u = utt[ui];
+  const long long tau = g - u.t_off;
+  const long long scale = (long long)kTile * 1000LL * c.ratio;
+  long long i_lo = (tau * scale + c.fs - 1) / c.fs;
+  long long i_hi = (tau == u.tiles - 1) ? (long long)u.frames - 1
+                                        : ((tau + 1) * scale + c.fs - 1) / c.fs - 1;
+  if (i_hi > u.frames - 1) i_hi = u.frames - 1;
+  if (i_lo > i_hi) return;
+  double* out = raw + (long long)b * total_frames + u.f_off;
+
+  bool valid = true;
+  for (int t = 0; t < 4; ++t)
+    valid = valid && totals[(long long)(bg * 4 + t) * c.count + ui] >= 4;
+  if (!valid) {
+    for (long long i = i_lo + threadIdx.x; i <= i_hi; i += kThreads) out[i] = 0.0;
+    return;
+  }
+  for (int t = 0; t < 4; ++t) {
+    const long long q = bg * 4 + t;
+    const int* qcnt = counts + q * total_tiles;
+    const int* qoff = offs + q * total_tiles;
+    const int E = totals[q * c.count + ui];
+    const int r_lo = qoff[g], n_in = qcnt[g];
+    const int wlo = max(0, r_lo - kWindowMargin);
+    const int whi = min(E - 1, r_lo + n_in - 1 + kWindowMargin);
+    for (int k = wlo + threadIdx.x; k <= whi; k += kThreads) {
+      long long gt;
+      int local;
+      if (k >= r_lo && k < r_lo + n_in) {
+        gt = g;
+        local = k - r_lo;
+      } else {
+        long long lo = u.t_off, hi = u.t_off + u.tiles - 1;
+        while (lo < hi) {
+          long long mid = (lo + hi + 1) >> 1;
+          if (qoff[mid] <= k) lo = mid; else hi = mid - 1;
+        }
+        gt = lo;
+        local = k - qoff[gt];
+      }
+      win[t][k - wlo] = slab[(q * total_tiles + gt) * kSlabCap + local];
+    }
+    if (threadIdx.x == 0) {
+      wlo_s[t] = wlo;
+      whi_s[t] = whi;
+      E_s[t] = E;
+    }
+  }
+  __syncthreads();
+
+  const double fs = c.fs_work;
+  const double bound = boundaries[b];
+  const double upper = bound * 1.1, lower = bound * 0.9;
+  for (long long i = i_lo + threadIdx.x; i <= i_hi; i += kThreads) {
+    const double t = (double)i / 1000.0;
+    double v[4];
+#pragma unroll
+    for (int ty = 0; ty < 4; ++ty) {
+      const double* fe = win[ty];
+      const int wlo = wlo_s[ty], E = E_s[ty];
+      // k* = number of interval locations <= t (histc); loc_k needs edges
+      // k and k + 1, so k ranges over [wlo, min(whi, E - 1)).
+      int lo = wlo;
+      int hi = min(whi_s[ty], E - 1);
+      while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        double loc = (fe[mid - wlo] + fe[mid + 1 - wlo]) / 2.0 / fs;
+        if (loc <= t) lo = mid + 1; else hi = mid;
+      }
+      const int kc = min(max(lo, 1), E - 2);
+      const double e0 = fe[kc - 1 - wlo], e1 = fe[kc - wlo], e2 = fe[kc + 1 - wlo];
+      const double x0 = (e0 + e1) / 2.0 / fs, x1 = (e1 + e2) / 2.0 / fs;
+      const double y0 = fs / (e1 - e0), y1 = fs / (e2 - e1);
+      const double s = (t - x0) / (x1 - x0);
+      v[ty] = y0 + s * (y1 - y0);
+    }
+    double cand = (v[0] + v[1] + v[2] + v[3]) / 4.0;
+    if (cand > upper || cand < lower || cand > c.f0_ceil || cand < c.f0_floor) cand = 0.0;
+    out[i] = cand;
+  }
+}
+
+// Official candidates (harvest.cpp:348-412): per frame, runs of >= 10 voiced
+// bands (bands 0 and B-1 forced unvoiced) averaged in band order.
+__global__ __launch_bounds__(kThreads) void k_official(Config c, const long long* f_off,
+                                                      long long total_frames,
+                                                      const double* raw,
+                                                      double* official, int* count) {
+  long long gf = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (gf >= total_frames) return;
+  const int ui = find_segment(f_off, c.count, gf);
+  double* o = official + gf * c.capacity;
+  int n = 0, st = 0;
+  bool prev = false;
+  double sum = 0.0;
+  for (int b = 0; b < c.bands; ++b) {
+    const double v = raw[(long long)b * total_frames + gf];
+    const bool on = b > 0 && b < c.bands - 1 && v > 0;
+    if (on) {
+      if (!prev) {
+        st = b;
+        sum = 0.0;
+      }
+      sum += v;
+    } else if (prev) {
+      const int len = b - st;
+      if (len >= 10) {
+        if (n < c.capacity) o[n] = sum / len;
+        ++n;
+      }
+    }
+    prev = on;
+  }
+  n = min(n, c.capacity);
+  for (int k = n; k < c.capacity; ++k) o[k] = 0.0;
+  atomicMax(count + ui, n);
+}
+
+// ---------------------------------------------------------------------------
+// Refinement (GetRefinedF0 harvest.cpp:589-617, GetMeanF0 :541-584, FixF0
+// :507-536).  One thread per (frame, slot); slot s < 7 count holds the
+// overlapped candidate of OverlapF0Candidates (harvest.cpp:417-429).  The
+// Blackman and derivative windows follow GetMainWindow/GetDiffWindow
+// (:446-468); the two spectra are evaluated only at the <= 6 harmonic bins
+// FixF0 reads, as DFT sums with exact integer phase indices.
+template <bool kTable>
+__global__ __launch_bounds__(kThreads) void k_refine(
+    Config c, const Utt* utt, const long long* f_off, long long total_frames,
+    const double* y, const double* official, const int* count,
+    const double* twiddle, double* rf0, double* rscore) {
+  const long long id = (long long)blockIdx.x * kThreads + threadIdx.x;
+  const long long gf = id / c.slots;
+  const int s = (int)(id % c.slots);
+  if (gf >= total_frames) return;
+  const int ui = find_segment(f_off, c.count, gf);
+  const int cnt = count[ui];
+  if (s >= 7 * cnt) return;
+  const Utt u = utt[ui];
+  const long long i = gf - u.f_off;
+  const int grp = s / cnt, jj = s % cnt;
+  long long src = i;
+  if (grp >= 1 && grp <= 3) src = i - grp;
+  else if (grp >= 4) src = i + (grp - 3);
+  const double f0 = (src >= 0 && src < u.frames)
+                        ? official[(u.f_off + src) * c.capacity + jj]
+                        : 0.0;
+  double refined = 0.0, score = 0.0;
+  if (f0 > 0.0) {
+    const double fs = c.fs_work;
+    const double t = (double)i / 1000.0;
+    const int hw = (int)(1.5 * fs / f0 + 1.0);
+    const double wl = (2.0 * hw + 1.0) / fs;
+    const int len = 2 * hw + 1;
+    const int n = 1 << (2 + (31 - __clz(len)));
+    const int basic = mround((t + (-hw) / fs) * fs + 0.001);
+    const int H = min((int)(fs / 2.0 / f0), 6);
+    int kb[6], ph[6];
+    double mre[6], mim[6], dre[6], dim[6];
+#pragma unroll
+    for (int hh = 0; hh < 6; ++hh) {
+      kb[hh] = hh < H ? mround(f0 * n / fs * (hh + 1)) : 0;
+      ph[hh] = 0;
+      mre[hh] = mim[hh] = dre[hh] = dim[hh] = 0.0;
+    }
+    const int stride = kTable ? c.twiddle_n / n : 0;
+    const double* yu = y + u.y_off;
+    auto window = [&](int m) -> double {
+      const double tmp = (basic + m - 1.0) / fs - t;
+      return 0.42 + 0.5 * cos(2.0 * kPi * tmp / wl) + 0.08 * cos(4.0 * kPi * tmp / wl);
+    };
+    double wprev = 0.0, wcur = window(0);
+    for (int m = 0; m < len; ++m) {
+      const double wnext = (m + 1 < len) ? window(m + 1) : 0.0;
+      double d;
+      if (m == 0) d = -wnext / 2.0;
+      else if (m == len - 1) d = wprev / 2.0;
+      else d = -(wnext - wprev) / 2.0;
+      int idx = basic + m - 1;
+      idx = idx < 0 ? 0 : (idx > u.n - 1 ? u.n - 1 : idx);
+      const double xv = yu[idx];
+      const double xm = xv * wcur, xd = xv * d;
+#pragma unroll
+      for (int hh = 0; hh < 6; ++hh) {
+        if (hh < H) {
+          double cs, sn;
+          if (kTable) {
+            const double* tw = twiddle + 2LL * ph[hh] * stride;
+            cs = tw[0];
+            sn = tw[1];
+          } else {
+            sincospi(2.0 * (double)ph[hh] / n, &sn, &cs);
+          }
+          mre[hh] = fma(xm, cs, mre[hh]);
+          mim[hh] = fma(-xm, sn, mim[hh]);
+          dre[hh] = fma(xd, cs, dre[hh]);
+          dim[hh] = fma(-xd, sn, dim[hh]);
+          ph[hh] = (ph[hh] + kb[hh]) & (n - 1);
+        }
+      }
+      wprev = wcur;
+      wcur = wnext;
+    }
+    double numer = 0.0, den = 0.0, dev = 0.0;
+    for (int hh = 0; hh < H; ++hh) {
+      const double pw = mre[hh] * mre[hh] + mim[hh] * mim[hh];
+      const double nu = mre[hh] * dim[hh] - mim[hh] * dre[hh];
+      const double ifr =
+          pw == 0.0 ? 0.0 : (double)kb[hh] * fs / n + nu / pw * fs / 2.0 / kPi;
+      const double amp = sqrt(pw);
+      numer += amp * ifr;
+      den += amp * (hh + 1.0);
+      dev += fabs((ifr / (hh + 1.0) - f0) / f0);
+    }
+    refined = numer / (den + kGuard);
+    score = 1.0 / (dev / H + kGuard);
+    if (refined < c.f0_floor || refined > c.f0_ceil || score < 2.5) {
+      refined = 0.0;
+      score = 0.0;
+    }
+  }
+  rf0[gf * c.slots + s] = refined;
+  rscore[gf * c.slots + s] = score;
+}
+
+// RemoveUnreliableCandidates (harvest.cpp:652-688) against the pre-prune
+// values of frames i +- 1.  min_k fl(|r - c_k| / r) = fl(min_k |r - c_k| / r)
+// because rounding is monotone, so one division per neighbour frame suffices.
+__global__ __launch_bounds__(kThreads) void k_prune(
+    Config c, const Utt* utt, const long long* f_off, long long total_frames,
+    const int* count, const double* rf0, const double* rscore, double* pf0,
+    double* pscore) {
+  const long long id = (long long)blockIdx.x * kThreads + threadIdx.x;
+  const long long gf = id / c.slots;
+  const int s = (int)(id % c.slots);
+  if (gf >= total_frames) return;
+  const int ui = find_segment(f_off, c.count, gf);
+  const int ncand = 7 * count[ui];
+  if (s >= ncand) return;
+  const Utt u = utt[ui];
+  const long long i = gf - u.f_off;
+  double r = rf0[gf * c.slots + s], sc = rscore[gf * c.slots + s];
+  if (r != 0.0 && i >= 1 && i <= u.frames - 2) {
+    double m1 = INFINITY, m2 = INFINITY;
+    const double* nxt = rf0 + (gf + 1) * c.slots;
+    const double* prv = rf0 + (gf - 1) * c.slots;
+    for (int k = 0; k < ncand; ++k) {
+      m1 = fmin(m1, fabs(r - nxt[k]));
+      m2 = fmin(m2, fabs(r - prv[k]));
+    }
+    double e1 = m1 / r, e2 = m2 / r;
+    if (e1 > 1.0) e1 = 1.0;
+    if (e2 > 1.0) e2 = 1.0;
+    const double e = e1 < e2 ? e1 : e2;
+    if (!(e <= 0.05)) {
+      r = 0.0;
+      sc = 0.0;
+    }
+  }
+  pf0[gf * c.slots + s] = r;
+  pscore[gf * c.slots + s] = sc;
+}
+
+// SearchF0Base (harvest.cpp:693-705) and the nonzero count of each frame.
+__global__ __launch_bounds__(kThreads) void k_best(Config c, const long long* f_off,
+                                                  long long total_frames,
+                                                  const int* count, const double* pf0,
+                                                  const double* pscore, double* best,
+                                                  int* row_nnz) {
+  const long long gf = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (gf >= total_frames) return;
+  const int ncand = 7 * count[find_segment(f_off, c.count, gf)];
+  double b = 0.0, bs = 0.0;
+  int nz = 0;
+  for (int k = 0; k < ncand; ++k) {
+    const double sc = pscore[gf * c.slots + k], f = pf0[gf * c.slots + k];
+    if (sc > bs) {
+      b = f;
+      bs = sc;
+    }
+    nz += f != 0.0;
+  }
+  best[gf] = b;
+  row_nnz[gf] = nz;
+}
+
+__global__ __launch_bounds__(kThreads) void k_compact(Config c, const long long* f_off,
+                                                     long long total_frames,
+                                                     const int* count, const double* pf0,
+                                                     const double* pscore,
+                                                     const int* offsets, double* of0,
+                                                     double* oscore) {
+  const long long gf = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (gf >= total_frames) return;
+  const int ncand = 7 * count[find_segment(f_off, c.count, gf)];
+  int pos = offsets[gf];
+  for (int k = 0; k < ncand; ++k) {
+    const double f = pf0[gf * c.slots + k];
+    if (f != 0.0) {
+      of0[pos] = f;
+      oscore[pos] = pscore[gf * c.slots + k];
+      ++pos;
+    }
+  }
+}
+
+// Generic exclusive scan of ints: per-block sums, one-workgroup scan of the
+// sums, then per-block local scans.  kScanBlock elements per workgroup.
+constexpr int kScanBlock = kThreads * 16;
+
+__global__ __launch_bounds__(kThreads) void k_scan_reduce(const int* in, long long n,
+                                                         int* block_sums) {
+  __shared__ unsigned long long wave_sums[kThreads / 64];
+  long long start = (long long)blockIdx.x * kScanBlock + threadIdx.x * 16;
+  unsigned long long s = 0;
+  for (int k = 0; k < 16; ++k)
+    if (start + k < n) s += (unsigned)in[start + k];
+  unsigned long long total;
+  block_scan_u64(s, wave_sums, total);
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = (int)total;
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_sums(int* block_sums, long long nb) {
+  __shared__ unsigned long long wave_sums[kThreads / 64];
+  const long long K = (nb + kThreads - 1) / kThreads;
+  const long long first = threadIdx.x * K, last = min(first + K, nb);
+  unsigned long long s = 0;
+  for (long long k = first; k < last; ++k) s += (unsigned)block_sums[k];
+  unsigned long long total;
+  unsigned long long run = block_scan_u64(s, wave_sums, total);
+  for (long long k = first; k < last; ++k) {
+    unsigned long long v = (unsigned)block_sums[k];
+    block_sums[k] = (int)run;
+    run += v;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_local(const int* in, int* out,
+                                                        long long n,
+                                                        const int* block_offs) {
+  __shared__ unsigned long long wave_sums[kThreads / 64];
+  long long start = (long long)blockIdx.x * kScanBlock + threadIdx.x * 16;
+  unsigned long long s = 0;
+  for (int k = 0; k < 16; ++k)
+    if (start + k < n) s += (unsigned)in[start + k];
+  unsigned long long total;
+  unsigned long long run = block_scan_u64(s, wave_sums, total) + (unsigned)block_offs[blockIdx.x];
+  for (int k = 0; k < 16; ++k) {
+    if (start + k < n) {
+      out[start + k] = (int)run;
+      run += (unsigned)in[start + k];
+    }
+  }
+  if (start < n && start + 16 >= n) out[n] = (int)run;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ launchers
+void launch_decimate(hipStream_t s, const Config& c, const Utt* utt,
+                     const long long* c_off, long long total_chunks,
+                     const double* x, double* pass_buffer, double* states,
+                     double* carries, double* y) {
+  const unsigned grid = grid_for(total_chunks);
+  k_iir_chunk_state<0><<<grid, kThreads, 0, s>>>(c, utt, c_off, total_chunks, x,
+                                                  pass_buffer, states);
+  k_iir_carry<<<c.count, kThreads, 0, s>>>(c, utt, states, carries);
+  k_iir_replay<0><<<grid, kThreads, 0, s>>>(c, utt, c_off, total_chunks, x,
+                                             pass_buffer, carries, y);
+  k_iir_chunk_state<1><<<grid, kThreads, 0, s>>>(c, utt, c_off, total_chunks, x,
+                                                  pass_buffer, states);
+  k_iir_carry<<<c.count, kThreads, 0, s>>>(c, utt, states, carries);
+  k_iir_replay<1><<<grid, kThreads, 0, s>>>(c, utt, c_off, total_chunks, x,
+                                             pass_buffer, carries, y);
+  check_launch("decimate");
+}
+
+void launch_copy_input(hipStream_t s, const Config& c, const Utt* utt,
+                       const long long* y_off, long long total, const double* x,
+                       double* y) {
+  k_copy_input<<<grid_for(total), kThreads, 0, s>>>(c, utt, y_off, total, x, y);
+  check_launch("copy_input");
+}
+
+void launch_remove_mean(hipStream_t s, const Config& c, const Utt* utt,
+                        const long long* p_off, long long total_blocks,
+                        const long long* y_off, long long total_y, double* y,
+                        double* partial, double* mean) {
+  k_partial_sum<<<(unsigned)total_blocks, kThreads, 0, s>>>(c, utt, p_off, y, partial);
+  k_mean_final<<<c.count, kThreads, 0, s>>>(c, utt, partial, mean);
+  k_subtract<<<grid_for(total_y), kThreads, 0, s>>>(c, y_off, total_y, y, mean);
+  check_launch("remove_mean");
+}
+
+void launch_quirk(hipStream_t s, const Config& c, const Utt* utt,
+                  const long long* p_off, long long total_blocks, const double* y,
+                  const double* coef, const int* coef_off, const int* half,
+                  double* partial3, double* sums3, Quirk* quirk) {
+  k_quirk_partial<<<(unsigned)total_blocks, kThreads, 0, s>>>(c, utt, p_off, y, partial3);
+  k_quirk_final<<<c.count, kThreads, 0, s>>>(c, utt, partial3, sums3);
+  k_quirk_bands<<<grid_for((long long)c.count * c.bands), kThreads, 0, s>>>(
+      c, utt, sums3, coef, coef_off, half, quirk);
+  check_launch("quirk");
+}
+
+void launch_fill_int(hipStream_t s, int* p, long long n, int value) {
+  if (n <= 0) return;
+  k_fill_int<<<grid_for(n), kThreads, 0, s>>>(p, n, value);
+  check_launch("fill_int");
+}
+
+void launch_filter_events(hipStream_t s, const Config& c, const Utt* utt,
+                          const long long* t_off, long long total_tiles,
+                          int band0, int group, int max_half, const double* y,
+                          const double* coef, const int* coef_off,
+                          const int* half, const Quirk* quirk, double* slab,
+                          int* counts) {
+  const int taps8 = (2 * max_half + 1 + 7) & ~7;
+  const size_t lds_bytes = sizeof(double) * (size_t)(kTile + taps8 + 10);
+  dim3 grid((unsigned)total_tiles, (unsigned)group);
+  if (lds_bytes <= 64 * 1024) {
+    k_filter_events<true><<<grid, kThreads, lds_bytes, s>>>(
+        c, utt, t_off, total_tiles, band0, y, coef, coef_off, half, quirk, slab, counts);
+  } else {
+    k_filter_events<false><<<grid, kThreads, sizeof(double) * (kTile + 2), s>>>(
+        c, utt, t_off, total_tiles, band0, y, coef, coef_off, half, quirk, slab, counts);
+  }
+  check_launch("filter_events");
+}
+
+void launch_tile_scan(hipStream_t s, const Config& c, const Utt* utt,
+                      long long total_tiles, int group, const int* counts,
+                      int* offs, int* totals) {
+  dim3 grid((unsigned)c.count, (unsigned)(group * 4));
+  k_tile_scan<<<grid, kThreads, 0, s>>>(c, utt, total_tiles, counts, offs, totals);
+  check_launch("tile_scan");
+}
+
+void launch_interp(hipStream_t s, const Config& c, const Utt* utt,
+                   const long long* t_off, long long total_tiles,
+                   long long total_frames, int band0, int group,
+                   const double* boundaries, const double* slab,
+                   const int* counts, const int* offs, const int* totals,
+                   double* raw) {
+  dim3 grid((unsigned)total_tiles, (unsigned)group);
+  k_interp<<<grid, kThreads, 0, s>>>(c, utt, t_off, total_tiles, total_frames, band0,
+                                     boundaries, slab, counts, offs, totals, raw);
+  check_launch("interp");
+}
+
+void launch_official(hipStream_t s, const Config& c, const long long* f_off,
+                     long long total_frames, const double* raw, double* official,
+                     int* count) {
+  k_official<<<grid_for(total_frames), kThreads, 0, s>>>(c, f_off, total_frames, raw,
+                                                        official, count);
+  check_launch("official");
+}
+
+void launch_refine(hipStream_t s, const Config& c, const Utt* utt,
+                   const long long* f_off, long long total_frames,
+                   const double* y, const double* official, const int* count,
+                   const double* twiddle, double* rf0, double* rscore) {
+  const long long n = total_frames * c.slots;
+  if (c.twiddle_n > 0)
+    k_refine<true><<<grid_for(n), kThreads, 0, s>>>(c, utt, f_off, total_frames, y,
+                                                     official, count, twiddle, rf0, rscore);
+  else
+    k_refine<false><<<grid_for(n), kThreads, 0, s>>>(c, utt, f_off, total_frames, y,
+                                                      official, count, twiddle, rf0, rscore);
+  check_launch("refine");
+}
+
+void launch_prune_best(hipStream_t s, const Config& c, const Utt* utt,
+                       const long long* f_off, long long total_frames,
+                       const int* count, const double* rf0,
+                       const double* rscore, double* pf0, double* pscore,
+                       double* best, int* row_nnz) {
+  k_prune<<<grid_for(total_frames * c.slots), kThreads, 0, s>>>(
+      c, utt, f_off, total_frames, count, rf0, rscore, pf0, pscore);
+  k_best<<<grid_for(total_frames), kThreads, 0, s>>>(c, f_off, total_frames, count, pf0,
+                                                    pscore, best, row_nnz);
+  check_launch("prune_best");
+}
+
+long long scan_block_count(long long n) { return (n + kScanBlock - 1) / kScanBlock; }
+
+void launch_exclusive_scan(hipStream_t s, const int* in, int* out, long long n,
+                           int* block_tmp, long long block_tmp_len) {
+  const long long nb = scan_block_count(n);
+  if (nb > block_tmp_len) throw std::runtime_error("scan scratch too small");
+  if (n == 0) {
+    k_fill_int<<<1, kThreads, 0, s>>>(out, 1, 0);
+    check_launch("scan");
+    return;
+  }
+  k_scan_reduce<<<(unsigned)nb, kThreads, 0, s>>>(in, n, block_tmp);
+  k_scan_sums<<<1, kThreads, 0, s>>>(block_tmp, nb);
+  k_scan_local<<<(unsigned)nb, kThreads, 0, s>>>(in, out, n, block_tmp);
+  check_launch("scan");
+}
+
+void launch_compact(hipStream_t s, const Config& c, const Utt* utt,
+                    const long long* f_off, long long total_frames,
+                    const int* count, const double* pf0, const double* pscore,
+                    const int* offsets, double* out_f0, double* out_score) {
+  (void)utt;
+  k_compact<<<grid_for(total_frames), kThreads, 0, s>>>(c, f_off, total_frames, count, pf0,
+                                                       pscore, offsets, out_f0, out_score);
+  check_launch("compact");
+}
+
+}  // namespace hv

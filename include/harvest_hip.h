@@ -1,0 +1,110 @@
+/*
+ * harvest_hip.h - C ABI of the MI355X (gfx950) Harvest backend.
+ *
+ * The backend replaces the reference's C++ GPU boundary (src/engine.h:20-34,
+ * implemented in src/cuda_backend.cu).  Every entry point:
+ *   - takes plain pointers and sizes (no C++ or torch types);
+ *   - catches everything internally and returns an hv_status; the message of
+ *     the last failure on the calling thread is available from hv_last_error();
+ *   - saves and restores the caller's current HIP device (the HIP counterpart
+ *     of CudaContextGuard, src/cuda_device.h:92-108);
+ *   - borrows host pointers only for the duration of the call.  Device memory
+ *     belongs to the workspace and is reused (grow-only) across calls.
+ * A workspace must be used by one thread at a time; the host-side
+ * GpuWorkspaceCache lease (workspace_cache.h) guarantees this.
+ */
+#ifndef HARVEST_HIP_H_
+#define HARVEST_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hv_workspace hv_workspace;
+
+typedef enum hv_status {
+  HV_OK = 0,
+  HV_EINVAL = 1,       /* invalid argument: maps to ValueError             */
+  HV_ENODEV = 2,       /* no usable GPU: maps to RuntimeError              */
+  HV_EUNSUPPORTED = 3, /* GPU present but not gfx950                       */
+  HV_ENOMEM = 4,       /* device or pinned host allocation failed          */
+  HV_EDEVICE = 5,      /* HIP runtime error                                */
+  HV_EINTERNAL = 6     /* anything else                                    */
+} hv_status;
+
+/* Number of stage timings written by hv_analyze_batch when `timings_ms` is
+ * not NULL: [0] preprocess (upload, decimation, mean), [1] filter_events
+ * (filterbank, crossings, raw and official candidates), [2] refine
+ * (refinement, pruning, compaction, download).  Device spans (hipEvents). */
+#define HV_TIMING_SLOTS 3
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* hv_last_error(void);
+
+/* 1 if `device` is a visible gfx950 GPU, else 0.  Never fails and leaves no
+ * sticky error.  Replaces cuda_available (src/engine.h:28,
+ * src/cuda_backend.cu:794-803). */
+int hv_device_supported(int device);
+
+/* Create / destroy the per-device workspace.  Replaces create_gpu_workspace
+ * and delete_gpu_workspace (src/engine.h:21-22, src/cuda_backend.cu:804-817).
+ * Errors: device < 0 -> HV_EINVAL ("nonnegative"); device out of range ->
+ * HV_EINVAL ("out of range"); no GPU -> HV_ENODEV; not gfx950 ->
+ * HV_EUNSUPPORTED (src/cuda_device.h:26-40). */
+hv_status hv_workspace_create(int device, hv_workspace** out);
+void hv_workspace_destroy(hv_workspace* ws);
+int hv_workspace_device(const hv_workspace* ws);
+
+/* WORLD's zero-phase Chebyshev decimation of x[length] by `ratio` (1..12)
+ * into y[y_length], y_length = ceil(length / ratio), before mean removal.
+ * Computed as an exact chunked linear-recurrence scan.  Replaces
+ * gpu_decimate (src/engine.h:29, src/cuda_backend.cu:818-838). */
+hv_status hv_decimate(hv_workspace* ws, const double* x, int64_t length,
+                      int ratio, double* y, int64_t y_length);
+
+/* Analysis of one decimated, mean-removed waveform y[y_length] at the working
+ * rate fs_work: filterbank, crossings, raw/official candidates, refinement,
+ * pruning and compaction.  Replaces gpu_analyze (src/engine.h:30-34,
+ * src/cuda_backend.cu:1129-1147).  Outputs (host):
+ *   offsets[frames + 1]  CSR row offsets of nonzero refined candidates,
+ *   best[frames]         f0 of the best-scoring candidate per frame,
+ *   *nnz                 offsets[frames].
+ * The candidates themselves are fetched by hv_fetch_candidates.  fft_size,
+ * bands, boundaries and capacity follow WORLD's HarvestGeneralBody
+ * (harvest.cpp:1149-1181); frames counts 1 ms frames. */
+hv_status hv_analyze(hv_workspace* ws, const double* y, int64_t y_length,
+                     int fft_size, double fs_work, const double* boundaries,
+                     int bands, int frames, double f0_floor, double f0_ceil,
+                     int capacity, int32_t* offsets, double* best,
+                     int64_t* nnz, double* timings_ms);
+
+/* Fused form used by the host engine: `count` raw waveforms at sample rate
+ * fs go through decimation, mean removal and analysis without returning to
+ * the host in between.  Utterance u has lengths[u] samples at xs[u]; its
+ * frames[u] = int(1000 lengths[u] / fs) + 1 one-millisecond frames occupy rows
+ * [frame_offset[u], frame_offset[u] + frames[u]) of the concatenated outputs,
+ * with frame_offset the exclusive prefix sum of frames.  Outputs:
+ *   offsets[total_frames + 1]  global CSR offsets (offsets[0] = 0),
+ *   best[total_frames],
+ *   *nnz = offsets[total_frames].
+ * Every utterance's results are bitwise identical to a call with count = 1.
+ * This is the batched form (no reference counterpart) and, at count = 1,
+ * the plain form; it fuses gpu_decimate, the host mean removal
+ * (src/engine.cpp:278-281) and gpu_analyze. */
+hv_status hv_analyze_batch(hv_workspace* ws, int count,
+                           const double* const* xs, const int64_t* lengths,
+                           int fs, double f0_floor, double f0_ceil,
+                           int32_t* offsets, double* best, int64_t* nnz,
+                           double* timings_ms);
+
+/* Copy the nnz candidates of the last hv_analyze / hv_analyze_batch call:
+ * f0[nnz] and scores[nnz] in CSR order (frame order, then slot order). */
+hv_status hv_fetch_candidates(hv_workspace* ws, double* f0, double* scores);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HARVEST_HIP_H_ */
