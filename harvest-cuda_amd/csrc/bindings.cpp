@@ -1,0 +1,248 @@
+// Python module harvest_cuda._core (counterpart of the reference's nanobind
+// module, src/bindings.cpp:1-105).  Same surface and argument semantics:
+// x must be a 1-D C-contiguous float64 NumPy array (no conversion, read-only
+// accepted), device is keyword-only, invalid arguments raise ValueError,
+// missing GPUs RuntimeError, and the GIL is released for the whole analysis.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <memory>
+#include <optional>
+#include <stdexcept>
+#include <thread>
+#include <vector>
+
+#include "engine.h"
+#include "workspace_cache.h"
+
+namespace py = pybind11;
+using namespace harvest;
+
+namespace {
+
+using Input = py::array_t<double, py::array::c_style>;
+using Cache = WorkspaceCache<hv_workspace>;
+
+Cache& cache() {
+  // Owned by the module for the life of the process; cleared explicitly.
+  static Cache* instance = new Cache(
+      [](int device) {
+        hv_workspace* ws = nullptr;
+        throw_status(hv_workspace_create(device, &ws));
+        return ws;
+      },
+      [](hv_workspace* ws) { hv_workspace_destroy(ws); });
+  return *instance;
+}
+
+const double* data_1d(const Input& x) {
+  if (x.ndim() != 1) throw py::type_error("x must be a 1-D float64 array");
+  return x.data();
+}
+
+py::array_t<double> to_numpy(std::vector<double>&& v) {
+  auto* owner = new std::vector<double>(std::move(v));
+  py::capsule capsule(owner, [](void* p) { delete static_cast<std::vector<double>*>(p); });
+  return py::array_t<double>({(py::ssize_t)owner->size()}, {(py::ssize_t)sizeof(double)},
+                             owner->data(), capsule);
+}
+
+int host_threads() {
+  unsigned n = std::thread::hardware_concurrency();
+  return (int)std::max(1u, std::min(n ? n : 1u, 16u));
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_core, m) {
+  m.doc() = "MI355X (gfx950) Harvest F0 estimator backend";
+  HostCache::instance().initialize();
+
+  m.def("clear_cache", [] {
+    py::gil_scoped_release release;
+    cache().clear();
+    HostCache::instance().clear();
+  }, "Release cached host and GPU working memory across threads and devices. "
+     "Workspaces in use are released when their calls finish.");
+
+  m.def("cuda_available", [](int device) { return hv_device_supported(device) != 0; },
+        py::arg("device") = 0,
+        "Whether the selected visible GPU is a supported gfx950 device. "
+        "Name kept for drop-in compatibility.");
+
+  m.def("harvest", [](Input x, int fs, double floor, double ceil, double period, int device) {
+    const double* p = data_1d(x);
+    const int64_t n = (int64_t)x.size();
+    validate(p, n, fs, floor, ceil, period);
+    Result r;
+    {
+      py::gil_scoped_release release;
+      Cache::Lease ws(cache(), device);
+      r = analyze(ws.get(), p, n, fs, floor, ceil, period);
+    }
+    return py::make_tuple(to_numpy(std::move(r.f0)), to_numpy(std::move(r.times)));
+  }, py::arg("x").noconvert(), py::arg("fs"), py::arg("f0_floor") = 71.0,
+     py::arg("f0_ceil") = 800.0, py::arg("frame_period") = 5.0, py::kw_only(),
+     py::arg("device") = 0,
+     "Estimate F0 on the GPU. x: contiguous 1-D float64 NumPy array; returns (f0, times).");
+
+  m.def("harvest_batch", [](py::list xs, int fs, double floor, double ceil, double period,
+                            int device, bool timings) {
+    Timings t;
+    std::vector<Input> keep;
+    std::vector<const double*> ptrs;
+    std::vector<int64_t> lengths;
+    for (py::handle h : xs) {
+      if (!Input::check_(h)) throw py::type_error("every utterance must be a 1-D C-contiguous float64 array");
+      Input a = py::reinterpret_borrow<Input>(h);
+      const double* p = data_1d(a);
+      validate(p, (int64_t)a.size(), fs, floor, ceil, period);
+      keep.push_back(a);
+      ptrs.push_back(p);
+      lengths.push_back((int64_t)a.size());
+    }
+    if (ptrs.empty()) return py::object(py::list());
+    std::vector<Result> rs;
+    {
+      py::gil_scoped_release release;
+      Cache::Lease ws(cache(), device);
+      rs = analyze_batch(ws.get(), ptrs, lengths, fs, floor, ceil, period, host_threads(),
+                         timings ? &t : nullptr);
+    }
+    py::list out;
+    for (auto& r : rs) out.append(py::make_tuple(to_numpy(std::move(r.f0)), to_numpy(std::move(r.times))));
+    if (timings) return py::object(py::make_tuple(out, t));
+    return py::object(out);
+  }, py::arg("xs"), py::arg("fs"), py::arg("f0_floor") = 71.0, py::arg("f0_ceil") = 800.0,
+     py::arg("frame_period") = 5.0, py::kw_only(), py::arg("device") = 0,
+     py::arg("timings") = false,
+     "Estimate F0 of many utterances in one batched GPU launch sequence.");
+
+  m.def("harvest_device", [](uintptr_t x_device, std::vector<int64_t> lengths, int fs,
+                             double floor, double ceil, double period, int device,
+                             bool timings) {
+    // Waveforms already resident in HBM (e.g. a torch float64 tensor's
+    // data_ptr()); validation of sample values is the caller's business.
+    for (int64_t n : lengths) {
+      double probe = 0.0;
+      validate(&probe, 1, fs, floor, ceil, period);
+      if (n <= 0) throw std::invalid_argument("lengths must be positive");
+    }
+    std::vector<Result> rs;
+    Timings t;
+    {
+      py::gil_scoped_release release;
+      Cache::Lease ws(cache(), device);
+      rs = analyze_batch_device(ws.get(), reinterpret_cast<const double*>(x_device), lengths, fs,
+                                floor, ceil, period, host_threads(), timings ? &t : nullptr);
+    }
+    py::list out;
+    for (auto& r : rs) out.append(py::make_tuple(to_numpy(std::move(r.f0)), to_numpy(std::move(r.times))));
+    if (timings) return py::object(py::make_tuple(out, t));
+    return py::object(out);
+  }, py::arg("x_device"), py::arg("lengths"), py::arg("fs"), py::arg("f0_floor") = 71.0,
+     py::arg("f0_ceil") = 800.0, py::arg("frame_period") = 5.0, py::kw_only(),
+     py::arg("device") = 0, py::arg("timings") = false,
+     "Batched analysis of waveforms already in device memory, stored back to back.");
+
+  m.def("_profile", [](Input x, int fs, double floor, double ceil, double period, bool gpu,
+                       int device) {
+    if (!gpu) throw std::invalid_argument("this build profiles the GPU path only (gpu=True)");
+    const double* p = data_1d(x);
+    const int64_t n = (int64_t)x.size();
+    validate(p, n, fs, floor, ceil, period);
+    Result r;
+    Timings t;
+    {
+      py::gil_scoped_release release;
+      Cache::Lease ws(cache(), device);
+      r = analyze(ws.get(), p, n, fs, floor, ceil, period, &t);
+    }
+    return py::make_tuple(to_numpy(std::move(r.f0)), to_numpy(std::move(r.times)), t);
+  }, py::arg("x").noconvert(), py::arg("fs"), py::arg("f0_floor") = 71.0,
+     py::arg("f0_ceil") = 800.0, py::arg("frame_period") = 5.0, py::arg("gpu") = true,
+     py::kw_only(), py::arg("device") = 0);
+
+  // ---- stage access for parity tests (device results, no host contour) ----
+  m.def("_decimate", [](Input x, int fs, int device) {
+    const double* p = data_1d(x);
+    const int64_t n = (int64_t)x.size();
+    validate(p, n, fs, 71.0, 800.0 < fs / 2.0 ? 800.0 : fs / 4.0, 5.0);
+    const Params prm = derive_params(n, fs, 71.0, 800.0, 5.0);
+    std::vector<double> y(prm.y_length);
+    {
+      py::gil_scoped_release release;
+      Cache::Lease ws(cache(), device);
+      throw_status(hv_decimate(ws.get(), p, n, prm.ratio, y.data(), prm.y_length));
+    }
+    return to_numpy(std::move(y));
+  }, py::arg("x").noconvert(), py::arg("fs"), py::kw_only(), py::arg("device") = 0,
+     "Device decimation before mean removal (WORLD decimate()).");
+
+  m.def("_candidates", [](py::list xs, int fs, double floor, double ceil, int device) {
+    std::vector<Input> keep;
+    std::vector<const double*> ptrs;
+    std::vector<int64_t> lengths;
+    int64_t frames = 0;
+    for (py::handle h : xs) {
+      Input a = py::reinterpret_borrow<Input>(h);
+      ptrs.push_back(data_1d(a));
+      lengths.push_back((int64_t)a.size());
+      validate(ptrs.back(), lengths.back(), fs, floor, ceil, 5.0);
+      frames += derive_params(lengths.back(), fs, floor, ceil, 5.0).frames;
+      keep.push_back(a);
+    }
+    std::vector<int32_t> offsets(frames + 1);
+    std::vector<double> best(frames), f0, scores;
+    int64_t nnz = 0;
+    {
+      py::gil_scoped_release release;
+      Cache::Lease ws(cache(), device);
+      throw_status(hv_analyze_batch(ws.get(), (int)ptrs.size(), ptrs.data(), lengths.data(), fs,
+                                    floor, ceil, offsets.data(), best.data(), &nnz, nullptr));
+      f0.resize(std::max<int64_t>(nnz, 1));
+      scores.resize(std::max<int64_t>(nnz, 1));
+      throw_status(hv_fetch_candidates(ws.get(), f0.data(), scores.data()));
+      f0.resize(nnz);
+      scores.resize(nnz);
+    }
+    py::array_t<int32_t> off(offsets.size());
+    std::copy(offsets.begin(), offsets.end(), off.mutable_data());
+    return py::make_tuple(off, to_numpy(std::move(f0)), to_numpy(std::move(scores)),
+                          to_numpy(std::move(best)));
+  }, py::arg("xs"), py::arg("fs"), py::arg("f0_floor") = 71.0, py::arg("f0_ceil") = 800.0,
+     py::kw_only(), py::arg("device") = 0,
+     "Device CSR of pruned refined candidates and the best f0 per 1 ms frame.");
+
+  // ---- host contour stages (CPU only) for bit-exactness tests ----
+  m.def("_fix_contour", [](py::array_t<int32_t, py::array::c_style> offsets, Input f0,
+                           Input scores, Input base) {
+    const int frames = (int)base.size();
+    if ((int)offsets.size() != frames + 1) throw std::invalid_argument("offsets must have frames + 1 entries");
+    std::vector<double> out(frames);
+    CandidateRows rows{offsets.data(), f0.data(), scores.data()};
+    fix_contour(rows, frames, base.data(), out.data());
+    return to_numpy(std::move(out));
+  });
+  m.def("_smooth", [](Input f0) {
+    std::vector<double> out(f0.size());
+    smooth_contour(f0.data(), (int)f0.size(), out.data());
+    return to_numpy(std::move(out));
+  });
+  m.def("_params", [](int64_t length, int fs, double floor, double ceil, double period) {
+    Params p = derive_params(length, fs, floor, ceil, period);
+    py::dict d;
+    d["bands"] = p.bands;
+    d["ratio"] = p.ratio;
+    d["y_length"] = p.y_length;
+    d["fft_size"] = p.fft_size;
+    d["frames"] = p.frames;
+    d["out_frames"] = p.out_frames;
+    d["capacity"] = p.capacity;
+    d["fs_work"] = p.fs_work;
+    return d;
+  });
+  m.def("_host_cache_allocated", [] { return HostCache::instance().allocated(); });
+  m.def("_cached_workspaces", [] { return cache().size(); });
+}

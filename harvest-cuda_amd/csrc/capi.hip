@@ -1,0 +1,778 @@
+// Workspace, planning and C ABI of the gfx950 Harvest backend
+// (include/harvest_hip.h).  Replaces the reference's GpuWorkspace and its
+// gpu_* entry points (src/cuda_backend.cu:92-135, 794-838, 1129-1147) with a
+// batch-native device pipeline: one launch sequence serves any number of
+// utterances, and a plain call is a batch of one.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/harvest_hip.h"
+#include "kernels.h"
+
+namespace hv {
+namespace {
+
+thread_local std::string g_error;
+
+struct Failure : std::runtime_error {
+  hv_status status;
+  Failure(hv_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+void check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return;
+  (void)hipGetLastError();
+  hv_status st = HV_EDEVICE;
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) st = HV_ENOMEM;
+  if (e == hipErrorNoDevice || e == hipErrorInsufficientDriver) st = HV_ENODEV;
+  std::string m = std::string("HIP: ") + hipGetErrorName(e) + ": " + hipGetErrorString(e) +
+                  " (" + what + ")";
+  if (st == HV_ENOMEM)
+    m += ". GPU memory allocation failed; try shorter audio or release unused GPU allocations.";
+  else if (st == HV_ENODEV)
+    m += ". Check the ROCm driver and HIP_VISIBLE_DEVICES before starting Python.";
+  throw Failure(st, m);
+}
+
+[[noreturn]] void invalid(const std::string& m) { throw Failure(HV_EINVAL, m); }
+
+// Saves the caller's current device and restores it on every exit path
+// (counterpart of CudaContextGuard, src/cuda_device.h:92-108).
+struct DeviceGuard {
+  int previous = -1;
+  bool switched = false;
+  explicit DeviceGuard(int device) {
+    check(hipGetDevice(&previous), "hipGetDevice");
+    if (previous != device) {
+      check(hipSetDevice(device), "hipSetDevice");
+      switched = true;
+    }
+  }
+  ~DeviceGuard() {
+    if (switched) (void)hipSetDevice(previous);
+  }
+};
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T> T* ensure(size_t n) {
+    size_t need = std::max<size_t>(n, 1) * sizeof(T);
+    if (need > bytes) {
+      release();
+      check(hipMalloc(&p, need), "hipMalloc");
+      bytes = need;
+    }
+    return static_cast<T*>(p);
+  }
+  template <class T> T* get() const { return static_cast<T*>(p); }
+};
+
+struct PinnedBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+  ~PinnedBuf() { release(); }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T> T* ensure(size_t n) {
+    size_t need = std::max<size_t>(n, 1) * sizeof(T);
+    if (need > bytes) {
+      release();
+      check(hipHostMalloc(&p, need, hipHostMallocDefault), "hipHostMalloc");
+      bytes = need;
+    }
+    return static_cast<T*>(p);
+  }
+  template <class T> T* get() const { return static_cast<T*>(p); }
+};
+
+// WORLD FilterForDecimate coefficients a0, a1, a2, b0, b1 for r = 2..12
+// (matlabfunctions.cpp:29-113).  The table must equal WORLD's bit for bit.
+const double kDecimation[13][5] = {
+    {0, 0, 0, 0, 0},
+    {0, 0, 0, 0, 0},
+    {0.041156734567757189, -0.42599112459189636, 0.041037215479961225,
+     0.16797464681802227, 0.50392394045406674},
+    {0.95039378983237421, -0.67429146741526791, 0.15412211621346475,
+     0.071221945171178636, 0.21366583551353591},
+    {1.4499664446880227, -0.98943497080950582, 0.24578252340690215,
+     0.036710750339322612, 0.11013225101796784},
+    {1.7610939654280557, -1.2554914843859768, 0.3237186507788215,
+     0.021334858522387423, 0.06400457556716227},
+    {1.9715352749512141, -1.4686795689225347, 0.3893908434965701,
+     0.013469181309343825, 0.040407543928031475},
+    {2.1225239019534703, -1.6395144861046302, 0.44469707800587366,
+     0.0090366882681608418, 0.027110064804482525},
+    {2.2357462340187593, -1.7780899984041358, 0.49152555365968692,
+     0.0063522763407111993, 0.019056829022133598},
+    {2.3236003491759578, -1.8921545617463598, 0.53148928133729068,
+     0.0046331164041389372, 0.013899349212416812},
+    {2.3936475118069387, -1.9873904075111861, 0.5658879979027055,
+     0.0034818622251927556, 0.010445586675578267},
+    {2.450743295230728, -2.06794904601978, 0.59574774438332101,
+     0.0026822508007163792, 0.0080467524021491377},
+    {2.4981398605924205, -2.1368928194784025, 0.62187513816221485,
+     0.0021097275904709001, 0.0063291827714127002},
+};
+
+constexpr size_t kSlabBudget = size_t(4) << 30;  // bytes of crossing slabs
+constexpr int kTwiddleMax = 1 << 20;
+
+int host_round(double x) { return x > 0 ? (int)(x + 0.5) : (int)(x - 0.5); }
+
+// GetSuitableFFTSize (common.cpp:51-54), evaluated with the host libm as WORLD.
+int suitable_fft_size(int sample) {
+  return (int)std::pow(2.0, (int)(std::log((double)sample) / kLog2) + 1.0);
+}
+
+// IIR state transition A^kChunk in extended precision.
+void chunk_power(const double* co, double* out) {
+  long double a[9] = {co[0], co[1], co[2], 1, 0, 0, 0, 1, 0};
+  long double r[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, t[9];
+  for (int step = 0; step < kChunk; ++step) {
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        t[i * 3 + j] = a[i * 3 + 0] * r[0 * 3 + j] + a[i * 3 + 1] * r[1 * 3 + j] +
+                       a[i * 3 + 2] * r[2 * 3 + j];
+    std::memcpy(r, t, sizeof(r));
+  }
+  for (int k = 0; k < 9; ++k) out[k] = (double)r[k];
+}
+
+}  // namespace
+}  // namespace hv
+
+using namespace hv;
+
+struct hv_workspace {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // Kernel timers: (slot, start, stop) event pairs recorded only when a call
+  // asks for timings.
+  std::vector<hipEvent_t> timer_events;
+  std::vector<int> timer_slots;
+  size_t timers_used = 0;
+  bool timing = false;
+  DevBuf x, pass, states, carries, y, partial, mean, partial3, sums3, quirk;
+  DevBuf slab, counts, offs, totals, raw, official, count;
+  DevBuf rf0, rsc, pf0, psc, best, row_nnz, offsets, scan_tmp;
+  DevBuf utt, c_off, t_off, f_off, p_off, y_off;
+  DevBuf coef, coef_off, half, bounds, twiddle;
+  PinnedBuf h_x, h_meta, h_offsets, h_best, h_cand;
+  // Filter-bank cache key (working rate, floor, ceil).
+  double key_fs = -1.0, key_floor = -1.0, key_ceil = -1.0;
+  std::vector<double> boundaries;
+  std::vector<int> halves;
+  int max_half = 0;
+  int twiddle_n = -1;
+  long long last_nnz = -1;
+
+  explicit hv_workspace(int dev) : device(dev) {
+    check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+    for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
+  }
+  ~hv_workspace() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& e : timer_events)
+      if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace hv {
+namespace {
+
+struct Plan {
+  Config cfg{};
+  std::vector<Utt> utts;
+  std::vector<long long> c_off, t_off, f_off, p_off, y_off;
+  long long total_x = 0, total_m = 0, total_y = 0, total_frames = 0;
+  long long total_tiles = 0, total_chunks = 0, total_pblocks = 0;
+};
+
+// Records `launch` between two events attributed to timing slot `slot`.
+template <class F> void timed(hv_workspace* ws, int slot, F&& launch) {
+  if (!ws->timing) {
+    launch();
+    return;
+  }
+  if (ws->timers_used + 2 > ws->timer_events.size()) {
+    for (int k = 0; k < 16; ++k) {
+      hipEvent_t e;
+      check(hipEventCreate(&e), "hipEventCreate");
+      ws->timer_events.push_back(e);
+    }
+  }
+  hipEvent_t a = ws->timer_events[ws->timers_used], b = ws->timer_events[ws->timers_used + 1];
+  ws->timers_used += 2;
+  ws->timer_slots.push_back(slot);
+  check(hipEventRecord(a, ws->stream), "hipEventRecord");
+  launch();
+  check(hipEventRecord(b, ws->stream), "hipEventRecord");
+}
+
+void start_timers(hv_workspace* ws, bool on) {
+  ws->timing = on;
+  ws->timers_used = 0;
+  ws->timer_slots.clear();
+}
+
+// Sum of the recorded kernel durations per slot (after the stream synced).
+void collect_timers(hv_workspace* ws, double* timings) {
+  for (size_t k = 0; k < ws->timer_slots.size(); ++k) {
+    float ms = 0.f;
+    check(hipEventElapsedTime(&ms, ws->timer_events[2 * k], ws->timer_events[2 * k + 1]), "timing");
+    timings[ws->timer_slots[k]] += ms;
+  }
+}
+
+// Band boundaries, half lengths and reversed filters of one (fs_w, floor,
+// ceil); cached in the workspace.  GetFilteredSignal harvest.cpp:99-108 and
+// NuttallWindow common.cpp:113-121, evaluated with the host libm like WORLD.
+void prepare_filters(hv_workspace* ws, double fs_work, double f0_floor,
+                     double f0_ceil, const double* given_boundaries, int given_bands) {
+  std::vector<double> bounds;
+  if (given_boundaries) {
+    bounds.assign(given_boundaries, given_boundaries + given_bands);
+  } else {
+    const double af = f0_floor * 0.9, ac = f0_ceil * 1.1;
+    const int bands = 1 + (int)(std::log(ac / af) / kLog2 * 40.0);
+    bounds.resize(bands);
+    for (int i = 0; i < bands; ++i) bounds[i] = af * std::pow(2.0, (i + 1) / 40.0);
+  }
+  if (ws->key_fs == fs_work && ws->key_floor == f0_floor && ws->key_ceil == f0_ceil &&
+      ws->boundaries == bounds)
+    return;
+  const int B = (int)bounds.size();
+  std::vector<int> halves(B), offs(B);
+  std::vector<double> coef;
+  int max_half = 0;
+  for (int b = 0; b < B; ++b) {
+    const int h = host_round(fs_work / bounds[b] * 2.0);
+    if (h < 1) invalid("band-pass filter shorter than 3 taps; f0_ceil is too high");
+    halves[b] = h;
+    max_half = std::max(max_half, h);
+    const int len = 2 * h + 1, taps8 = (len + 7) & ~7;
+    std::vector<double> c(len);
+    for (int i = 0; i < len; ++i) {
+      const double tmp = i / (len - 1.0);
+      c[i] = 0.355768 - 0.487396 * std::cos(2.0 * kPi * tmp) +
+             0.144232 * std::cos(4.0 * kPi * tmp) - 0.012604 * std::cos(6.0 * kPi * tmp);
+    }
+    for (int i = -h; i <= h; ++i) c[i + h] *= std::cos(2 * kPi * bounds[b] * i / fs_work);
+    offs[b] = (int)coef.size();
+    for (int mm = 0; mm < taps8; ++mm) coef.push_back(mm < len ? c[len - 1 - mm] : 0.0);
+  }
+  double* dcoef = ws->coef.ensure<double>(coef.size());
+  int* doff = ws->coef_off.ensure<int>(B);
+  int* dhalf = ws->half.ensure<int>(B);
+  double* dbound = ws->bounds.ensure<double>(B);
+  check(hipMemcpyAsync(dcoef, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice,
+                       ws->stream), "upload filters");
+  check(hipMemcpyAsync(doff, offs.data(), B * sizeof(int), hipMemcpyHostToDevice, ws->stream),
+        "upload filters");
+  check(hipMemcpyAsync(dhalf, halves.data(), B * sizeof(int), hipMemcpyHostToDevice, ws->stream),
+        "upload filters");
+  check(hipMemcpyAsync(dbound, bounds.data(), B * sizeof(double), hipMemcpyHostToDevice,
+                       ws->stream), "upload filters");
+  check(hipStreamSynchronize(ws->stream), "upload filters");
+  ws->key_fs = fs_work;
+  ws->key_floor = f0_floor;
+  ws->key_ceil = f0_ceil;
+  ws->boundaries = bounds;
+  ws->halves = halves;
+  ws->max_half = max_half;
+}
+
+void prepare_twiddles(hv_workspace* ws, double fs_work, double f0_floor) {
+  // Largest refinement window: candidates are >= f0_floor (harvest.cpp:250-252).
+  const double hmax = 1.5 * fs_work / (f0_floor * (1.0 - 1e-9)) + 2.0;
+  int need = 0;
+  if (hmax < 1e9) {
+    const long long len = 2 * (long long)hmax + 1;
+    int lg = 0;
+    while ((1LL << (lg + 1)) <= len) ++lg;
+    need = lg + 2 <= 20 ? 1 << (lg + 2) : 0;
+  }
+  if (need > kTwiddleMax) need = 0;
+  if (ws->twiddle_n < 0) ws->twiddle_n = 0;
+  // A table of size T serves every n dividing T (stride T / n); windows larger
+  // than the table fall back to sincospi inside the refinement kernel.
+  if (need <= ws->twiddle_n) return;
+  double* tw = ws->twiddle.ensure<double>(2 * (size_t)need);
+  launch_twiddle(ws->stream, tw, need);
+  ws->twiddle_n = need;
+}
+
+void fill_config(Config& c, int count, int ratio, int fs, double fs_work, double f0_floor,
+                 double f0_ceil, int bands, int capacity, int twiddle_n) {
+  c.count = count;
+  c.ratio = ratio;
+  c.lag = ratio > 1 ? (int)(std::ceil(140.0 / ratio) * ratio) : 0;
+  c.fs = fs;
+  c.fs_work = fs_work;
+  c.f0_floor = f0_floor;
+  c.f0_ceil = f0_ceil;
+  c.bands = bands;
+  c.capacity = capacity;
+  c.slots = 7 * capacity;
+  c.twiddle_n = twiddle_n;
+  const double* co = kDecimation[ratio >= 2 && ratio <= 12 ? ratio : 0];
+  c.a0 = co[0];
+  c.a1 = co[1];
+  c.a2 = co[2];
+  c.b0 = co[3];
+  c.b1 = co[4];
+  chunk_power(co, c.chunk_pow);
+}
+
+// Per-utterance layout.  `ns`, `ffts` and `frames` override WORLD's formulas
+// for the y-input entry point (hv_analyze).
+void layout(Plan& p, const std::vector<long long>& lengths, const std::vector<int>& ns,
+            const std::vector<int>& ffts, const std::vector<int>& frames) {
+  const int U = (int)lengths.size();
+  const Config& c = p.cfg;
+  p.utts.resize(U);
+  p.c_off.assign(U + 1, 0);
+  p.t_off.assign(U + 1, 0);
+  p.f_off.assign(U + 1, 0);
+  p.p_off.assign(U + 1, 0);
+  p.y_off.assign(U + 1, 0);
+  long long x = 0, m = 0;
+  for (int u = 0; u < U; ++u) {
+    Utt& t = p.utts[u];
+    std::memset(&t, 0, sizeof(t));
+    t.x_off = x;
+    t.length = lengths[u];
+    t.m_off = m;
+    if (c.ratio > 1) {
+      const long long padded = t.length + 2LL * c.lag;
+      t.m = padded + 18;
+      const long long nout = (padded - 1) / c.ratio + 1;
+      t.nbeg = (int)(c.ratio - c.ratio * nout + padded);
+    }
+    t.n = ns[u];
+    t.fft = ffts[u];
+    t.frames = frames[u];
+    t.tiles = (int)((t.n + kTile - 1) / kTile);
+    t.chunks = (int)((t.m + kChunk - 1) / kChunk);
+    t.pblocks = (int)((t.n + kMeanBlock - 1) / kMeanBlock);
+    t.y_off = p.y_off[u];
+    t.f_off = p.f_off[u];
+    t.t_off = p.t_off[u];
+    t.c_off = p.c_off[u];
+    t.p_off = p.p_off[u];
+    p.y_off[u + 1] = p.y_off[u] + t.n;
+    p.f_off[u + 1] = p.f_off[u] + t.frames;
+    p.t_off[u + 1] = p.t_off[u] + t.tiles;
+    p.c_off[u + 1] = p.c_off[u] + t.chunks;
+    p.p_off[u + 1] = p.p_off[u] + t.pblocks;
+    x += t.length;
+    m += t.m;
+  }
+  p.total_x = x;
+  p.total_m = m;
+  p.total_y = p.y_off[U];
+  p.total_frames = p.f_off[U];
+  p.total_tiles = p.t_off[U];
+  p.total_chunks = p.c_off[U];
+  p.total_pblocks = p.p_off[U];
+  if (p.total_frames * (long long)c.slots >= (1LL << 31) - 1)
+    invalid("input or analysis settings exceed supported allocation sizes");
+}
+
+// Upload the plan's descriptor arrays through one pinned staging buffer.
+void upload_plan(hv_workspace* ws, const Plan& p) {
+  const int U = p.cfg.count;
+  const size_t bytes = U * sizeof(Utt) + 5 * (U + 1) * sizeof(long long);
+  char* h = ws->h_meta.ensure<char>(bytes);
+  std::memcpy(h, p.utts.data(), U * sizeof(Utt));
+  size_t at = U * sizeof(Utt);
+  const std::vector<long long>* arrays[5] = {&p.c_off, &p.t_off, &p.f_off, &p.p_off, &p.y_off};
+  DevBuf* dst[5] = {&ws->c_off, &ws->t_off, &ws->f_off, &ws->p_off, &ws->y_off};
+  check(hipMemcpyAsync(ws->utt.ensure<Utt>(U), h, U * sizeof(Utt), hipMemcpyHostToDevice,
+                       ws->stream), "upload plan");
+  for (int k = 0; k < 5; ++k) {
+    std::memcpy(h + at, arrays[k]->data(), (U + 1) * sizeof(long long));
+    check(hipMemcpyAsync(dst[k]->ensure<long long>(U + 1), h + at, (U + 1) * sizeof(long long),
+                         hipMemcpyHostToDevice, ws->stream), "upload plan");
+    at += (U + 1) * sizeof(long long);
+  }
+}
+
+// Everything after the decimated, mean-removed y is on the device.
+void analyze_device(hv_workspace* ws, const Plan& p, int32_t* offsets, double* best,
+                    int64_t* nnz, double* timings) {
+  const Config& c = p.cfg;
+  const int U = c.count, B = c.bands;
+  hipStream_t s = ws->stream;
+  const Utt* dutt = ws->utt.get<Utt>();
+  double* y = ws->y.get<double>();
+  const double* coef = ws->coef.get<double>();
+  const int* coef_off = ws->coef_off.get<int>();
+  const int* half = ws->half.get<int>();
+
+  Quirk* quirk = ws->quirk.ensure<Quirk>((size_t)U * B);
+  launch_quirk(s, c, dutt, ws->p_off.get<long long>(), p.total_pblocks, y, coef, coef_off, half,
+               ws->partial3.ensure<double>(3 * p.total_pblocks), ws->sums3.ensure<double>(3 * U),
+               quirk);
+
+  // Band groups bounded by the slab budget.
+  const size_t per_band = (size_t)4 * p.total_tiles * kSlabCap * sizeof(double);
+  int group = (int)std::max<size_t>(1, std::min<size_t>(B, kSlabBudget / std::max<size_t>(per_band, 1)));
+  double* slab = ws->slab.ensure<double>((size_t)group * 4 * p.total_tiles * kSlabCap);
+  int* counts = ws->counts.ensure<int>((size_t)group * 4 * p.total_tiles);
+  int* offs = ws->offs.ensure<int>((size_t)group * 4 * p.total_tiles);
+  int* totals = ws->totals.ensure<int>((size_t)group * 4 * U);
+  double* raw = ws->raw.ensure<double>((size_t)B * p.total_frames);
+  for (int b0 = 0; b0 < B; b0 += group) {
+    const int g = std::min(group, B - b0);
+    int mh = 0;
+    for (int b = b0; b < b0 + g; ++b) mh = std::max(mh, ws->halves[b]);
+    timed(ws, 4, [&] {
+      launch_filter_events(s, c, dutt, ws->t_off.get<long long>(), p.total_tiles, b0, g, mh, y,
+                           coef, coef_off, half, quirk, slab, counts);
+    });
+    timed(ws, 5, [&] {
+      launch_tile_scan(s, c, dutt, p.total_tiles, g, counts, offs, totals);
+      launch_interp(s, c, dutt, ws->t_off.get<long long>(), p.total_tiles, p.total_frames, b0, g,
+                    ws->bounds.get<double>(), slab, counts, offs, totals, raw);
+    });
+  }
+  int* count = ws->count.ensure<int>(U);
+  launch_fill_int(s, count, U, 0);
+  double* official = ws->official.ensure<double>((size_t)p.total_frames * c.capacity);
+  launch_official(s, c, ws->f_off.get<long long>(), p.total_frames, raw, official, count);
+  check(hipEventRecord(ws->ev[2], s), "hipEventRecord");
+
+  const size_t slots = (size_t)p.total_frames * c.slots;
+  double* rf0 = ws->rf0.ensure<double>(slots);
+  double* rsc = ws->rsc.ensure<double>(slots);
+  timed(ws, 6, [&] {
+    launch_refine(s, c, dutt, ws->f_off.get<long long>(), p.total_frames, y, official, count,
+                  ws->twiddle.get<double>(), rf0, rsc);
+  });
+  double* pf0 = ws->pf0.ensure<double>(slots);
+  double* psc = ws->psc.ensure<double>(slots);
+  double* dbest = ws->best.ensure<double>(p.total_frames);
+  int* row_nnz = ws->row_nnz.ensure<int>(p.total_frames);
+  int* doffsets = ws->offsets.ensure<int>(p.total_frames + 1);
+  const long long nb = scan_block_count(p.total_frames);
+  int* scan_tmp = ws->scan_tmp.ensure<int>(nb + 1);
+  timed(ws, 7, [&] {
+    launch_prune_best(s, c, dutt, ws->f_off.get<long long>(), p.total_frames, count, rf0, rsc,
+                      pf0, psc, dbest, row_nnz);
+    launch_exclusive_scan(s, row_nnz, doffsets, p.total_frames, scan_tmp, nb + 1);
+    // Compacted candidates reuse the refinement buffers (no longer needed).
+    launch_compact(s, c, dutt, ws->f_off.get<long long>(), p.total_frames, count, pf0, psc,
+                   doffsets, rf0, rsc);
+  });
+  int32_t* hoff = ws->h_offsets.ensure<int32_t>(p.total_frames + 1);
+  double* hbest = ws->h_best.ensure<double>(p.total_frames);
+  check(hipMemcpyAsync(hoff, doffsets, (p.total_frames + 1) * sizeof(int32_t),
+                       hipMemcpyDeviceToHost, s), "download offsets");
+  check(hipMemcpyAsync(hbest, dbest, p.total_frames * sizeof(double), hipMemcpyDeviceToHost, s),
+        "download best");
+  check(hipEventRecord(ws->ev[3], s), "hipEventRecord");
+  check(hipStreamSynchronize(s), "analysis");
+  std::memcpy(offsets, hoff, (p.total_frames + 1) * sizeof(int32_t));
+  std::memcpy(best, hbest, p.total_frames * sizeof(double));
+  ws->last_nnz = hoff[p.total_frames];
+  *nnz = ws->last_nnz;
+  if (timings) {
+    float ms[3];
+    check(hipEventElapsedTime(&ms[0], ws->ev[0], ws->ev[1]), "timing");
+    check(hipEventElapsedTime(&ms[1], ws->ev[1], ws->ev[2]), "timing");
+    check(hipEventElapsedTime(&ms[2], ws->ev[2], ws->ev[3]), "timing");
+    for (int k = 0; k < HV_TIMING_SLOTS; ++k) timings[k] = k < 3 ? ms[k] : 0.0;
+    collect_timers(ws, timings);
+  }
+  ws->timing = false;
+}
+
+template <class F> hv_status guarded(F&& body) {
+  try {
+    body();
+    g_error.clear();
+    return HV_OK;
+  } catch (const Failure& f) {
+    g_error = f.what();
+    return f.status;
+  } catch (const std::bad_alloc&) {
+    g_error = "host memory allocation failed";
+    return HV_ENOMEM;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return HV_EINTERNAL;
+  } catch (...) {
+    g_error = "unknown error";
+    return HV_EINTERNAL;
+  }
+}
+
+// WORLD's per-utterance sizes (harvest.cpp:1149-1181, 1219-1221).
+void world_sizes(long long length, int fs, double f0_floor, int ratio, double fs_work, int* n,
+                 int* fft, int* frames) {
+  *n = (int)std::ceil((double)length / ratio);
+  const double b0 = f0_floor * 0.9 * std::pow(2.0, 1 / 40.0);
+  *fft = suitable_fft_size(*n + 5 + 2 * (int)(2.0 * fs_work / b0));
+  *frames = (int)(1000.0 * (double)length / fs / 1.0) + 1;
+}
+
+void check_common(int fs, double f0_floor, double f0_ceil) {
+  if (fs < 8000 || fs > 192000) invalid("fs must be between 8000 and 192000 Hz");
+  if (!std::isfinite(f0_floor) || !std::isfinite(f0_ceil) || f0_floor <= 0 || f0_ceil <= f0_floor)
+    invalid("require finite 0 < f0_floor < f0_ceil");
+}
+
+// Shared body of the waveform entry points; `device_x` is true when xs[0]
+// already points to the concatenated batch in device memory.
+void run_batch(hv_workspace* ws, int count, const double* const* xs, const int64_t* lengths,
+               int fs, double f0_floor, double f0_ceil, int32_t* offsets, double* best,
+               int64_t* nnz, double* timings, bool device_x) {
+  if (!ws) invalid("null workspace");
+  if (count < 1) invalid("count must be positive");
+  check_common(fs, f0_floor, f0_ceil);
+  DeviceGuard guard(ws->device);
+  const int ratio = std::max(std::min(host_round(fs / 8000.0), 12), 1);
+  const double fs_work = (double)fs / ratio;
+  if (f0_ceil >= fs_work / 2) invalid("f0_ceil must be below the internal Nyquist frequency");
+  std::vector<long long> lens(count);
+  std::vector<int> ns(count), ffts(count), frames(count);
+  for (int u = 0; u < count; ++u) {
+    if (lengths[u] < 2LL * ratio || lengths[u] > 0x7fffffffLL)
+      invalid("x is too short: require at least two samples after internal decimation");
+    lens[u] = lengths[u];
+    world_sizes(lens[u], fs, f0_floor, ratio, fs_work, &ns[u], &ffts[u], &frames[u]);
+  }
+  prepare_filters(ws, fs_work, f0_floor, f0_ceil, nullptr, 0);
+  prepare_twiddles(ws, fs_work, f0_floor);
+  Plan p;
+  const int B = (int)ws->boundaries.size();
+  fill_config(p.cfg, count, ratio, fs, fs_work, f0_floor, f0_ceil, B, host_round(B / 10.0),
+              ws->twiddle_n);
+  layout(p, lens, ns, ffts, frames);
+  hipStream_t s = ws->stream;
+  start_timers(ws, timings != nullptr);
+  check(hipEventRecord(ws->ev[0], s), "hipEventRecord");
+  upload_plan(ws, p);
+  const double* dx;
+  if (device_x) {
+    dx = xs[0];
+  } else {
+    double* hx = ws->h_x.ensure<double>(p.total_x);
+    for (int u = 0; u < count; ++u)
+      std::memcpy(hx + p.utts[u].x_off, xs[u], lens[u] * sizeof(double));
+    double* d = ws->x.ensure<double>(p.total_x);
+    check(hipMemcpyAsync(d, hx, p.total_x * sizeof(double), hipMemcpyHostToDevice, s), "upload x");
+    dx = d;
+  }
+  double* y = ws->y.ensure<double>(p.total_y);
+  double* pass = ws->pass.ensure<double>(p.total_m);
+  double* states = ws->states.ensure<double>(3 * p.total_chunks);
+  double* carries = ws->carries.ensure<double>(3 * p.total_chunks);
+  timed(ws, 3, [&] {
+    if (ratio > 1)
+      launch_decimate(s, p.cfg, ws->utt.get<Utt>(), ws->c_off.get<long long>(), p.total_chunks,
+                      dx, pass, states, carries, y);
+    else
+      launch_copy_input(s, p.cfg, ws->utt.get<Utt>(), ws->y_off.get<long long>(), p.total_y, dx,
+                        y);
+  });
+  launch_remove_mean(s, p.cfg, ws->utt.get<Utt>(), ws->p_off.get<long long>(), p.total_pblocks,
+                     ws->y_off.get<long long>(), p.total_y, y,
+                     ws->partial.ensure<double>(p.total_pblocks), ws->mean.ensure<double>(count));
+  check(hipEventRecord(ws->ev[1], s), "hipEventRecord");
+  analyze_device(ws, p, offsets, best, nnz, timings);
+}
+
+}  // namespace
+}  // namespace hv
+
+extern "C" {
+
+const char* hv_last_error(void) { return g_error.c_str(); }
+
+int hv_device_supported(int device) {
+  if (device < 0) return 0;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  if (device >= n) return 0;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
+}
+
+hv_status hv_workspace_create(int device, hv_workspace** out) {
+  return guarded([&] {
+    if (!out) invalid("null output pointer");
+    *out = nullptr;
+    if (device < 0) invalid("device must be a nonnegative visible GPU index");
+    int n = 0;
+    check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (n == 0) throw Failure(HV_ENODEV, "HIP: no GPU is visible. Check the ROCm driver and HIP_VISIBLE_DEVICES before starting Python.");
+    if (device >= n)
+      invalid("device " + std::to_string(device) + " is out of range; " + std::to_string(n) +
+              " GPU(s) are visible through HIP_VISIBLE_DEVICES");
+    hipDeviceProp_t prop;
+    check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+      throw Failure(HV_EUNSUPPORTED, std::string("Harvest requires an MI355X-class gfx950 GPU; device ") +
+                                         std::to_string(device) + " (" + prop.name + ") is " +
+                                         prop.gcnArchName);
+    DeviceGuard guard(device);
+    *out = new hv_workspace(device);
+  });
+}
+
+void hv_workspace_destroy(hv_workspace* ws) {
+  if (!ws) return;
+  try {
+    DeviceGuard guard(ws->device);
+    delete ws;
+  } catch (...) {
+    // Best effort after driver shutdown.
+  }
+}
+
+int hv_workspace_device(const hv_workspace* ws) { return ws ? ws->device : -1; }
+
+hv_status hv_decimate(hv_workspace* ws, const double* x, int64_t length, int ratio, double* y,
+                      int64_t y_length) {
+  return guarded([&] {
+    if (!ws || !x || !y) invalid("null pointer");
+    if (ratio < 1 || ratio > 12) invalid("ratio must be between 1 and 12");
+    if (length < 2LL * ratio || length > 0x7fffffffLL) invalid("invalid length");
+    if (y_length != (length + ratio - 1) / ratio) invalid("y_length must be ceil(length / ratio)");
+    DeviceGuard guard(ws->device);
+    Plan p;
+    fill_config(p.cfg, 1, ratio, 8000 * ratio, 8000.0, 71.0, 800.0, 0, 0, 0);
+    layout(p, {length}, {(int)y_length}, {0}, {0});
+    hipStream_t s = ws->stream;
+    upload_plan(ws, p);
+    double* hx = ws->h_x.ensure<double>(length);
+    std::memcpy(hx, x, length * sizeof(double));
+    double* dx = ws->x.ensure<double>(length);
+    check(hipMemcpyAsync(dx, hx, length * sizeof(double), hipMemcpyHostToDevice, s), "upload x");
+    double* dy = ws->y.ensure<double>(y_length);
+    if (ratio > 1)
+      launch_decimate(s, p.cfg, ws->utt.get<Utt>(), ws->c_off.get<long long>(), p.total_chunks, dx,
+                      ws->pass.ensure<double>(p.total_m), ws->states.ensure<double>(3 * p.total_chunks),
+                      ws->carries.ensure<double>(3 * p.total_chunks), dy);
+    else
+      launch_copy_input(s, p.cfg, ws->utt.get<Utt>(), ws->y_off.get<long long>(), p.total_y, dx, dy);
+    double* hy = ws->h_best.ensure<double>(y_length);
+    check(hipMemcpyAsync(hy, dy, y_length * sizeof(double), hipMemcpyDeviceToHost, s), "download y");
+    check(hipStreamSynchronize(s), "decimate");
+    std::memcpy(y, hy, y_length * sizeof(double));
+  });
+}
+
+hv_status hv_analyze(hv_workspace* ws, const double* y, int64_t y_length, int fft_size,
+                     double fs_work, const double* boundaries, int bands, int frames,
+                     double f0_floor, double f0_ceil, int capacity, int32_t* offsets,
+                     double* best, int64_t* nnz, double* timings_ms) {
+  return guarded([&] {
+    if (!ws || !y || !boundaries || !offsets || !best || !nnz) invalid("null pointer");
+    if (y_length < 2 || y_length > 0x7fffffffLL || bands < 3 || frames < 1 || capacity < 1)
+      invalid("invalid analysis sizes");
+    if (fft_size < y_length || (fft_size & (fft_size - 1)) != 0)
+      invalid("fft_size must be a power of two no smaller than y_length");
+    // Frames map to filterbank tiles with integer arithmetic: fs_work = num / den.
+    int den = 0;
+    for (int d = 1; d <= 12 && !den; ++d)
+      if (std::floor(fs_work * d) == fs_work * d) den = d;
+    if (!den) invalid("fs_work must be an integer sample rate divided by 1..12");
+    DeviceGuard guard(ws->device);
+    prepare_filters(ws, fs_work, f0_floor, f0_ceil, boundaries, bands);
+    prepare_twiddles(ws, fs_work, f0_floor);
+    Plan p;
+    fill_config(p.cfg, 1, den, (int)(fs_work * den), fs_work, f0_floor, f0_ceil, bands, capacity,
+                ws->twiddle_n);
+    p.cfg.lag = 0;
+    layout(p, {y_length}, {(int)y_length}, {fft_size}, {frames});
+    hipStream_t s = ws->stream;
+    start_timers(ws, timings_ms != nullptr);
+    check(hipEventRecord(ws->ev[0], s), "hipEventRecord");
+    upload_plan(ws, p);
+    double* hy = ws->h_x.ensure<double>(y_length);
+    std::memcpy(hy, y, y_length * sizeof(double));
+    check(hipMemcpyAsync(ws->y.ensure<double>(y_length), hy, y_length * sizeof(double),
+                         hipMemcpyHostToDevice, s), "upload y");
+    check(hipEventRecord(ws->ev[1], s), "hipEventRecord");
+    analyze_device(ws, p, offsets, best, nnz, timings_ms);
+  });
+}
+
+hv_status hv_analyze_batch(hv_workspace* ws, int count, const double* const* xs,
+                           const int64_t* lengths, int fs, double f0_floor, double f0_ceil,
+                           int32_t* offsets, double* best, int64_t* nnz, double* timings_ms) {
+  return guarded([&] {
+    if (!xs || !lengths || !offsets || !best || !nnz) invalid("null pointer");
+    for (int u = 0; u < count; ++u)
+      if (!xs[u]) invalid("null waveform pointer");
+    run_batch(ws, count, xs, lengths, fs, f0_floor, f0_ceil, offsets, best, nnz, timings_ms, false);
+  });
+}
+
+hv_status hv_analyze_batch_device(hv_workspace* ws, int count, const double* x_device,
+                                  const int64_t* lengths, int fs, double f0_floor,
+                                  double f0_ceil, int32_t* offsets, double* best, int64_t* nnz,
+                                  double* timings_ms) {
+  return guarded([&] {
+    if (!x_device || !lengths || !offsets || !best || !nnz) invalid("null pointer");
+    const double* xs[1] = {x_device};
+    run_batch(ws, count, xs, lengths, fs, f0_floor, f0_ceil, offsets, best, nnz, timings_ms, true);
+  });
+}
+
+hv_status hv_fetch_candidates(hv_workspace* ws, double* f0, double* scores) {
+  return guarded([&] {
+    if (!ws) invalid("null workspace");
+    if (ws->last_nnz < 0) invalid("no analysis result to fetch");
+    if (ws->last_nnz == 0) return;
+    if (!f0 || !scores) invalid("null pointer");
+    DeviceGuard guard(ws->device);
+    const long long n = ws->last_nnz;
+    double* h = ws->h_cand.ensure<double>(2 * n);
+    check(hipMemcpyAsync(h, ws->rf0.get<double>(), n * sizeof(double), hipMemcpyDeviceToHost,
+                         ws->stream), "download candidates");
+    check(hipMemcpyAsync(h + n, ws->rsc.get<double>(), n * sizeof(double), hipMemcpyDeviceToHost,
+                         ws->stream), "download candidates");
+    check(hipStreamSynchronize(ws->stream), "download candidates");
+    std::memcpy(f0, h, n * sizeof(double));
+    std::memcpy(scores, h + n, n * sizeof(double));
+  });
+}
+
+}  // extern "C"

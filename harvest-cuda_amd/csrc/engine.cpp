@@ -1,0 +1,498 @@
+// Host engine: WORLD's parameter derivation, the backend call, and the
+// sequential contour stages in O(F).  Reference counterpart: src/engine.cpp
+// (analyze :240-333, contour adapters :55-214).  This file is built with
+// -ffp-contract=off; every floating-point expression below keeps WORLD's
+// evaluation order because each one feeds a discrete decision.
+#include "engine.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <numeric>
+#include <stdexcept>
+#include <thread>
+
+namespace harvest {
+namespace {
+
+constexpr double kLog2 = 0.69314718055994529;  // constantnumbers.h:24
+
+int mround(double x) { return x > 0 ? (int)(x + 0.5) : (int)(x - 0.5); }
+
+// Sections of a contour as GetBoundaryList (harvest.cpp:727-743) reports
+// them: entries alternate start (first voiced frame) and end (last voiced
+// frame); the first and last frames count as unvoiced.
+void boundary_list(const double* f0, int frames, std::vector<int>& list) {
+  list.clear();
+  int prev = 0;
+  for (int i = 1; i < frames; ++i) {
+    const int v = (i == frames - 1) ? 0 : (f0[i] > 0 ? 1 : 0);
+    if (v != prev) list.push_back(i - (int)(list.size() % 2));
+    prev = v;
+  }
+}
+
+// FixStep1 (harvest.cpp:710-722), allowed range 0.008.
+void fix_step1(const double* base, int frames, double* out) {
+  for (int i = 0; i < frames; ++i) out[i] = 0.0;
+  for (int i = 2; i < frames; ++i) {
+    if (base[i] == 0.0) continue;
+    const double reference = base[i - 1] * 2 - base[i - 2];
+    const bool jump = std::fabs((base[i] - reference) / reference) > 0.008 &&
+                      std::fabs((base[i] - base[i - 1])) / base[i - 1] > 0.008;
+    out[i] = jump ? 0.0 : base[i];
+  }
+}
+
+// FixStep2 (harvest.cpp:748-762): voiced sections shorter than 6 frames.
+void fix_step2(const double* in, int frames, double* out, std::vector<int>& list) {
+  std::copy(in, in + frames, out);
+  boundary_list(in, frames, list);
+  for (size_t s = 0; s + 1 < list.size(); s += 2) {
+    if (list[s + 1] - list[s] >= 6) continue;
+    for (int j = list[s]; j <= list[s + 1]; ++j) out[j] = 0.0;
+  }
+}
+
+// One voiced section of FixStep3 together with everything ExtendF0 can write:
+// at most 101 frames either side (harvest.cpp:865-873).
+struct Section {
+  int offset = 0, start = 0, end = 0;
+  std::vector<double> values;
+  double at(int frame) const {
+    const int i = frame - offset;
+    return i >= 0 && i < (int)values.size() ? values[i] : 0.0;
+  }
+};
+
+// SelectBestF0 (harvest.cpp:636-650) over the nonzero candidates of a frame;
+// zero candidates have error 1 and never beat the 0.18 allowance.
+double select_best(double reference, const CandidateRows& rows, int frame) {
+  double best = 0.0, error = 0.18;
+  for (int k = rows.offsets[frame]; k < rows.offsets[frame + 1]; ++k) {
+    const double e = std::fabs(reference - rows.f0[k]) / reference;
+    if (e > error) continue;
+    best = rows.f0[k];
+    error = e;
+  }
+  return best;
+}
+
+// ExtendF0 (harvest.cpp:791-820): follow the closest candidate frame by
+// frame, stop after 4 misses, return the last frame that found one.
+int extend(Section& s, int origin, int last, int shift, const CandidateRows& rows) {
+  double reference = s.at(origin);
+  int reached = origin, misses = 0;
+  const int distance = std::abs(last - origin);
+  for (int i = 0; i <= distance; ++i) {
+    const int frame = origin + shift * i + shift;
+    const double v = select_best(reference, rows, frame);
+    s.values[frame - s.offset] = v;
+    if (v == 0.0) {
+      ++misses;
+    } else {
+      reference = v;
+      misses = 0;
+      reached = frame;
+    }
+    if (misses == 4) break;
+  }
+  return reached;
+}
+
+// SearchScore (harvest.cpp:901-907): the largest score of a candidate equal
+// to f0; zero candidates carry zero score and cannot raise it.
+double search_score(double f0, const CandidateRows& rows, int frame) {
+  double score = 0.0;
+  for (int k = rows.offsets[frame]; k < rows.offsets[frame + 1]; ++k)
+    if (f0 == rows.f0[k] && score < rows.scores[k]) score = rows.scores[k];
+  return score;
+}
+
+// MakeSortedOrder (harvest.cpp:883-896): WORLD's exchange pass, kept as is.
+void sorted_order(const std::vector<int>& list, int n, std::vector<int>& order) {
+  order.resize(n);
+  std::iota(order.begin(), order.end(), 0);
+  for (int i = 1; i < n; ++i)
+    for (int j = i - 1; j >= 0; --j) {
+      if (list[order[j] * 2] > list[order[i] * 2]) std::swap(order[i], order[j]);
+      else break;
+    }
+}
+
+// FixStep3 (harvest.cpp:968-995) in O(F): each section keeps only its own
+// neighbourhood instead of a full-length channel.
+void fix_step3(const double* in, int frames, const CandidateRows& rows, double* out,
+               std::vector<int>& list) {
+  std::copy(in, in + frames, out);
+  boundary_list(in, frames, list);
+  const int count = (int)list.size() / 2;
+  std::vector<Section> sections(count);
+  for (int i = 0; i < count; ++i) {
+    Section& s = sections[i];
+    const int st = list[2 * i], ed = list[2 * i + 1];
+    s.offset = std::max(0, st - 101);
+    s.values.assign(std::min(frames, ed + 102) - s.offset, 0.0);
+    std::copy(in + st, in + ed + 1, s.values.begin() + (st - s.offset));
+    s.end = extend(s, ed, std::min(frames - 2, ed + 100), 1, rows);
+    s.start = extend(s, st, std::max(1, st - 100), -1, rows);
+  }
+  // ExtendSub (harvest.cpp:840-856): the running mean is carried across
+  // sections, and selected sections move to the front in order.
+  int selected = 0;
+  double mean = 0.0;
+  for (int i = 0; i < count; ++i) {
+    Section& s = sections[i];
+    for (int j = s.start; j < s.end; ++j) mean += s.at(j);
+    mean /= s.end - s.start;
+    if (2200.0 / mean < s.end - s.start) std::swap(sections[selected++], sections[i]);
+  }
+  if (selected == 0) return;
+  // MergeF0 (harvest.cpp:937-963).  Boundary pair 0 doubles as the merge
+  // cursor and is mutated in place, also when section 0 is revisited.
+  std::vector<int> bl(2 * selected);
+  for (int k = 0; k < selected; ++k) {
+    bl[2 * k] = sections[k].start;
+    bl[2 * k + 1] = sections[k].end;
+  }
+  std::vector<int> order;
+  sorted_order(bl, selected, order);
+  for (int i = 0; i < frames; ++i) out[i] = sections[0].at(i);
+  for (int i = 1; i < selected; ++i) {
+    const int k = order[i];
+    const Section& s = sections[k];
+    if (bl[2 * k] - bl[1] > 0) {
+      for (int j = bl[2 * k]; j <= bl[2 * k + 1]; ++j) out[j] = s.at(j);
+      bl[0] = bl[2 * k];
+      bl[1] = bl[2 * k + 1];
+      continue;
+    }
+    // MergeF0Sub (harvest.cpp:912-932).
+    const int st1 = bl[0], ed1 = bl[1], st2 = bl[2 * k], ed2 = bl[2 * k + 1];
+    if (st1 <= st2 && ed1 >= ed2) {
+      bl[1] = ed1;
+      continue;
+    }
+    double score1 = 0.0, score2 = 0.0;
+    for (int j = st2; j <= ed1; ++j) {
+      score1 += search_score(out[j], rows, j);
+      score2 += search_score(s.at(j), rows, j);
+    }
+    const int from = score1 > score2 ? ed1 : st2;
+    for (int j = from; j <= ed2; ++j) out[j] = s.at(j);
+    bl[1] = ed2;
+  }
+}
+
+// FixStep4 (harvest.cpp:1000-1022): bridge gaps shorter than 9 frames.
+void fix_step4(const double* in, int frames, double* out, std::vector<int>& list) {
+  std::copy(in, in + frames, out);
+  boundary_list(in, frames, list);
+  const int sections = (int)list.size() / 2;
+  for (int i = 0; i < sections - 1; ++i) {
+    const int distance = list[(i + 1) * 2] - list[i * 2 + 1] - 1;
+    if (distance >= 9) continue;
+    const double tmp0 = in[list[i * 2 + 1]] + 1;
+    const double tmp1 = in[list[(i + 1) * 2]] - 1;
+    const double coefficient = (tmp1 - tmp0) / (distance + 1.0);
+    int count = 1;
+    for (int j = list[i * 2 + 1] + 1; j <= list[(i + 1) * 2] - 1; ++j)
+      out[j] = tmp0 + coefficient * count++;
+  }
+}
+
+// The zero-lag smoother's second-order recursion (FilteringF0,
+// harvest.cpp:1049-1074).
+struct Smoother {
+  static constexpr double b0 = 0.0078202080334971724, b1 = 0.015640416066994345;
+  static constexpr double a0 = 1.7347257688092754, a1 = -0.76600660094326412;
+  double w0 = 0.0, w1 = 0.0;
+  double step(double x) {
+    const double wt = x + a0 * w0 + a1 * w1;
+    const double y = b0 * wt + b1 * w0 + b0 * w1;
+    w1 = w0;
+    w0 = wt;
+    return y;
+  }
+  // Runs `count` steps of a constant input.  Once a step leaves the state
+  // unchanged (w0 == w1 == new w0) every later step repeats it bit for bit,
+  // so the remaining steps are skipped; returns the index of that step, or
+  // -1 if the state never settled.  Outputs are not needed by the callers.
+  long settle(double x, long count) {
+    for (long s = 0; s < count; ++s) {
+      const double wt = x + a0 * w0 + a1 * w1;
+      if (wt == w0 && w0 == w1) return s;
+      w1 = w0;
+      w0 = wt;
+    }
+    return -1;
+  }
+};
+
+}  // namespace
+
+void throw_status(hv_status status) {
+  if (status == HV_OK) return;
+  std::string message = hv_last_error();
+  if (message.empty()) message = "Harvest backend error " + std::to_string((int)status);
+  if (status == HV_EINVAL) throw std::invalid_argument(message);
+  throw std::runtime_error(message);
+}
+
+Params derive_params(int64_t length, int fs, double f0_floor, double f0_ceil,
+                     double frame_period) {
+  Params p{};
+  const double af = f0_floor * 0.9, ac = f0_ceil * 1.1;
+  p.bands = 1 + (int)(std::log(ac / af) / kLog2 * 40.0);
+  p.ratio = std::max(std::min(mround(fs / 8000.0), 12), 1);
+  p.y_length = (int)std::ceil((double)length / p.ratio);
+  p.fs_work = (double)fs / p.ratio;
+  const double b0 = af * std::pow(2.0, 1 / 40.0);
+  const int sample = p.y_length + 5 + 2 * (int)(2.0 * p.fs_work / b0);
+  p.fft_size = (int)std::pow(2.0, (int)(std::log((double)sample) / kLog2) + 1.0);
+  p.frames = (int)(1000.0 * (double)length / fs / 1.0) + 1;
+  p.out_frames = (int)(1000.0 * (double)length / fs / frame_period) + 1;
+  p.capacity = mround(p.bands / 10.0);
+  return p;
+}
+
+void validate(const double* x, int64_t length, int fs, double f0_floor, double f0_ceil,
+              double frame_period) {
+  if (length <= 0 || length > INT_MAX)
+    throw std::invalid_argument("x must be nonempty and contain at most INT_MAX samples");
+  if (fs < 8000 || fs > 192000)
+    throw std::invalid_argument("fs must be between 8000 and 192000 Hz");
+  if (!std::isfinite(f0_floor) || !std::isfinite(f0_ceil) || f0_floor <= 0 || f0_ceil <= f0_floor)
+    throw std::invalid_argument("require finite 0 < f0_floor < f0_ceil");
+  const int ratio = std::min(12, std::max(1, (int)(fs / 8000.0 + 0.5)));
+  if (length < 2LL * ratio)
+    throw std::invalid_argument(
+        "x is too short: require at least two samples after internal decimation");
+  if (f0_ceil >= (double)fs / ratio / 2)
+    throw std::invalid_argument("f0_ceil must be below the internal Nyquist frequency");
+  if (!std::isfinite(frame_period) || frame_period <= 0)
+    throw std::invalid_argument("frame_period must be finite and positive (milliseconds)");
+  if (1000.0 * length / fs / frame_period >= INT_MAX || 1000.0 * length / fs >= INT_MAX ||
+      (double)length / ratio + 10.0 * fs / ratio / f0_floor >= (1 << 28))
+    throw std::invalid_argument("input or analysis settings exceed supported allocation sizes");
+  bool finite = true;
+  for (int64_t i = 0; i < length; ++i) finite &= std::isfinite(x[i]);
+  if (!finite) throw std::invalid_argument("x must contain only finite samples");
+}
+
+void fix_contour(const CandidateRows& rows, int frames, const double* base, double* out) {
+  std::vector<double> a(frames), b(frames);
+  std::vector<int> list;
+  list.reserve(64);
+  fix_step1(base, frames, a.data());
+  fix_step2(a.data(), frames, b.data(), list);
+  fix_step3(b.data(), frames, rows, a.data(), list);
+  fix_step4(a.data(), frames, out, list);
+}
+
+// SmoothF0Contour (harvest.cpp:1079-1113).  WORLD pads the contour with 300
+// zeros on each side and, for every voiced section, runs the zero-phase
+// filter over the whole padded length with the section's end values held
+// constant outside it.  Only the stretches where the recursion has not yet
+// settled on a constant input are simulated; the rest is skipped exactly
+// (Smoother::settle), which leaves every output bit-identical to WORLD's.
+void smooth_contour(const double* f0, int frames, double* out) {
+  constexpr int lag = 300;
+  const long n = (long)frames + 2 * lag;
+  auto padded = [&](long i) { return i >= lag && i < lag + frames ? f0[i - lag] : 0.0; };
+  for (int i = 0; i < frames; ++i) out[i] = 0.0;
+  std::vector<double> section_out, tail;
+  long i = 1;
+  while (i < n - 1) {
+    if (!(padded(i) > 0)) {
+      ++i;
+      continue;
+    }
+    const long st = i;
+    long ed = i;
+    while (ed + 1 < n - 1 && padded(ed + 1) > 0) ++ed;
+    i = ed + 1;
+    // Forward pass: constant prefix, the section, then the constant suffix.
+    Smoother fwd;
+    fwd.settle(padded(st), st);
+    section_out.resize(ed - st + 1);
+    for (long k = st; k <= ed; ++k) section_out[k - st] = fwd.step(padded(k));
+    const long steps = n - 1 - ed;
+    const double held = padded(ed);
+    tail.clear();
+    long fixed_at = -1;
+    double steady = 0.0;
+    for (long k = 0; k < steps; ++k) {
+      const bool settled_state = fwd.w0 == fwd.w1;
+      const double before = fwd.w0;
+      const double y = fwd.step(held);
+      tail.push_back(y);
+      if (settled_state && fwd.w0 == before) {
+        fixed_at = k;
+        steady = y;
+        break;
+      }
+    }
+    // Backward pass from the padded end: the settled tail is constant input.
+    Smoother bwd;
+    long explicit_last = (long)tail.size() - 1;
+    if (fixed_at >= 0) bwd.settle(steady, steps - 1 - fixed_at);
+    for (long k = explicit_last; k >= 0; --k) {
+      if (fixed_at >= 0 && k == fixed_at) {
+        bwd.step(steady);
+        continue;
+      }
+      bwd.step(tail[k]);
+    }
+    for (long k = ed; k >= st; --k) out[k - lag] = bwd.step(section_out[k - st]);
+  }
+}
+
+void resample(const double* basic, int frames, double frame_period, int out_frames, double* f0,
+              double* times) {
+  for (int i = 0; i < out_frames; ++i) {
+    times[i] = i * frame_period / 1000.0;
+    f0[i] = basic[std::min(frames - 1, mround(times[i] * 1000.0))];
+  }
+}
+
+namespace {
+using Clock = std::chrono::steady_clock;
+
+// Device spans and kernel sums (slot meaning in include/harvest_hip.h).
+void record_device(const double* ms, Timings& t) {
+  static const char* names[HV_TIMING_SLOTS] = {
+      "preprocess",          "filter_events",  "refine",        "kernel_decimate",
+      "kernel_filterbank",   "kernel_interp",  "kernel_refine", "kernel_prune_compact"};
+  for (int k = 0; k < HV_TIMING_SLOTS; ++k) t[names[k]] = ms[k];
+}
+
+double ms_since(Clock::time_point& mark) {
+  auto now = Clock::now();
+  double ms = std::chrono::duration<double, std::milli>(now - mark).count();
+  mark = now;
+  return ms;
+}
+
+void finish_utterance(const CandidateRows& rows, int frames, const double* best,
+                      double frame_period, int out_frames, Result& r, Timings* timings,
+                      Clock::time_point* mark) {
+  std::vector<double> contour(frames), basic(frames);
+  fix_contour(rows, frames, best, contour.data());
+  if (timings) (*timings)["contour"] += ms_since(*mark);
+  smooth_contour(contour.data(), frames, basic.data());
+  if (timings) (*timings)["smooth"] += ms_since(*mark);
+  r.f0.resize(out_frames);
+  r.times.resize(out_frames);
+  resample(basic.data(), frames, frame_period, out_frames, r.f0.data(), r.times.data());
+  if (timings) (*timings)["resample"] += ms_since(*mark);
+}
+}  // namespace
+
+Result analyze(hv_workspace* ws, const double* x, int64_t length, int fs, double f0_floor,
+               double f0_ceil, double frame_period, Timings* timings) {
+  Clock::time_point mark = Clock::now();
+  if (timings) timings->clear();
+  const Params p = derive_params(length, fs, f0_floor, f0_ceil, frame_period);
+  HostCache::Lease lease;
+  HostArray<int32_t> offsets(lease.arena());
+  HostArray<double> best(lease.arena()), cf0(lease.arena()), cscore(lease.arena());
+  offsets.allocate(p.frames + 1);
+  best.allocate(p.frames);
+  if (timings) (*timings)["initialize"] = ms_since(mark);
+  int64_t nnz = 0;
+  double device_ms[HV_TIMING_SLOTS];
+  const int64_t len = length;
+  throw_status(hv_analyze_batch(ws, 1, &x, &len, fs, f0_floor, f0_ceil, offsets.data(),
+                                best.data(), &nnz, timings ? device_ms : nullptr));
+  cf0.allocate(nnz);
+  cscore.allocate(nnz);
+  throw_status(hv_fetch_candidates(ws, cf0.data(), cscore.data()));
+  if (timings) {
+    ms_since(mark);
+    record_device(device_ms, *timings);
+  }
+  Result r;
+  CandidateRows rows{offsets.data(), cf0.data(), cscore.data()};
+  finish_utterance(rows, p.frames, best.data(), frame_period, p.out_frames, r, timings, &mark);
+  return r;
+}
+
+namespace {
+template <class Call>
+std::vector<Result> batch_common(hv_workspace* ws, Call&& call,
+                                 const std::vector<int64_t>& lengths, int fs, double f0_floor,
+                                 double f0_ceil, double frame_period, int host_threads,
+                                 Timings* timings) {
+  Clock::time_point mark = Clock::now();
+  if (timings) timings->clear();
+  const int count = (int)lengths.size();
+  std::vector<Params> params(count);
+  std::vector<int64_t> frame_off(count + 1, 0);
+  for (int u = 0; u < count; ++u) {
+    params[u] = derive_params(lengths[u], fs, f0_floor, f0_ceil, frame_period);
+    frame_off[u + 1] = frame_off[u] + params[u].frames;
+  }
+  std::vector<int32_t> offsets(frame_off[count] + 1);
+  std::vector<double> best(frame_off[count]);
+  int64_t nnz = 0;
+  double device_ms[HV_TIMING_SLOTS];
+  throw_status(call(offsets.data(), best.data(), &nnz, timings ? device_ms : nullptr));
+  std::vector<double> cf0(std::max<int64_t>(nnz, 1)), cscore(std::max<int64_t>(nnz, 1));
+  throw_status(hv_fetch_candidates(ws, cf0.data(), cscore.data()));
+  if (timings) {
+    (*timings)["device_call"] = ms_since(mark);
+    record_device(device_ms, *timings);
+  }
+  std::vector<Result> results(count);
+  // Longest utterance first over a small pool of host threads.
+  std::vector<int> order(count);
+  std::iota(order.begin(), order.end(), 0);
+  std::sort(order.begin(), order.end(),
+            [&](int a, int b) { return params[a].frames > params[b].frames; });
+  std::atomic<int> next{0};
+  auto worker = [&] {
+    for (int k = next++; k < count; k = next++) {
+      const int u = order[k];
+      CandidateRows rows{offsets.data() + frame_off[u], cf0.data(), cscore.data()};
+      finish_utterance(rows, params[u].frames, best.data() + frame_off[u], frame_period,
+                       params[u].out_frames, results[u], nullptr, nullptr);
+    }
+  };
+  const int threads = std::max(1, std::min(host_threads, count));
+  std::vector<std::thread> pool;
+  for (int t = 1; t < threads; ++t) pool.emplace_back(worker);
+  worker();
+  for (auto& t : pool) t.join();
+  if (timings) (*timings)["contour"] = ms_since(mark);
+  return results;
+}
+}  // namespace
+
+std::vector<Result> analyze_batch(hv_workspace* ws, const std::vector<const double*>& xs,
+                                  const std::vector<int64_t>& lengths, int fs,
+                                  double f0_floor, double f0_ceil, double frame_period,
+                                  int host_threads, Timings* timings) {
+  auto call = [&](int32_t* offsets, double* best, int64_t* nnz, double* ms) {
+    return hv_analyze_batch(ws, (int)xs.size(), xs.data(), lengths.data(), fs, f0_floor,
+                            f0_ceil, offsets, best, nnz, ms);
+  };
+  return batch_common(ws, call, lengths, fs, f0_floor, f0_ceil, frame_period, host_threads,
+                      timings);
+}
+
+std::vector<Result> analyze_batch_device(hv_workspace* ws, const double* x_device,
+                                         const std::vector<int64_t>& lengths, int fs,
+                                         double f0_floor, double f0_ceil, double frame_period,
+                                         int host_threads, Timings* timings) {
+  auto call = [&](int32_t* offsets, double* best, int64_t* nnz, double* ms) {
+    return hv_analyze_batch_device(ws, (int)lengths.size(), x_device, lengths.data(), fs,
+                                   f0_floor, f0_ceil, offsets, best, nnz, ms);
+  };
+  return batch_common(ws, call, lengths, fs, f0_floor, f0_ceil, frame_period, host_threads,
+                      timings);
+}
+
+}  // namespace harvest

@@ -1,0 +1,69 @@
+// Host engine of the MI355X Harvest (counterpart of the reference's
+// src/engine.h:1-35).  It derives WORLD's analysis parameters, calls the
+// gfx950 backend through its C ABI (include/harvest_hip.h), and runs the
+// sequential contour stages on the host in O(F).
+#pragma once
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/harvest_hip.h"
+#include "host_memory.h"
+
+namespace harvest {
+
+struct Result {
+  std::vector<double> f0, times;
+};
+
+// Nonzero refined candidates in slot order, rows offsets[i]..offsets[i+1].
+struct CandidateRows {
+  const int32_t* offsets;
+  const double* f0;
+  const double* scores;
+};
+
+using Timings = std::map<std::string, double>;
+
+struct Params {
+  int bands, ratio, y_length, fft_size, frames, out_frames, capacity;
+  double fs_work;
+};
+
+// WORLD's parameter derivation (harvest.cpp:1149-1181, 1219-1228).
+Params derive_params(int64_t length, int fs, double f0_floor, double f0_ceil,
+                     double frame_period);
+// Validation of the public entry points (reference src/bindings.cpp:18-39).
+void validate(const double* x, int64_t length, int fs, double f0_floor, double f0_ceil,
+              double frame_period);
+
+// Plain analysis of one waveform on the leased workspace.
+Result analyze(hv_workspace* ws, const double* x, int64_t length, int fs, double f0_floor,
+               double f0_ceil, double frame_period, Timings* timings = nullptr);
+// Batched analysis: one device launch sequence for all utterances, then the
+// host contour of every utterance on a pool of host threads.
+std::vector<Result> analyze_batch(hv_workspace* ws, const std::vector<const double*>& xs,
+                                  const std::vector<int64_t>& lengths, int fs,
+                                  double f0_floor, double f0_ceil, double frame_period,
+                                  int host_threads, Timings* timings = nullptr);
+
+// Batched analysis of waveforms already in device memory (back to back).
+std::vector<Result> analyze_batch_device(hv_workspace* ws, const double* x_device,
+                                         const std::vector<int64_t>& lengths, int fs,
+                                         double f0_floor, double f0_ceil, double frame_period,
+                                         int host_threads, Timings* timings = nullptr);
+
+// Contour stages, exposed for the bit-exactness tests against WORLD.
+// FixF0Contour (harvest.cpp:1027-1044) from SearchF0Base's output `base`.
+void fix_contour(const CandidateRows& rows, int frames, const double* base, double* out);
+// SmoothF0Contour (harvest.cpp:1079-1113), bit-exact, O(F + sections * settle).
+void smooth_contour(const double* f0, int frames, double* out);
+// Resampling to the output frame period (harvest.cpp:1246-1251).
+void resample(const double* basic, int frames, double frame_period, int out_frames,
+              double* f0, double* times);
+
+// Throws std::invalid_argument / std::runtime_error for a failed backend call.
+void throw_status(hv_status status);
+
+}  // namespace harvest

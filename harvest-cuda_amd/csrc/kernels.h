@@ -29,6 +29,8 @@ void launch_quirk(hipStream_t s, const Config& c, const Utt* utt,
                   const double* coef, const int* coef_off, const int* half,
                   double* partial3, double* sums3, Quirk* quirk);
 void launch_fill_int(hipStream_t s, int* p, long long n, int value);
+// Refinement twiddles (cos, sin) of 2 pi q / n for q < n.
+void launch_twiddle(hipStream_t s, double* tw, int n);
 
 // Filterbank + crossing detection for bands [band0, band0 + group).
 void launch_filter_events(hipStream_t s, const Config& c, const Utt* utt,

@@ -352,6 +352,17 @@ __global__ __launch_bounds__(kThreads) void k_quirk_bands(
   quirk[id] = q;
 }
 
+// e^{-2 pi i q / n} is read as (cos, sin)[q] of 2 pi q / n; sincospi takes the
+// exact rational argument 2q/n (n a power of two).
+__global__ __launch_bounds__(kThreads) void k_twiddle(double* tw, int n) {
+  long long q = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (q >= n) return;
+  double sn, cs;
+  sincospi(2.0 * (double)q / n, &sn, &cs);
+  tw[2 * q] = cs;
+  tw[2 * q + 1] = sn;
+}
+
 __global__ __launch_bounds__(kThreads) void k_fill_int(int* p, long long n, int v) {
   long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
   if (i < n) p[i] = v;
@@ -605,7 +616,10 @@ __global__ __launch_bounds__(kThreads) void k_interp(
         double loc = (fe[mid - wlo] + fe[mid + 1 - wlo]) / 2.0 / fs;
         if (loc <= t) lo = mid + 1; else hi = mid;
       }
-      const int kc = min(max(lo, 1), E - 2);
+      // histc clamps to [1, E - 2]; the window bounds hold by construction
+      // (ranks either side of the tile) and are enforced here as well.
+      int kc = min(max(lo, 1), E - 2);
+      kc = min(max(kc, wlo + 1), whi_s[ty] - 1);
       const double e0 = fe[kc - 1 - wlo], e1 = fe[kc - wlo], e2 = fe[kc + 1 - wlo];
       const double x0 = (e0 + e1) / 2.0 / fs, x1 = (e1 + e2) / 2.0 / fs;
       const double y0 = fs / (e1 - e0), y1 = fs / (e2 - e1);
@@ -701,6 +715,7 @@ __global__ __launch_bounds__(kThreads) void k_refine(
       mre[hh] = mim[hh] = dre[hh] = dim[hh] = 0.0;
     }
     const int stride = kTable ? c.twiddle_n / n : 0;
+    const bool table = kTable && stride > 0;
     const double* yu = y + u.y_off;
     auto window = [&](int m) -> double {
       const double tmp = (basic + m - 1.0) / fs - t;
@@ -721,7 +736,7 @@ __global__ __launch_bounds__(kThreads) void k_refine(
       for (int hh = 0; hh < 6; ++hh) {
         if (hh < H) {
           double cs, sn;
-          if (kTable) {
+          if (table) {
             const double* tw = twiddle + 2LL * ph[hh] * stride;
             cs = tw[0];
             sn = tw[1];
@@ -938,6 +953,11 @@ void launch_quirk(hipStream_t s, const Config& c, const Utt* utt,
   k_quirk_bands<<<grid_for((long long)c.count * c.bands), kThreads, 0, s>>>(
       c, utt, sums3, coef, coef_off, half, quirk);
   check_launch("quirk");
+}
+
+void launch_twiddle(hipStream_t s, double* tw, int n) {
+  k_twiddle<<<grid_for(n), kThreads, 0, s>>>(tw, n);
+  check_launch("twiddle");
 }
 
 void launch_fill_int(hipStream_t s, int* p, long long n, int value) {

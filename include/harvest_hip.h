@@ -35,11 +35,17 @@ typedef enum hv_status {
   HV_EINTERNAL = 6     /* anything else                                    */
 } hv_status;
 
-/* Number of stage timings written by hv_analyze_batch when `timings_ms` is
- * not NULL: [0] preprocess (upload, decimation, mean), [1] filter_events
- * (filterbank, crossings, raw and official candidates), [2] refine
- * (refinement, pruning, compaction, download).  Device spans (hipEvents). */
-#define HV_TIMING_SLOTS 3
+/* Timings (milliseconds, hipEvents on the workspace stream) written by the
+ * analysis entry points when `timings_ms` is not NULL.  Stage spans:
+ *   [0] preprocess     upload, decimation, mean removal
+ *   [1] filter_events  two-bin terms, filterbank, crossings, raw and official
+ *                      candidates
+ *   [2] refine         refinement, pruning, compaction, download
+ * Kernel sums inside those spans:
+ *   [3] decimation scan kernels      [4] filterbank + crossing kernel
+ *   [5] tile scan + interpolation    [6] refinement kernel
+ *   [7] prune, best, scan, compaction */
+#define HV_TIMING_SLOTS 8
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* hv_last_error(void);
@@ -99,6 +105,16 @@ hv_status hv_analyze_batch(hv_workspace* ws, int count,
                            int fs, double f0_floor, double f0_ceil,
                            int32_t* offsets, double* best, int64_t* nnz,
                            double* timings_ms);
+
+/* hv_analyze_batch with the waveforms already resident in device memory:
+ * x_device holds the `count` utterances back to back (lengths[u] samples
+ * each) on the workspace's device, and must be complete before the call
+ * (synchronise the producing stream first).  Outputs as hv_analyze_batch.
+ * No reference counterpart: the reference always starts from host memory. */
+hv_status hv_analyze_batch_device(hv_workspace* ws, int count, const double* x_device,
+                                  const int64_t* lengths, int fs, double f0_floor,
+                                  double f0_ceil, int32_t* offsets, double* best,
+                                  int64_t* nnz, double* timings_ms);
 
 /* Copy the nnz candidates of the last hv_analyze / hv_analyze_batch call:
  * f0[nnz] and scores[nnz] in CSR order (frame order, then slot order). */
