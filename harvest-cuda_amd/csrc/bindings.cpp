@@ -124,11 +124,8 @@ PYBIND11_MODULE(_core, m) {
                              bool timings) {
     // Waveforms already resident in HBM (e.g. a torch float64 tensor's
     // data_ptr()); validation of sample values is the caller's business.
-    for (int64_t n : lengths) {
-      double probe = 0.0;
-      validate(&probe, 1, fs, floor, ceil, period);
-      if (n <= 0) throw std::invalid_argument("lengths must be positive");
-    }
+    // Parameter checks only; the samples stay on the device.
+    for (int64_t n : lengths) validate_params(n, fs, floor, ceil, period);
     std::vector<Result> rs;
     Timings t;
     {

@@ -179,8 +179,8 @@ struct hv_workspace {
   DevBuf slab, counts, offs, totals, raw, official, count;
   DevBuf rf0, rsc, pf0, psc, best, row_nnz, offsets, scan_tmp;
   DevBuf utt, c_off, t_off, f_off, p_off, y_off;
-  DevBuf coef, coef_off, half, bounds, twiddle;
-  PinnedBuf h_x, h_meta, h_offsets, h_best, h_cand;
+  DevBuf coef, coef_off, half, bounds, twiddle, work;
+  PinnedBuf h_x, h_meta, h_offsets, h_best, h_cand, h_work;
   // Filter-bank cache key (working rate, floor, ceil).
   double key_fs = -1.0, key_floor = -1.0, key_ceil = -1.0;
   std::vector<double> boundaries;
@@ -188,8 +188,12 @@ struct hv_workspace {
   int max_half = 0;
   int twiddle_n = -1;
   long long last_nnz = -1;
+  int lds_limit = 64 * 1024;
 
   explicit hv_workspace(int dev) : device(dev) {
+    hipDeviceProp_t prop;
+    check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    lds_limit = (int)prop.sharedMemPerBlock;
     check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
     for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
   }
@@ -341,6 +345,7 @@ void fill_config(Config& c, int count, int ratio, int fs, double fs_work, double
   c.capacity = capacity;
   c.slots = 7 * capacity;
   c.twiddle_n = twiddle_n;
+  c.lds_limit = 64 * 1024;
   const double* co = kDecimation[ratio >= 2 && ratio <= 12 ? ratio : 0];
   c.a0 = co[0];
   c.a1 = co[1];
@@ -472,9 +477,11 @@ void analyze_device(hv_workspace* ws, const Plan& p, int32_t* offsets, double* b
   const size_t slots = (size_t)p.total_frames * c.slots;
   double* rf0 = ws->rf0.ensure<double>(slots);
   double* rsc = ws->rsc.ensure<double>(slots);
+  unsigned long long* work = ws->work.ensure<unsigned long long>(HV_STAT_SLOTS);
+  launch_fill_int(s, reinterpret_cast<int*>(work), 2 * HV_STAT_SLOTS, 0);
   timed(ws, 6, [&] {
     launch_refine(s, c, dutt, ws->f_off.get<long long>(), p.total_frames, y, official, count,
-                  ws->twiddle.get<double>(), rf0, rsc);
+                  ws->twiddle.get<double>(), rf0, rsc, work);
   });
   double* pf0 = ws->pf0.ensure<double>(slots);
   double* psc = ws->psc.ensure<double>(slots);
@@ -497,6 +504,9 @@ void analyze_device(hv_workspace* ws, const Plan& p, int32_t* offsets, double* b
                        hipMemcpyDeviceToHost, s), "download offsets");
   check(hipMemcpyAsync(hbest, dbest, p.total_frames * sizeof(double), hipMemcpyDeviceToHost, s),
         "download best");
+  unsigned long long* hwork = ws->h_work.ensure<unsigned long long>(HV_STAT_SLOTS);
+  check(hipMemcpyAsync(hwork, work, HV_STAT_SLOTS * sizeof(unsigned long long),
+                       hipMemcpyDeviceToHost, s), "download counters");
   check(hipEventRecord(ws->ev[3], s), "hipEventRecord");
   check(hipStreamSynchronize(s), "analysis");
   std::memcpy(offsets, hoff, (p.total_frames + 1) * sizeof(int32_t));
@@ -575,6 +585,7 @@ void run_batch(hv_workspace* ws, int count, const double* const* xs, const int64
   const int B = (int)ws->boundaries.size();
   fill_config(p.cfg, count, ratio, fs, fs_work, f0_floor, f0_ceil, B, host_round(B / 10.0),
               ws->twiddle_n);
+  p.cfg.lds_limit = ws->lds_limit;
   layout(p, lens, ns, ffts, frames);
   hipStream_t s = ws->stream;
   start_timers(ws, timings != nullptr);
@@ -719,6 +730,7 @@ hv_status hv_analyze(hv_workspace* ws, const double* y, int64_t y_length, int ff
     Plan p;
     fill_config(p.cfg, 1, den, (int)(fs_work * den), fs_work, f0_floor, f0_ceil, bands, capacity,
                 ws->twiddle_n);
+    p.cfg.lds_limit = ws->lds_limit;
     p.cfg.lag = 0;
     layout(p, {y_length}, {(int)y_length}, {fft_size}, {frames});
     hipStream_t s = ws->stream;
@@ -753,6 +765,15 @@ hv_status hv_analyze_batch_device(hv_workspace* ws, int count, const double* x_d
     if (!x_device || !lengths || !offsets || !best || !nnz) invalid("null pointer");
     const double* xs[1] = {x_device};
     run_batch(ws, count, xs, lengths, fs, f0_floor, f0_ceil, offsets, best, nnz, timings_ms, true);
+  });
+}
+
+hv_status hv_last_stats(hv_workspace* ws, int64_t* stats) {
+  return guarded([&] {
+    if (!ws || !stats) invalid("null pointer");
+    if (ws->last_nnz < 0) invalid("no analysis result");
+    const unsigned long long* h = ws->h_work.get<unsigned long long>();
+    for (int k = 0; k < HV_STAT_SLOTS; ++k) stats[k] = h ? (int64_t)h[k] : 0;
   });
 }
 

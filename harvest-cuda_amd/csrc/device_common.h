@@ -51,6 +51,7 @@ struct Config {
   int capacity;      // official candidates per frame
   int slots;         // 7 * capacity
   int twiddle_n;     // size of the refinement twiddle table (0: none)
+  int lds_limit;     // dynamic LDS bytes a workgroup may use on this device
   double a0, a1, a2, b0, b1;  // decimation IIR
   double chunk_pow[9];        // A^kChunk of the IIR state transition
 };

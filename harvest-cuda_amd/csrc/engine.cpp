@@ -258,8 +258,8 @@ Params derive_params(int64_t length, int fs, double f0_floor, double f0_ceil,
   return p;
 }
 
-void validate(const double* x, int64_t length, int fs, double f0_floor, double f0_ceil,
-              double frame_period) {
+void validate_params(int64_t length, int fs, double f0_floor, double f0_ceil,
+                     double frame_period) {
   if (length <= 0 || length > INT_MAX)
     throw std::invalid_argument("x must be nonempty and contain at most INT_MAX samples");
   if (fs < 8000 || fs > 192000)
@@ -277,6 +277,11 @@ void validate(const double* x, int64_t length, int fs, double f0_floor, double f
   if (1000.0 * length / fs / frame_period >= INT_MAX || 1000.0 * length / fs >= INT_MAX ||
       (double)length / ratio + 10.0 * fs / ratio / f0_floor >= (1 << 28))
     throw std::invalid_argument("input or analysis settings exceed supported allocation sizes");
+}
+
+void validate(const double* x, int64_t length, int fs, double f0_floor, double f0_ceil,
+              double frame_period) {
+  validate_params(length, fs, f0_floor, f0_ceil, frame_period);
   bool finite = true;
   for (int64_t i = 0; i < length; ++i) finite &= std::isfinite(x[i]);
   if (!finite) throw std::invalid_argument("x must contain only finite samples");
@@ -369,6 +374,14 @@ void record_device(const double* ms, Timings& t) {
   for (int k = 0; k < HV_TIMING_SLOTS; ++k) t[names[k]] = ms[k];
 }
 
+void record_stats(hv_workspace* ws, Timings& t) {
+  int64_t st[HV_STAT_SLOTS];
+  if (hv_last_stats(ws, st) != HV_OK) return;
+  t["work_window_samples"] = (double)st[0];
+  t["work_bin_samples"] = (double)st[1];
+  t["work_bins"] = (double)st[2];
+}
+
 double ms_since(Clock::time_point& mark) {
   auto now = Clock::now();
   double ms = std::chrono::duration<double, std::milli>(now - mark).count();
@@ -413,6 +426,7 @@ Result analyze(hv_workspace* ws, const double* x, int64_t length, int fs, double
   if (timings) {
     ms_since(mark);
     record_device(device_ms, *timings);
+    record_stats(ws, *timings);
   }
   Result r;
   CandidateRows rows{offsets.data(), cf0.data(), cscore.data()};
@@ -445,6 +459,7 @@ std::vector<Result> batch_common(hv_workspace* ws, Call&& call,
   if (timings) {
     (*timings)["device_call"] = ms_since(mark);
     record_device(device_ms, *timings);
+    record_stats(ws, *timings);
   }
   std::vector<Result> results(count);
   // Longest utterance first over a small pool of host threads.

@@ -34,7 +34,10 @@ struct Params {
 // WORLD's parameter derivation (harvest.cpp:1149-1181, 1219-1228).
 Params derive_params(int64_t length, int fs, double f0_floor, double f0_ceil,
                      double frame_period);
-// Validation of the public entry points (reference src/bindings.cpp:18-39).
+// Validation of the public entry points (reference src/bindings.cpp:18-39);
+// validate_params checks everything but the sample values.
+void validate_params(int64_t length, int fs, double f0_floor, double f0_ceil,
+                     double frame_period);
 void validate(const double* x, int64_t length, int fs, double f0_floor, double f0_ceil,
               double frame_period);
 

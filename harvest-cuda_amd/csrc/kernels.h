@@ -54,7 +54,8 @@ void launch_official(hipStream_t s, const Config& c, const long long* f_off,
 void launch_refine(hipStream_t s, const Config& c, const Utt* utt,
                    const long long* f_off, long long total_frames,
                    const double* y, const double* official, const int* count,
-                   const double* twiddle, double* rf0, double* rscore);
+                   const double* twiddle, double* rf0, double* rscore,
+                   unsigned long long* work);
 void launch_prune_best(hipStream_t s, const Config& c, const Utt* utt,
                        const long long* f_off, long long total_frames,
                        const int* count, const double* rf0,

@@ -775,6 +775,363 @@ __global__ __launch_bounds__(kThreads) void k_refine(
   rscore[gf * c.slots + s] = score;
 }
 
+// Refinement, one workgroup per frame.  The same numbers as k_refine, with
+// the work shared: the windowed samples depend only on (frame, h), so each
+// distinct half length h of the frame gets its Blackman and derivative
+// windows (and the two cosines per sample) computed once into LDS; and a DFT
+// bin depends only on (h, k), so duplicate (h, k) pairs among the frame's
+// slots x harmonics are evaluated once (LDS hash).  Each bin is one thread's
+// sequential sum over the window in increasing sample order, exactly as in
+// k_refine, so both kernels give bitwise identical results.
+constexpr int kHash = 1024;          // (h, k) hash slots per frame
+constexpr unsigned long long kEmpty = ~0ull;
+
+struct RefineLds {  // carve-up of the dynamic LDS of k_refine_frames
+  double* t_res;    // [2 * 6S] power and IF numerator of each distinct bin
+  double* f0;       // [S] candidate per slot (0: inactive)
+  double* tw;       // [2 * kTwLds] twiddles (when the table is small enough)
+  double* wv;       // [BW] window values of the current round
+  double* xw;       // [2 BW] (x w, x d) of the current round
+  unsigned long long* keys;  // [kHash]
+  int* vals;        // [kHash]
+  int* h;           // [S] half window length per slot
+  int* win;         // [S] distinct-window index of each slot
+  int* tref;        // [6S] hash position of each (slot, harmonic)
+  int* w_h;         // [S] half length of each distinct window
+  int* w_off;       // [S] LDS offset of each window in its round
+  int* w_round;     // [S] round of each window (-1: too long for LDS)
+  int* w_cnt;       // [S] distinct bins of each window
+  int* w_first;     // [S] first position of the window's bins in t_order
+  int* r_first;     // [S + 1] first t_order position of each round
+  int* t_win;       // [6S] window of each distinct bin
+  int* t_k;         // [6S] bin index of each distinct bin
+  int* t_rank;      // [6S] rank of the bin inside its window
+  int* t_order;     // [6S] distinct bins sorted by window
+  int* misc;        // [8] counters
+};
+
+constexpr int kTwLds = 1024;  // twiddle entries staged in LDS
+constexpr int kSplit = 8;     // lanes sharing one DFT bin
+
+// Walks the carve-up; with base == nullptr it only measures.
+__host__ __device__ inline size_t carve_refine_lds(void* base, int S, int BW, RefineLds* l) {
+  size_t at = 0;
+  auto take = [&](size_t bytes) -> char* {
+    char* r = base ? static_cast<char*>(base) + at : nullptr;
+    at += (bytes + 15) & ~size_t(15);
+    return r;
+  };
+  RefineLds t;
+  t.t_res = (double*)take(sizeof(double) * 12 * S);
+  t.f0 = (double*)take(sizeof(double) * S);
+  t.tw = (double*)take(sizeof(double) * 2 * kTwLds);
+  t.wv = (double*)take(sizeof(double) * BW);
+  t.xw = (double*)take(sizeof(double) * 2 * BW);
+  t.keys = (unsigned long long*)take(sizeof(unsigned long long) * kHash);
+  t.vals = (int*)take(sizeof(int) * kHash);
+  t.h = (int*)take(sizeof(int) * S);
+  t.win = (int*)take(sizeof(int) * S);
+  t.tref = (int*)take(sizeof(int) * 6 * S);
+  t.w_h = (int*)take(sizeof(int) * S);
+  t.w_off = (int*)take(sizeof(int) * S);
+  t.w_round = (int*)take(sizeof(int) * S);
+  t.w_cnt = (int*)take(sizeof(int) * S);
+  t.w_first = (int*)take(sizeof(int) * S);
+  t.r_first = (int*)take(sizeof(int) * (S + 1));
+  t.t_win = (int*)take(sizeof(int) * 6 * S);
+  t.t_k = (int*)take(sizeof(int) * 6 * S);
+  t.t_rank = (int*)take(sizeof(int) * 6 * S);
+  t.t_order = (int*)take(sizeof(int) * 6 * S);
+  t.misc = (int*)take(sizeof(int) * 8);
+  if (l) *l = t;
+  return at;
+}
+
+__device__ __forceinline__ double blackman(int basic, int m, double fs, double t, double wl) {
+  const double tmp = (basic + m - 1.0) / fs - t;
+  return 0.42 + 0.5 * cos(2.0 * kPi * tmp / wl) + 0.08 * cos(4.0 * kPi * tmp / wl);
+}
+
+__global__ __launch_bounds__(kThreads) void k_refine_frames(
+    Config c, const Utt* utt, const long long* f_off, int BW, const double* y,
+    const double* official, const int* count, const double* twiddle, double* rf0,
+    double* rscore, unsigned long long* work) {
+  extern __shared__ double lds_raw[];
+  const int S = c.slots;
+  RefineLds L;
+  carve_refine_lds(lds_raw, S, BW, &L);
+  const long long gf = blockIdx.x;
+  const int ui = find_segment(f_off, c.count, gf);
+  const int cnt = count[ui];
+  const int ncand = 7 * cnt;
+  if (ncand == 0) return;
+  const Utt u = utt[ui];
+  const long long i = gf - u.f_off;
+  const double fs = c.fs_work;
+  const double t = (double)i / 1000.0;
+  const double* yu = y + u.y_off;
+  const int tid = threadIdx.x;
+  const bool tw_lds = c.twiddle_n > 0 && c.twiddle_n <= kTwLds;
+
+  for (int k = tid; k < kHash; k += kThreads) L.keys[k] = kEmpty;
+  for (int k = tid; k < S; k += kThreads) L.w_cnt[k] = 0;
+  if (tw_lds)
+    for (int k = tid; k < 2 * c.twiddle_n; k += kThreads) L.tw[k] = twiddle[k];
+  if (tid < 8) L.misc[tid] = 0;
+  // Slot candidates (OverlapF0Candidates layout) and half window lengths.
+  for (int s = tid; s < ncand; s += kThreads) {
+    const int grp = s / cnt, jj = s % cnt;
+    long long src = i;
+    if (grp >= 1 && grp <= 3) src = i - grp;
+    else if (grp >= 4) src = i + (grp - 3);
+    const double f0 = (src >= 0 && src < u.frames) ? official[(u.f_off + src) * c.capacity + jj]
+                                                   : 0.0;
+    L.f0[s] = f0;
+    L.h[s] = f0 > 0.0 ? (int)(1.5 * fs / f0 + 1.0) : 0;
+  }
+  __syncthreads();
+  // Distinct windows: the first slot with a given h owns it.
+  for (int s = tid; s < ncand; s += kThreads) {
+    const int hs = L.h[s];
+    int owner = -1;
+    if (hs > 0) {
+      owner = s;
+      for (int q = 0; q < s; ++q)
+        if (L.h[q] == hs) {
+          owner = q;
+          break;
+        }
+    }
+    if (owner == s) {
+      const int w = atomicAdd(&L.misc[0], 1);
+      L.win[s] = w;
+      L.w_h[w] = hs;
+    }
+    L.tref[6 * s] = owner;  // temporary: owner slot
+  }
+  __syncthreads();
+  for (int s = tid; s < ncand; s += kThreads) {
+    const int owner = L.tref[6 * s];
+    if (owner >= 0 && owner != s) L.win[s] = L.win[owner];
+  }
+  __syncthreads();
+  // Distinct bins: insert (h, k) of every (slot, harmonic) into the hash.
+  for (int s = tid; s < ncand; s += kThreads) {
+    const double f0 = L.f0[s];
+    const int hs = L.h[s];
+    if (hs <= 0) continue;
+    const int n = 1 << (2 + (31 - __clz(2 * hs + 1)));
+    const int H = min((int)(fs / 2.0 / f0), 6);
+    for (int hh = 0; hh < H; ++hh) {
+      const int k = mround(f0 * n / fs * (hh + 1));
+      const unsigned long long key = ((unsigned long long)(unsigned)hs << 32) | (unsigned)k;
+      unsigned pos = (unsigned)((key * 0x9E3779B97F4A7C15ull) >> 54) & (kHash - 1);
+      while (true) {
+        const unsigned long long prev = atomicCAS(&L.keys[pos], kEmpty, key);
+        if (prev == kEmpty) {
+          const int tix = atomicAdd(&L.misc[1], 1);
+          const int w = L.win[s];
+          L.vals[pos] = tix;
+          L.t_win[tix] = w;
+          L.t_k[tix] = k;
+          L.t_rank[tix] = atomicAdd(&L.w_cnt[w], 1);
+          break;
+        }
+        if (prev == key) break;
+        pos = (pos + 1) & (kHash - 1);
+      }
+      L.tref[6 * s + hh] = (int)pos;
+    }
+  }
+  __syncthreads();
+  const int nwin = L.misc[0], ntask = L.misc[1];
+  // Rounds of consecutive windows packed into the BW-sample buffer, and the
+  // bins ordered by window so that a round's bins are contiguous.
+  if (tid == 0) {
+    int round = 0, used = 0, first = 0;
+    for (int w = 0; w < nwin; ++w) {
+      L.w_first[w] = first;
+      first += L.w_cnt[w];
+      const int len = 2 * L.w_h[w] + 1;
+      if (len > BW) {
+        L.w_round[w] = -1;
+        continue;
+      }
+      if (used + len > BW) {
+        L.r_first[++round] = L.w_first[w];
+        used = 0;
+      } else if (used == 0 && round == 0) {
+        L.r_first[0] = L.w_first[w];
+      }
+      L.w_round[w] = round;
+      L.w_off[w] = used;
+      used += len;
+    }
+    const int rounds = used > 0 || round > 0 ? round + 1 : 0;
+    L.r_first[rounds] = first;
+    L.misc[2] = rounds;
+  }
+  __syncthreads();
+  for (int tk = tid; tk < ntask; tk += kThreads)
+    L.t_order[L.w_first[L.t_win[tk]] + L.t_rank[tk]] = tk;
+  __syncthreads();
+  const int rounds = L.misc[2];
+  const int grp = tid / kSplit, sub = tid % kSplit;
+  for (int r = 0; r < rounds; ++r) {
+    // Window values, then windowed samples, of every window of this round.
+    for (int w = 0; w < nwin; ++w) {
+      if (L.w_round[w] != r) continue;
+      const int hw = L.w_h[w], len = 2 * hw + 1, off = L.w_off[w];
+      const double wl = (2.0 * hw + 1.0) / fs;
+      const int basic = mround((t + (-hw) / fs) * fs + 0.001);
+      for (int m = tid; m < len; m += kThreads) L.wv[off + m] = blackman(basic, m, fs, t, wl);
+    }
+    __syncthreads();
+    for (int w = 0; w < nwin; ++w) {
+      if (L.w_round[w] != r) continue;
+      const int hw = L.w_h[w], len = 2 * hw + 1, off = L.w_off[w];
+      const int basic = mround((t + (-hw) / fs) * fs + 0.001);
+      for (int m = tid; m < len; m += kThreads) {
+        double d;
+        if (m == 0) d = -L.wv[off + 1] / 2.0;
+        else if (m == len - 1) d = L.wv[off + len - 2] / 2.0;
+        else d = -(L.wv[off + m + 1] - L.wv[off + m - 1]) / 2.0;
+        int idx = basic + m - 1;
+        idx = idx < 0 ? 0 : (idx > u.n - 1 ? u.n - 1 : idx);
+        const double xv = yu[idx];
+        L.xw[2 * (off + m)] = xv * L.wv[off + m];
+        L.xw[2 * (off + m) + 1] = xv * d;
+      }
+    }
+    __syncthreads();
+    // Each distinct bin is summed by kSplit lanes (samples m = sub mod kSplit)
+    // and reduced by a fixed butterfly, so the result does not depend on the
+    // batch or on scheduling.
+    const int p_end = L.r_first[r + 1];
+    for (int p = L.r_first[r] + grp; p < p_end; p += kThreads / kSplit) {
+      const int tk = L.t_order[p];
+      const int w = L.t_win[tk];
+      if (L.w_round[w] != r) continue;
+      const int hw = L.w_h[w], len = 2 * hw + 1, off = L.w_off[w];
+      const int n = 1 << (2 + (31 - __clz(len)));
+      const int k = L.t_k[tk];
+      const int stride = c.twiddle_n / n;
+      const int step = (k * kSplit) & (n - 1);
+      int ph = (int)(((long long)k * sub) & (n - 1));
+      double mre = 0.0, mim = 0.0, dre = 0.0, dim = 0.0;
+      for (int m = sub; m < len; m += kSplit) {
+        const double xm = L.xw[2 * (off + m)], xd = L.xw[2 * (off + m) + 1];
+        double cs, sn;
+        if (stride > 0) {
+          const double* tw = (tw_lds ? L.tw : twiddle) + 2 * ph * stride;
+          cs = tw[0];
+          sn = tw[1];
+        } else {
+          sincospi(2.0 * (double)ph / n, &sn, &cs);
+        }
+        mre = fma(xm, cs, mre);
+        mim = fma(-xm, sn, mim);
+        dre = fma(xd, cs, dre);
+        dim = fma(-xd, sn, dim);
+        ph = (ph + step) & (n - 1);
+      }
+#pragma unroll
+      for (int d = kSplit / 2; d > 0; d >>= 1) {
+        mre += __shfl_xor(mre, d, kSplit);
+        mim += __shfl_xor(mim, d, kSplit);
+        dre += __shfl_xor(dre, d, kSplit);
+        dim += __shfl_xor(dim, d, kSplit);
+      }
+      if (sub == 0) {
+        L.t_res[2 * tk + 0] = mre * mre + mim * mim;
+        L.t_res[2 * tk + 1] = mre * dim - mim * dre;
+      }
+    }
+    __syncthreads();
+  }
+  // Windows longer than the LDS buffer: windowed samples on the fly.
+  for (int tk = tid; tk < ntask; tk += kThreads) {
+    const int w = L.t_win[tk];
+    if (L.w_round[w] >= 0) continue;
+    const int hw = L.w_h[w], len = 2 * hw + 1;
+    const int n = 1 << (2 + (31 - __clz(len)));
+    const int k = L.t_k[tk];
+    const double wl = (2.0 * hw + 1.0) / fs;
+    const int basic = mround((t + (-hw) / fs) * fs + 0.001);
+    const int stride = c.twiddle_n / n;
+    double mre = 0.0, mim = 0.0, dre = 0.0, dim = 0.0;
+    double wprev = 0.0, wcur = blackman(basic, 0, fs, t, wl);
+    int ph = 0;
+    for (int m = 0; m < len; ++m) {
+      const double wnext = (m + 1 < len) ? blackman(basic, m + 1, fs, t, wl) : 0.0;
+      double d;
+      if (m == 0) d = -wnext / 2.0;
+      else if (m == len - 1) d = wprev / 2.0;
+      else d = -(wnext - wprev) / 2.0;
+      int idx = basic + m - 1;
+      idx = idx < 0 ? 0 : (idx > u.n - 1 ? u.n - 1 : idx);
+      const double xv = yu[idx];
+      const double xm = xv * wcur, xd = xv * d;
+      double cs, sn;
+      if (stride > 0) {
+        const double* tw = twiddle + 2LL * ph * stride;
+        cs = tw[0];
+        sn = tw[1];
+      } else {
+        sincospi(2.0 * (double)ph / n, &sn, &cs);
+      }
+      mre = fma(xm, cs, mre);
+      mim = fma(-xm, sn, mim);
+      dre = fma(xd, cs, dre);
+      dim = fma(-xd, sn, dim);
+      ph = (ph + k) & (n - 1);
+      wprev = wcur;
+      wcur = wnext;
+    }
+    L.t_res[2 * tk + 0] = mre * mre + mim * mim;
+    L.t_res[2 * tk + 1] = mre * dim - mim * dre;
+  }
+  __syncthreads();
+  // FixF0 per slot (harvest.cpp:507-536) and the gate of GetRefinedF0.
+  for (int s = tid; s < ncand; s += kThreads) {
+    const double f0 = L.f0[s];
+    const int hs = L.h[s];
+    double refined = 0.0, score = 0.0;
+    if (hs > 0) {
+      const int n = 1 << (2 + (31 - __clz(2 * hs + 1)));
+      const int H = min((int)(fs / 2.0 / f0), 6);
+      double numer = 0.0, den = 0.0, dev = 0.0;
+      for (int hh = 0; hh < H; ++hh) {
+        const int tk = L.vals[L.tref[6 * s + hh]];
+        const double pw = L.t_res[2 * tk], nu = L.t_res[2 * tk + 1];
+        const int k = L.t_k[tk];
+        const double ifr = pw == 0.0 ? 0.0 : (double)k * fs / n + nu / pw * fs / 2.0 / kPi;
+        const double amp = sqrt(pw);
+        numer += amp * ifr;
+        den += amp * (hh + 1.0);
+        dev += fabs((ifr / (hh + 1.0) - f0) / f0);
+      }
+      refined = numer / (den + kGuard);
+      score = 1.0 / (dev / H + kGuard);
+      if (refined < c.f0_floor || refined > c.f0_ceil || score < 2.5) {
+        refined = 0.0;
+        score = 0.0;
+      }
+    }
+    rf0[gf * c.slots + s] = refined;
+    rscore[gf * c.slots + s] = score;
+  }
+  // Work actually performed: distinct window samples and bin-samples.
+  if (tid == 0 && work) {
+    unsigned long long wsamp = 0, bsamp = 0;
+    for (int w = 0; w < nwin; ++w) wsamp += 2ull * L.w_h[w] + 1;
+    for (int tk = 0; tk < ntask; ++tk) bsamp += 2ull * L.w_h[L.t_win[tk]] + 1;
+    atomicAdd(&work[0], wsamp);
+    atomicAdd(&work[1], bsamp);
+    atomicAdd(&work[2], (unsigned long long)ntask);
+  }
+}
+
 // RemoveUnreliableCandidates (harvest.cpp:652-688) against the pre-prune
 // values of frames i +- 1.  min_k fl(|r - c_k| / r) = fl(min_k |r - c_k| / r)
 // because rounding is monotone, so one division per neighbour frame suffices.
@@ -1016,7 +1373,22 @@ void launch_official(hipStream_t s, const Config& c, const long long* f_off,
 void launch_refine(hipStream_t s, const Config& c, const Utt* utt,
                    const long long* f_off, long long total_frames,
                    const double* y, const double* official, const int* count,
-                   const double* twiddle, double* rf0, double* rscore) {
+                   const double* twiddle, double* rf0, double* rscore,
+                   unsigned long long* work) {
+  // Frame-per-workgroup kernel when its LDS (at most 80 KiB, two
+  // workgroups per 160 KiB CU) leaves a useful window buffer and the (h, k)
+  // hash stays below 75 % load.
+  const size_t fixed = carve_refine_lds(nullptr, c.slots, 0, nullptr);
+  const long long limit = std::min(80 * 1024, c.lds_limit);
+  const long long budget = limit - (long long)fixed - 64;
+  const int bw = budget > 0 ? (int)(budget / (3 * sizeof(double))) : 0;
+  if (6 * c.slots <= (kHash * 3) / 4 && bw >= 512) {
+    const size_t bytes = carve_refine_lds(nullptr, c.slots, bw, nullptr);
+    k_refine_frames<<<(unsigned)total_frames, kThreads, bytes, s>>>(
+        c, utt, f_off, bw, y, official, count, twiddle, rf0, rscore, work);
+    check_launch("refine_frames");
+    return;
+  }
   const long long n = total_frames * c.slots;
   if (c.twiddle_n > 0)
     k_refine<true><<<grid_for(n), kThreads, 0, s>>>(c, utt, f_off, total_frames, y,

@@ -80,11 +80,16 @@ def _voiced_unvoiced(rng: np.random.Generator, n: int, fs: int, snr_db: float,
             amp = rng.uniform(0.5, 1.0, harmonics) / np.arange(1, harmonics + 1)
             theta = rng.uniform(0, 2 * np.pi, harmonics)
             seg = np.zeros(length)
+            # e^{i k phi} by repeated multiplication: one complex exponential
+            # per sample instead of one sine per harmonic.
+            base = np.exp(1j * phi)
+            rot = np.ones(length, dtype=np.complex128)
             for k in range(1, harmonics + 1):
+                rot *= base
                 mask = k * f < fs / 2
                 if not mask.any():
                     break
-                seg += np.where(mask, amp[k - 1] * np.sin(k * phi + theta[k - 1]), 0.0)
+                seg += np.where(mask, amp[k - 1] * (rot * np.exp(1j * theta[k - 1])).imag, 0.0)
             sigma = np.sqrt(np.mean(seg ** 2) / 10 ** (snr_db / 10))
             out[pos:pos + length] = seg + rng.normal(0, sigma, length)
         else:

@@ -47,6 +47,13 @@ typedef enum hv_status {
  *   [7] prune, best, scan, compaction */
 #define HV_TIMING_SLOTS 8
 
+/* Work counters of the last analysis call (hv_last_stats):
+ *   [0] refinement window samples computed (distinct (frame, h) windows)
+ *   [1] refinement bin-samples: one term of the main and of the derivative
+ *       spectrum DFT at one bin (distinct (frame, h, k) bins)
+ *   [2] distinct refinement bins */
+#define HV_STAT_SLOTS 3
+
 /* Message of the last failed call on this thread ("" if none). */
 const char* hv_last_error(void);
 
@@ -115,6 +122,10 @@ hv_status hv_analyze_batch_device(hv_workspace* ws, int count, const double* x_d
                                   const int64_t* lengths, int fs, double f0_floor,
                                   double f0_ceil, int32_t* offsets, double* best,
                                   int64_t* nnz, double* timings_ms);
+
+/* Work counters (HV_STAT_SLOTS entries) of the last analysis on `ws`, for
+ * roofline accounting.  No reference counterpart. */
+hv_status hv_last_stats(hv_workspace* ws, int64_t* stats);
 
 /* Copy the nnz candidates of the last hv_analyze / hv_analyze_batch call:
  * f0[nnz] and scores[nnz] in CSR order (frame order, then slot order). */
