@@ -1,0 +1,134 @@
+"""Drop-in API surface, validation and the C ABI (CPU; no GPU calls).
+
+Mirrors the reference's tests/test_compatibility.py:60-142 (argument
+validation, wrong array types, device arguments, GPU-required behaviour,
+clearing an empty cache)."""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+@pytest.mark.parametrize("kwargs", [
+    {"fs": 0}, {"f0_floor": 0}, {"f0_floor": 800},
+    {"f0_ceil": float("inf")}, {"f0_ceil": 5000},
+    {"frame_period": 0}, {"frame_period": float("nan")},
+])
+def test_invalid_parameters(core, kwargs):
+    params = {"fs": 24000, **kwargs}
+    with pytest.raises(ValueError):
+        core.harvest(np.zeros(2400), **params)
+
+
+@pytest.mark.parametrize("x", [np.zeros(0), np.array([np.nan, 0.0]), np.array([np.inf, 0.0]),
+                               np.zeros(5)])
+def test_invalid_samples(core, x):
+    with pytest.raises(ValueError):
+        core.harvest(x, 24000)
+
+
+@pytest.mark.parametrize("x", [np.zeros(100, dtype=np.float32), np.zeros((2, 100)),
+                               np.zeros(100)[::2], [0.0] * 100])
+def test_rejects_wrong_array_type(core, x):
+    with pytest.raises(TypeError):
+        core.harvest(x, 24000)
+
+
+def test_gpu_required(core):
+    if core.cuda_available():
+        pytest.skip("GPU available")
+    with pytest.raises(RuntimeError):
+        core.harvest(np.zeros(2400), 24000)
+
+
+def test_device_argument_validation(core):
+    x = np.zeros(2400)
+    assert not core.cuda_available(device=-1)
+    with pytest.raises(ValueError, match="nonnegative"):
+        core.harvest(x, 24000, device=-1)
+    with pytest.raises(TypeError):
+        core.harvest(x, 24000, 71.0, 800.0, 5.0, 0)
+    with pytest.raises(TypeError):
+        core.harvest(x, 24000, device="cuda:0")
+
+
+def test_batch_validation(core):
+    with pytest.raises(TypeError):
+        core.harvest_batch([np.zeros(2400, dtype=np.float32)], 24000)
+    with pytest.raises(ValueError):
+        core.harvest_batch([np.zeros(2400), np.array([np.nan] * 10)], 24000)
+    assert core.harvest_batch([], 24000) == []
+
+
+def test_clear_empty_cache(core):
+    from harvest_cuda import clear_cache
+    clear_cache()
+    clear_cache()
+    assert not core._host_cache_allocated()
+    assert core._cached_workspaces() == 0
+
+
+def test_public_exports():
+    import harvest_cuda
+    for name in ("harvest", "harvest_batch", "cuda_available", "clear_cache", "__version__"):
+        assert hasattr(harvest_cuda, name)
+
+
+def test_params_match_oracle(core, oracle):
+    for length, fs, lo, hi, period in [(80000, 16000, 71, 800, 5), (2880000, 48000, 71, 800, 5),
+                                       (96000, 24000, 50, 900, 1), (44100, 44100, 65, 1000, 7.3),
+                                       (6, 24000, 71, 800, 5), (79380000, 44100, 71, 800, 5)]:
+        a = core._params(length, fs, lo, hi, period)
+        b = oracle.params(length, fs, lo, hi, period)
+        for k in ("bands", "ratio", "y_length", "fft_size", "frames", "out_frames", "capacity"):
+            assert a[k] == getattr(b, k), (k, length, fs)
+        assert a["fs_work"] == b.fs_work
+
+
+def _declared_symbols():
+    text = (ROOT / "include" / "harvest_hip.h").read_text()
+    return sorted(set(re.findall(r"\b(hv_[a-z_]+)\s*\(", text)))
+
+
+def test_c_abi_exports_every_declared_symbol():
+    lib = ctypes.CDLL(str(ROOT / "harvest-cuda_amd" / "harvest_cuda" / "libharvest_hip.so"))
+    names = _declared_symbols()
+    assert len(names) >= 10
+    for name in names:
+        assert hasattr(lib, name), name
+
+
+def test_c_abi_errors_without_gpu(core):
+    """Entry points report failures through statuses and hv_last_error."""
+    lib = ctypes.CDLL(str(ROOT / "harvest-cuda_amd" / "harvest_cuda" / "libharvest_hip.so"))
+    lib.hv_last_error.restype = ctypes.c_char_p
+    ws = ctypes.c_void_p()
+    assert lib.hv_workspace_create(-1, ctypes.byref(ws)) == 1  # HV_EINVAL
+    assert b"nonnegative" in lib.hv_last_error()
+    assert lib.hv_device_supported(-1) == 0
+    assert lib.hv_fetch_candidates(None, None, None) == 1
+    if not core.cuda_available():
+        assert lib.hv_workspace_create(0, ctypes.byref(ws)) == 2  # HV_ENODEV
+        assert lib.hv_device_supported(0) == 0
+
+
+def test_metrics_compare():
+    from harvest_cuda.metrics import compare
+    for invalid in (np.nan, np.inf, -np.inf, -1.0):
+        for side in ("reference", "actual"):
+            ref, act = np.array([0.0, 100.0]), np.array([0.0, 100.0])
+            (ref if side == "reference" else act)[0] = invalid
+            with pytest.raises(ValueError, match=rf"{side} F0.*finite, nonnegative"):
+                compare(ref, act)
+    a = compare(np.zeros(4), np.zeros(4))
+    assert a["vuv_disagreement"] == 0 and a["cents_p99"] is None
+    a = compare(np.array([0.0, 100.0, 200.0]), np.array([100.0, 200.0, 200.0]))
+    assert a["vuv_disagreement"] == pytest.approx(1 / 3)
+    assert a["joint_voiced_frames"] == 2
+    assert a["cents_max"] == pytest.approx(1200)
+    assert a["cents_p99"] == pytest.approx(1188)
+    assert a["over_600_cents"] == 1
