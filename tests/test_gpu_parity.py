@@ -168,16 +168,17 @@ def test_device_resident_entry_matches(gpu):
 
 
 def test_deterministic_and_power_of_two_scale_invariant(gpu):
-    """Scaling by a power of two is exact through every linear stage, so the
-    F0 track must be bitwise unchanged; repeated calls are bitwise equal."""
+    """Repeated calls are bitwise equal.  Scaling by a power of two is exact
+    through every linear stage; only FixF0's additive 1e-12 guards
+    (harvest.cpp:531-532) see the scale, so the track moves by ~1e-13 and
+    every voicing decision stays put."""
     x, fs = config1()
     a, _ = gpu.harvest(x, fs)
     b, _ = gpu.harvest(x, fs)
-    c, _ = gpu.harvest(x * 4.0, fs)
-    d, _ = gpu.harvest(x / 1024.0, fs)
     np.testing.assert_array_equal(a, b)
-    np.testing.assert_array_equal(a, c)
-    np.testing.assert_array_equal(a, d)
+    for scale in (4.0, 1 / 1024.0):
+        c, _ = gpu.harvest(x * scale, fs)
+        assert_parity(a, c, 1e-9)
 
 
 def test_config1_matches_oracle(gpu, oracle):
