@@ -188,6 +188,7 @@ struct hv_workspace {
   int max_half = 0;
   int twiddle_n = -1;
   long long last_nnz = -1;
+  long long last_frames = 0;
   int lds_limit = 64 * 1024;
 
   explicit hv_workspace(int dev) : device(dev) {
@@ -509,9 +510,21 @@ void analyze_device(hv_workspace* ws, const Plan& p, int32_t* offsets, double* b
                        hipMemcpyDeviceToHost, s), "download counters");
   check(hipEventRecord(ws->ev[3], s), "hipEventRecord");
   check(hipStreamSynchronize(s), "analysis");
-  std::memcpy(offsets, hoff, (p.total_frames + 1) * sizeof(int32_t));
-  std::memcpy(best, hbest, p.total_frames * sizeof(double));
-  ws->last_nnz = hoff[p.total_frames];
+  // The candidates go straight to the pinned staging buffer, where the host
+  // contour reads them in place (hv_result_view) or callers copy them out.
+  const long long n = hoff[p.total_frames];
+  if (n > 0) {
+    double* hc = ws->h_cand.ensure<double>(2 * n);
+    check(hipMemcpyAsync(hc, rf0, n * sizeof(double), hipMemcpyDeviceToHost, s),
+          "download candidates");
+    check(hipMemcpyAsync(hc + n, rsc, n * sizeof(double), hipMemcpyDeviceToHost, s),
+          "download candidates");
+    check(hipStreamSynchronize(s), "download candidates");
+  }
+  if (offsets) std::memcpy(offsets, hoff, (p.total_frames + 1) * sizeof(int32_t));
+  if (best) std::memcpy(best, hbest, p.total_frames * sizeof(double));
+  ws->last_nnz = n;
+  ws->last_frames = p.total_frames;
   *nnz = ws->last_nnz;
   if (timings) {
     float ms[3];
@@ -714,7 +727,7 @@ hv_status hv_analyze(hv_workspace* ws, const double* y, int64_t y_length, int ff
                      double f0_floor, double f0_ceil, int capacity, int32_t* offsets,
                      double* best, int64_t* nnz, double* timings_ms) {
   return guarded([&] {
-    if (!ws || !y || !boundaries || !offsets || !best || !nnz) invalid("null pointer");
+    if (!ws || !y || !boundaries || !nnz) invalid("null pointer");
     if (y_length < 2 || y_length > 0x7fffffffLL || bands < 3 || frames < 1 || capacity < 1)
       invalid("invalid analysis sizes");
     if (fft_size < y_length || (fft_size & (fft_size - 1)) != 0)
@@ -750,7 +763,7 @@ hv_status hv_analyze_batch(hv_workspace* ws, int count, const double* const* xs,
                            const int64_t* lengths, int fs, double f0_floor, double f0_ceil,
                            int32_t* offsets, double* best, int64_t* nnz, double* timings_ms) {
   return guarded([&] {
-    if (!xs || !lengths || !offsets || !best || !nnz) invalid("null pointer");
+    if (!xs || !lengths || !nnz) invalid("null pointer");
     for (int u = 0; u < count; ++u)
       if (!xs[u]) invalid("null waveform pointer");
     run_batch(ws, count, xs, lengths, fs, f0_floor, f0_ceil, offsets, best, nnz, timings_ms, false);
@@ -762,7 +775,7 @@ hv_status hv_analyze_batch_device(hv_workspace* ws, int count, const double* x_d
                                   double f0_ceil, int32_t* offsets, double* best, int64_t* nnz,
                                   double* timings_ms) {
   return guarded([&] {
-    if (!x_device || !lengths || !offsets || !best || !nnz) invalid("null pointer");
+    if (!x_device || !lengths || !nnz) invalid("null pointer");
     const double* xs[1] = {x_device};
     run_batch(ws, count, xs, lengths, fs, f0_floor, f0_ceil, offsets, best, nnz, timings_ms, true);
   });
@@ -783,16 +796,23 @@ hv_status hv_fetch_candidates(hv_workspace* ws, double* f0, double* scores) {
     if (ws->last_nnz < 0) invalid("no analysis result to fetch");
     if (ws->last_nnz == 0) return;
     if (!f0 || !scores) invalid("null pointer");
-    DeviceGuard guard(ws->device);
     const long long n = ws->last_nnz;
-    double* h = ws->h_cand.ensure<double>(2 * n);
-    check(hipMemcpyAsync(h, ws->rf0.get<double>(), n * sizeof(double), hipMemcpyDeviceToHost,
-                         ws->stream), "download candidates");
-    check(hipMemcpyAsync(h + n, ws->rsc.get<double>(), n * sizeof(double), hipMemcpyDeviceToHost,
-                         ws->stream), "download candidates");
-    check(hipStreamSynchronize(ws->stream), "download candidates");
+    const double* h = ws->h_cand.get<double>();
     std::memcpy(f0, h, n * sizeof(double));
     std::memcpy(scores, h + n, n * sizeof(double));
+  });
+}
+
+hv_status hv_result_view(hv_workspace* ws, const int32_t** offsets, const double** best,
+                         const double** f0, const double** scores, int64_t* nnz) {
+  return guarded([&] {
+    if (!ws || !offsets || !best || !f0 || !scores || !nnz) invalid("null pointer");
+    if (ws->last_nnz < 0) invalid("no analysis result to view");
+    *offsets = ws->h_offsets.get<int32_t>();
+    *best = ws->h_best.get<double>();
+    *f0 = ws->h_cand.get<double>();
+    *scores = ws->h_cand.get<double>() + ws->last_nnz;
+    *nnz = ws->last_nnz;
   });
 }
 

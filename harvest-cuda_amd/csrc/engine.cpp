@@ -10,6 +10,7 @@
 #include <chrono>
 #include <climits>
 #include <cmath>
+#include <cstring>
 #include <numeric>
 #include <stdexcept>
 #include <thread>
@@ -204,7 +205,7 @@ void fix_step4(const double* in, int frames, double* out, std::vector<int>& list
 }
 
 // The zero-lag smoother's second-order recursion (FilteringF0,
-// harvest.cpp:1049-1074).
+// harvest.cpp:1049-1074).  State (w0, w1); one step consumes one input.
 struct Smoother {
   static constexpr double b0 = 0.0078202080334971724, b1 = 0.015640416066994345;
   static constexpr double a0 = 1.7347257688092754, a1 = -0.76600660094326412;
@@ -216,20 +217,86 @@ struct Smoother {
     w0 = wt;
     return y;
   }
-  // Runs `count` steps of a constant input.  Once a step leaves the state
-  // unchanged (w0 == w1 == new w0) every later step repeats it bit for bit,
-  // so the remaining steps are skipped; returns the index of that step, or
-  // -1 if the state never settled.  Outputs are not needed by the callers.
-  long settle(double x, long count) {
-    for (long s = 0; s < count; ++s) {
-      const double wt = x + a0 * w0 + a1 * w1;
-      if (wt == w0 && w0 == w1) return s;
-      w1 = w0;
-      w0 = wt;
-    }
-    return -1;
+  bool same(const Smoother& o) const {
+    return std::memcmp(&w0, &o.w0, sizeof(double)) == 0 &&
+           std::memcmp(&w1, &o.w1, sizeof(double)) == 0;
   }
 };
+
+// Orbit of a deterministic map F on filter states (one step, or one period of
+// a periodic input).  In floating point the orbit from any state is
+// eventually periodic: it settles on a fixed point or, for some inputs, on a
+// short limit cycle.  find_cycle returns (mu, lam): the state after mu
+// iterations lies on a cycle of length lam.  The last kRing states are kept
+// so that the usual short cycles are found in one pass; longer ones fall
+// back to Brent's algorithm.  `limit` bounds the search.
+struct Cycle {
+  long mu = -1, lam = 0;  // mu < 0: none found within the limit
+};
+
+template <class F> Cycle find_cycle(const Smoother& start, long limit, F&& f) {
+  constexpr int kRing = 8;
+  Smoother ring[kRing];
+  Smoother s = start;
+  ring[0] = s;
+  for (long k = 1; k <= limit; ++k) {
+    f(s);
+    for (int d = 1; d <= kRing && d <= k; ++d)
+      if (ring[(k - d) % kRing].same(s)) {
+        // State k equals state k - d: cycle of length d entered at or
+        // before k - d; locate its first entry.
+        Cycle c;
+        c.lam = d;
+        Smoother a = start, b = start;
+        for (int q = 0; q < d; ++q) f(b);
+        long mu = 0;
+        while (!a.same(b)) {
+          f(a);
+          f(b);
+          ++mu;
+        }
+        c.mu = mu;
+        return c;
+      }
+    ring[k % kRing] = s;
+  }
+  // Brent's algorithm for longer cycles.
+  long power = 1, lam = 1;
+  Smoother tortoise = start, hare = start;
+  f(hare);
+  while (!tortoise.same(hare)) {
+    if (power == lam) {
+      tortoise = hare;
+      power *= 2;
+      lam = 0;
+    }
+    f(hare);
+    ++lam;
+    if (power > limit) return Cycle{};
+  }
+  Smoother a = start, b = start;
+  for (long q = 0; q < lam; ++q) f(b);
+  long mu = 0;
+  while (!a.same(b)) {
+    f(a);
+    f(b);
+    ++mu;
+  }
+  Cycle c;
+  c.mu = mu;
+  c.lam = lam;
+  return c;
+}
+
+// F applied `count` times to s, exactly, skipping whole cycles.
+template <class F> Smoother iterate(Smoother s, long count, F&& f) {
+  if (count > 512) {
+    const Cycle c = find_cycle(s, count, f);
+    if (c.mu >= 0 && count > c.mu) count = c.mu + (count - c.mu) % c.lam;
+  }
+  for (long k = 0; k < count; ++k) f(s);
+  return s;
+}
 
 }  // namespace
 
@@ -300,9 +367,10 @@ void fix_contour(const CandidateRows& rows, int frames, const double* base, doub
 // SmoothF0Contour (harvest.cpp:1079-1113).  WORLD pads the contour with 300
 // zeros on each side and, for every voiced section, runs the zero-phase
 // filter over the whole padded length with the section's end values held
-// constant outside it.  Only the stretches where the recursion has not yet
-// settled on a constant input are simulated; the rest is skipped exactly
-// (Smoother::settle), which leaves every output bit-identical to WORLD's.
+// constant outside it.  Outside the section the filter sees constant (or,
+// in the backward pass, eventually periodic) input, so those stretches are
+// advanced exactly with iterate() instead of sample by sample: every output
+// stays bit-identical to WORLD's, in O(section + transient) per section.
 void smooth_contour(const double* f0, int frames, double* out) {
   constexpr int lag = 300;
   const long n = (long)frames + 2 * lag;
@@ -319,39 +387,50 @@ void smooth_contour(const double* f0, int frames, double* out) {
     long ed = i;
     while (ed + 1 < n - 1 && padded(ed + 1) > 0) ++ed;
     i = ed + 1;
-    // Forward pass: constant prefix, the section, then the constant suffix.
-    Smoother fwd;
-    fwd.settle(padded(st), st);
+    // Forward pass: st steps of the constant x[st] from rest, then the section.
+    const double head = padded(st);
+    Smoother fwd = iterate(Smoother{}, st, [&](Smoother& s) { s.step(head); });
     section_out.resize(ed - st + 1);
     for (long k = st; k <= ed; ++k) section_out[k - st] = fwd.step(padded(k));
+    // Forward suffix: `steps` steps of the constant x[ed].  Its outputs are a
+    // function of the state, so once the state orbit enters its cycle (after
+    // mu steps, period lam) the outputs repeat with period lam.
     const long steps = n - 1 - ed;
     const double held = padded(ed);
-    tail.clear();
-    long fixed_at = -1;
-    double steady = 0.0;
-    for (long k = 0; k < steps; ++k) {
-      const bool settled_state = fwd.w0 == fwd.w1;
-      const double before = fwd.w0;
-      const double y = fwd.step(held);
-      tail.push_back(y);
-      if (settled_state && fwd.w0 == before) {
-        fixed_at = k;
-        steady = y;
-        break;
+    auto fstep = [&](Smoother& s) { s.step(held); };
+    long mu = steps, lam = 1;
+    if (steps > 512) {
+      const Cycle c = find_cycle(fwd, steps, fstep);
+      if (c.mu >= 0) {
+        mu = std::min(steps, c.mu);
+        lam = c.lam;
       }
     }
-    // Backward pass from the padded end: the settled tail is constant input.
+    const long explicit_steps = std::min(steps, mu + lam);
+    tail.resize(explicit_steps);
+    {
+      Smoother s = fwd;
+      for (long k = 0; k < explicit_steps; ++k) tail[k] = s.step(held);
+    }
+    auto suffix_out = [&](long k) {  // forward output at padded index ed + 1 + k
+      return k < mu ? tail[k] : tail[mu + (k - mu) % lam];
+    };
+    // Backward pass from the padded end (suffix outputs in reverse order).
     Smoother bwd;
-    long explicit_last = (long)tail.size() - 1;
-    if (fixed_at >= 0) bwd.settle(steady, steps - 1 - fixed_at);
-    for (long k = explicit_last; k >= 0; --k) {
-      if (fixed_at >= 0 && k == fixed_at) {
-        bwd.step(steady);
-        continue;
-      }
-      bwd.step(tail[k]);
+    long k = steps - 1;
+    if (steps > explicit_steps) {
+      // Inputs k = steps-1 down to mu are periodic: advance whole periods.
+      const long periodic = steps - mu;
+      const long periods = periodic / lam;
+      std::vector<double> period(lam);
+      for (long q = 0; q < lam; ++q) period[q] = suffix_out(steps - 1 - q);
+      bwd = iterate(bwd, periods, [&](Smoother& s) {
+        for (long q = 0; q < lam; ++q) s.step(period[q]);
+      });
+      k = steps - 1 - periods * lam;
     }
-    for (long k = ed; k >= st; --k) out[k - lag] = bwd.step(section_out[k - st]);
+    for (; k >= 0; --k) bwd.step(suffix_out(k));
+    for (long q = ed; q >= st; --q) out[q - lag] = bwd.step(section_out[q - st]);
   }
 }
 
@@ -390,9 +469,11 @@ double ms_since(Clock::time_point& mark) {
 }
 
 void finish_utterance(const CandidateRows& rows, int frames, const double* best,
-                      double frame_period, int out_frames, Result& r, Timings* timings,
-                      Clock::time_point* mark) {
-  std::vector<double> contour(frames), basic(frames);
+                      double frame_period, int out_frames, Result& r, HostArena* arena,
+                      Timings* timings, Clock::time_point* mark) {
+  HostArray<double> contour(arena), basic(arena);
+  contour.allocate(frames);
+  basic.allocate(frames);
   fix_contour(rows, frames, best, contour.data());
   if (timings) (*timings)["contour"] += ms_since(*mark);
   smooth_contour(contour.data(), frames, basic.data());
@@ -402,6 +483,21 @@ void finish_utterance(const CandidateRows& rows, int frames, const double* best,
   resample(basic.data(), frames, frame_period, out_frames, r.f0.data(), r.times.data());
   if (timings) (*timings)["resample"] += ms_since(*mark);
 }
+
+// Results of the last analysis on `ws`, read in place from pinned memory.
+struct View {
+  const int32_t* offsets;
+  const double* best;
+  const double* f0;
+  const double* scores;
+  int64_t nnz;
+};
+
+View result_view(hv_workspace* ws) {
+  View v{};
+  throw_status(hv_result_view(ws, &v.offsets, &v.best, &v.f0, &v.scores, &v.nnz));
+  return v;
+}
 }  // namespace
 
 Result analyze(hv_workspace* ws, const double* x, int64_t length, int fs, double f0_floor,
@@ -410,27 +506,22 @@ Result analyze(hv_workspace* ws, const double* x, int64_t length, int fs, double
   if (timings) timings->clear();
   const Params p = derive_params(length, fs, f0_floor, f0_ceil, frame_period);
   HostCache::Lease lease;
-  HostArray<int32_t> offsets(lease.arena());
-  HostArray<double> best(lease.arena()), cf0(lease.arena()), cscore(lease.arena());
-  offsets.allocate(p.frames + 1);
-  best.allocate(p.frames);
   if (timings) (*timings)["initialize"] = ms_since(mark);
   int64_t nnz = 0;
   double device_ms[HV_TIMING_SLOTS];
   const int64_t len = length;
-  throw_status(hv_analyze_batch(ws, 1, &x, &len, fs, f0_floor, f0_ceil, offsets.data(),
-                                best.data(), &nnz, timings ? device_ms : nullptr));
-  cf0.allocate(nnz);
-  cscore.allocate(nnz);
-  throw_status(hv_fetch_candidates(ws, cf0.data(), cscore.data()));
+  throw_status(hv_analyze_batch(ws, 1, &x, &len, fs, f0_floor, f0_ceil, nullptr, nullptr, &nnz,
+                                timings ? device_ms : nullptr));
+  const View v = result_view(ws);
   if (timings) {
     ms_since(mark);
     record_device(device_ms, *timings);
     record_stats(ws, *timings);
   }
   Result r;
-  CandidateRows rows{offsets.data(), cf0.data(), cscore.data()};
-  finish_utterance(rows, p.frames, best.data(), frame_period, p.out_frames, r, timings, &mark);
+  CandidateRows rows{v.offsets, v.f0, v.scores};
+  finish_utterance(rows, p.frames, v.best, frame_period, p.out_frames, r, lease.arena(), timings,
+                   &mark);
   return r;
 }
 
@@ -449,13 +540,10 @@ std::vector<Result> batch_common(hv_workspace* ws, Call&& call,
     params[u] = derive_params(lengths[u], fs, f0_floor, f0_ceil, frame_period);
     frame_off[u + 1] = frame_off[u] + params[u].frames;
   }
-  std::vector<int32_t> offsets(frame_off[count] + 1);
-  std::vector<double> best(frame_off[count]);
   int64_t nnz = 0;
   double device_ms[HV_TIMING_SLOTS];
-  throw_status(call(offsets.data(), best.data(), &nnz, timings ? device_ms : nullptr));
-  std::vector<double> cf0(std::max<int64_t>(nnz, 1)), cscore(std::max<int64_t>(nnz, 1));
-  throw_status(hv_fetch_candidates(ws, cf0.data(), cscore.data()));
+  throw_status(call(nullptr, nullptr, &nnz, timings ? device_ms : nullptr));
+  const View v = result_view(ws);
   if (timings) {
     (*timings)["device_call"] = ms_since(mark);
     record_device(device_ms, *timings);
@@ -471,9 +559,9 @@ std::vector<Result> batch_common(hv_workspace* ws, Call&& call,
   auto worker = [&] {
     for (int k = next++; k < count; k = next++) {
       const int u = order[k];
-      CandidateRows rows{offsets.data() + frame_off[u], cf0.data(), cscore.data()};
-      finish_utterance(rows, params[u].frames, best.data() + frame_off[u], frame_period,
-                       params[u].out_frames, results[u], nullptr, nullptr);
+      CandidateRows rows{v.offsets + frame_off[u], v.f0, v.scores};
+      finish_utterance(rows, params[u].frames, v.best + frame_off[u], frame_period,
+                       params[u].out_frames, results[u], nullptr, nullptr, nullptr);
     }
   };
   const int threads = std::max(1, std::min(host_threads, count));

@@ -85,7 +85,8 @@ hv_status hv_decimate(hv_workspace* ws, const double* x, int64_t length,
  *   offsets[frames + 1]  CSR row offsets of nonzero refined candidates,
  *   best[frames]         f0 of the best-scoring candidate per frame,
  *   *nnz                 offsets[frames].
- * The candidates themselves are fetched by hv_fetch_candidates.  fft_size,
+ * The candidates themselves are fetched by hv_fetch_candidates (or read in
+ * place through hv_result_view; offsets and best may then be NULL).  fft_size,
  * bands, boundaries and capacity follow WORLD's HarvestGeneralBody
  * (harvest.cpp:1149-1181); frames counts 1 ms frames. */
 hv_status hv_analyze(hv_workspace* ws, const double* y, int64_t y_length,
@@ -130,6 +131,14 @@ hv_status hv_last_stats(hv_workspace* ws, int64_t* stats);
 /* Copy the nnz candidates of the last hv_analyze / hv_analyze_batch call:
  * f0[nnz] and scores[nnz] in CSR order (frame order, then slot order). */
 hv_status hv_fetch_candidates(hv_workspace* ws, double* f0, double* scores);
+
+/* Zero-copy access to the last analysis' results in the workspace's pinned
+ * host buffers: offsets[total_frames + 1], best[total_frames], f0[nnz],
+ * scores[nnz].  Valid until the next call on `ws`.  The offsets/best outputs
+ * of the analysis entry points may then be NULL.  No reference counterpart
+ * (the reference copies into HostArrays, src/cuda_backend.cu:1117-1126). */
+hv_status hv_result_view(hv_workspace* ws, const int32_t** offsets, const double** best,
+                         const double** f0, const double** scores, int64_t* nnz);
 
 #ifdef __cplusplus
 }
