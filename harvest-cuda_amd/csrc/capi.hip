@@ -347,6 +347,9 @@ void fill_config(Config& c, int count, int ratio, int fs, double fs_work, double
   c.slots = 7 * capacity;
   c.twiddle_n = twiddle_n;
   c.lds_limit = 64 * 1024;
+  // Candidates are means of values in [f0_floor, f0_ceil]; the 1e-9 slack
+  // covers their rounding, the + 1 the truncation of harvest.cpp:598.
+  c.hmax = (int)(1.5 * fs_work / (f0_floor * (1.0 - 1e-9)) + 1.0) + 1;
   const double* co = kDecimation[ratio >= 2 && ratio <= 12 ? ratio : 0];
   c.a0 = co[0];
   c.a1 = co[1];
@@ -478,8 +481,10 @@ void analyze_device(hv_workspace* ws, const Plan& p, int32_t* offsets, double* b
   const size_t slots = (size_t)p.total_frames * c.slots;
   double* rf0 = ws->rf0.ensure<double>(slots);
   double* rsc = ws->rsc.ensure<double>(slots);
-  unsigned long long* work = ws->work.ensure<unsigned long long>(HV_STAT_SLOTS);
-  launch_fill_int(s, reinterpret_cast<int*>(work), 2 * HV_STAT_SLOTS, 0);
+  // HV_STAT_SLOTS work counters, then a flag the refinement sets if a
+  // candidate's window exceeds the bound its LDS was sized for.
+  unsigned long long* work = ws->work.ensure<unsigned long long>(HV_STAT_SLOTS + 1);
+  launch_fill_int(s, reinterpret_cast<int*>(work), 2 * (HV_STAT_SLOTS + 1), 0);
   timed(ws, 6, [&] {
     launch_refine(s, c, dutt, ws->f_off.get<long long>(), p.total_frames, y, official, count,
                   ws->twiddle.get<double>(), rf0, rsc, work);
@@ -505,11 +510,13 @@ void analyze_device(hv_workspace* ws, const Plan& p, int32_t* offsets, double* b
                        hipMemcpyDeviceToHost, s), "download offsets");
   check(hipMemcpyAsync(hbest, dbest, p.total_frames * sizeof(double), hipMemcpyDeviceToHost, s),
         "download best");
-  unsigned long long* hwork = ws->h_work.ensure<unsigned long long>(HV_STAT_SLOTS);
-  check(hipMemcpyAsync(hwork, work, HV_STAT_SLOTS * sizeof(unsigned long long),
+  unsigned long long* hwork = ws->h_work.ensure<unsigned long long>(HV_STAT_SLOTS + 1);
+  check(hipMemcpyAsync(hwork, work, (HV_STAT_SLOTS + 1) * sizeof(unsigned long long),
                        hipMemcpyDeviceToHost, s), "download counters");
   check(hipEventRecord(ws->ev[3], s), "hipEventRecord");
   check(hipStreamSynchronize(s), "analysis");
+  if (hwork[HV_STAT_SLOTS])
+    throw std::runtime_error("refinement window beyond its precomputed bound");
   // The candidates go straight to the pinned staging buffer, where the host
   // contour reads them in place (hv_result_view) or callers copy them out.
   const long long n = hoff[p.total_frames];

@@ -52,6 +52,7 @@ struct Config {
   int slots;         // 7 * capacity
   int twiddle_n;     // size of the refinement twiddle table (0: none)
   int lds_limit;     // dynamic LDS bytes a workgroup may use on this device
+  int hmax;          // largest refinement half window (candidates >= f0_floor)
   double a0, a1, a2, b0, b1;  // decimation IIR
   double chunk_pow[9];        // A^kChunk of the IIR state transition
 };

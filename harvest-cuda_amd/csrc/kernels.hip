@@ -670,197 +670,162 @@ __global__ __launch_bounds__(kThreads) void k_official(Config c, const long long
 
 // ---------------------------------------------------------------------------
 // Refinement (GetRefinedF0 harvest.cpp:589-617, GetMeanF0 :541-584, FixF0
-// :507-536).  One thread per (frame, slot); slot s < 7 count holds the
-// overlapped candidate of OverlapF0Candidates (harvest.cpp:417-429).  The
-// Blackman and derivative windows follow GetMainWindow/GetDiffWindow
-// (:446-468); the two spectra are evaluated only at the <= 6 harmonic bins
-// FixF0 reads, as DFT sums with exact integer phase indices.
-template <bool kTable>
-__global__ __launch_bounds__(kThreads) void k_refine(
-    Config c, const Utt* utt, const long long* f_off, long long total_frames,
-    const double* y, const double* official, const int* count,
-    const double* twiddle, double* rf0, double* rscore) {
-  const long long id = (long long)blockIdx.x * kThreads + threadIdx.x;
-  const long long gf = id / c.slots;
-  const int s = (int)(id % c.slots);
-  if (gf >= total_frames) return;
-  const int ui = find_segment(f_off, c.count, gf);
-  const int cnt = count[ui];
-  if (s >= 7 * cnt) return;
-  const Utt u = utt[ui];
-  const long long i = gf - u.f_off;
-  const int grp = s / cnt, jj = s % cnt;
-  long long src = i;
-  if (grp >= 1 && grp <= 3) src = i - grp;
-  else if (grp >= 4) src = i + (grp - 3);
-  const double f0 = (src >= 0 && src < u.frames)
-                        ? official[(u.f_off + src) * c.capacity + jj]
-                        : 0.0;
-  double refined = 0.0, score = 0.0;
-  if (f0 > 0.0) {
-    const double fs = c.fs_work;
-    const double t = (double)i / 1000.0;
-    const int hw = (int)(1.5 * fs / f0 + 1.0);
-    const double wl = (2.0 * hw + 1.0) / fs;
-    const int len = 2 * hw + 1;
-    const int n = 1 << (2 + (31 - __clz(len)));
-    const int basic = mround((t + (-hw) / fs) * fs + 0.001);
-    const int H = min((int)(fs / 2.0 / f0), 6);
-    int kb[6], ph[6];
-    double mre[6], mim[6], dre[6], dim[6];
-#pragma unroll
-    for (int hh = 0; hh < 6; ++hh) {
-      kb[hh] = hh < H ? mround(f0 * n / fs * (hh + 1)) : 0;
-      ph[hh] = 0;
-      mre[hh] = mim[hh] = dre[hh] = dim[hh] = 0.0;
-    }
-    const int stride = kTable ? c.twiddle_n / n : 0;
-    const bool table = kTable && stride > 0;
-    const double* yu = y + u.y_off;
-    auto window = [&](int m) -> double {
-      const double tmp = (basic + m - 1.0) / fs - t;
-      return 0.42 + 0.5 * cos(2.0 * kPi * tmp / wl) + 0.08 * cos(4.0 * kPi * tmp / wl);
-    };
-    double wprev = 0.0, wcur = window(0);
-    for (int m = 0; m < len; ++m) {
-      const double wnext = (m + 1 < len) ? window(m + 1) : 0.0;
-      double d;
-      if (m == 0) d = -wnext / 2.0;
-      else if (m == len - 1) d = wprev / 2.0;
-      else d = -(wnext - wprev) / 2.0;
-      int idx = basic + m - 1;
-      idx = idx < 0 ? 0 : (idx > u.n - 1 ? u.n - 1 : idx);
-      const double xv = yu[idx];
-      const double xm = xv * wcur, xd = xv * d;
-#pragma unroll
-      for (int hh = 0; hh < 6; ++hh) {
-        if (hh < H) {
-          double cs, sn;
-          if (table) {
-            const double* tw = twiddle + 2LL * ph[hh] * stride;
-            cs = tw[0];
-            sn = tw[1];
-          } else {
-            sincospi(2.0 * (double)ph[hh] / n, &sn, &cs);
-          }
-          mre[hh] = fma(xm, cs, mre[hh]);
-          mim[hh] = fma(-xm, sn, mim[hh]);
-          dre[hh] = fma(xd, cs, dre[hh]);
-          dim[hh] = fma(-xd, sn, dim[hh]);
-          ph[hh] = (ph[hh] + kb[hh]) & (n - 1);
-        }
-      }
-      wprev = wcur;
-      wcur = wnext;
-    }
-    double numer = 0.0, den = 0.0, dev = 0.0;
-    for (int hh = 0; hh < H; ++hh) {
-      const double pw = mre[hh] * mre[hh] + mim[hh] * mim[hh];
-      const double nu = mre[hh] * dim[hh] - mim[hh] * dre[hh];
-      const double ifr =
-          pw == 0.0 ? 0.0 : (double)kb[hh] * fs / n + nu / pw * fs / 2.0 / kPi;
-      const double amp = sqrt(pw);
-      numer += amp * ifr;
-      den += amp * (hh + 1.0);
-      dev += fabs((ifr / (hh + 1.0) - f0) / f0);
-    }
-    refined = numer / (den + kGuard);
-    score = 1.0 / (dev / H + kGuard);
-    if (refined < c.f0_floor || refined > c.f0_ceil || score < 2.5) {
-      refined = 0.0;
-      score = 0.0;
-    }
-  }
-  rf0[gf * c.slots + s] = refined;
-  rscore[gf * c.slots + s] = score;
-}
+// :507-536), one wavefront per 1 ms frame, four frames per workgroup.
+//
+// Slot s < 7 count of a frame holds the overlapped candidate of
+// OverlapF0Candidates (harvest.cpp:417-429).  A slot's windows depend only on
+// its half length h (GetMainWindow/GetDiffWindow :446-468) and a DFT bin only
+// on (h, k), so the wave
+//   * groups the frame's slots by h (LDS hash, then per-window slot lists)
+//     and handles one window at a time: the Blackman window by rotation of
+//     one exact sincospi per lane, the derivative window from the
+//     neighbouring lanes' values, and the windowed samples x w, x d staged in
+//     LDS, kStage samples per pass;
+//   * marks the window's distinct bins k in an LDS bitmap, whose rank
+//     (popcount) gives each (slot, harmonic) its result index;
+//   * sums each distinct bin with 8 lanes (samples m = sub mod 8, increasing
+//     m) and a fixed butterfly; each lane's twiddle starts from the exact
+//     table value for its first sample and advances by complex rotation;
+//   * keeps the frame's bin results in LDS and finishes all of its slots
+//     (FixF0 and the gate) in one lane-parallel pass at the end.
+// The order of operations per bin is fixed, so results do not depend on the
+// batch, on the staging passes or on which path served the bin.
+constexpr int kWaves = 4;        // frames per workgroup
+constexpr int kSplit = 8;        // lanes sharing one DFT bin
+constexpr int kPer = 2;          // bins per lane group per pass
+constexpr int kFrameBins = 128;  // bin results of one frame kept in LDS (< 256)
+constexpr int kStage = 128;      // windowed samples staged per pass (multiple of 64)
+constexpr int kMaxWords = 128;   // bitmap words (bins 0..8191)
+constexpr int kDirectWords = 16; // bitmaps up to this size take a direct prefix
 
-// Refinement, one workgroup per frame.  The same numbers as k_refine, with
-// the work shared: the windowed samples depend only on (frame, h), so each
-// distinct half length h of the frame gets its Blackman and derivative
-// windows (and the two cosines per sample) computed once into LDS; and a DFT
-// bin depends only on (h, k), so duplicate (h, k) pairs among the frame's
-// slots x harmonics are evaluated once (LDS hash).  Each bin is one thread's
-// sequential sum over the window in increasing sample order, exactly as in
-// k_refine, so both kernels give bitwise identical results.
-constexpr int kHash = 1024;          // (h, k) hash slots per frame
-constexpr unsigned long long kEmpty = ~0ull;
+// LDS-qualified pointers: accesses through them compile to ds_* operations
+// however the pointers are passed around.
+#define LDS __attribute__((address_space(3)))
 
-struct RefineLds {  // carve-up of the dynamic LDS of k_refine_frames
-  double* t_res;    // [2 * 6S] power and IF numerator of each distinct bin
-  double* f0;       // [S] candidate per slot (0: inactive)
-  double* tw;       // [2 * kTwLds] twiddles (when the table is small enough)
-  double* wv;       // [BW] window values of the current round
-  double* xw;       // [2 BW] (x w, x d) of the current round
-  unsigned long long* keys;  // [kHash]
-  int* vals;        // [kHash]
-  int* h;           // [S] half window length per slot
-  int* win;         // [S] distinct-window index of each slot
-  int* tref;        // [6S] hash position of each (slot, harmonic)
-  int* w_h;         // [S] half length of each distinct window
-  int* w_off;       // [S] LDS offset of each window in its round
-  int* w_round;     // [S] round of each window (-1: too long for LDS)
-  int* w_cnt;       // [S] distinct bins of each window
-  int* w_first;     // [S] first position of the window's bins in t_order
-  int* r_first;     // [S + 1] first t_order position of each round
-  int* t_win;       // [6S] window of each distinct bin
-  int* t_k;         // [6S] bin index of each distinct bin
-  int* t_rank;      // [6S] rank of the bin inside its window
-  int* t_order;     // [6S] distinct bins sorted by window
-  int* misc;        // [8] counters
+// One wave's part of k_refine_waves' dynamic LDS, in carve order:
+//   f0     [S]          candidate per slot
+//   q      [S]          f0 n / fs of the slot's window
+//   res    [2 kFrameBins] power and IF numerator per bin result
+//   xwd    [2 kStage]   x w and x d of the staged samples
+//   bits   [W]          distinct bins of the current window
+//   wpre   [W]          bins below each bitmap word
+//   h      [S]          half window length (0: inactive slot)
+//   nh     [S]          harmonics FixF0 uses
+//   win    [S]          window of each slot
+//   rank   [S]          position of the slot in its window's list
+//   wslots [S]          slots grouped by window
+//   win_h  [S]          half length of each window
+//   hkey   [HS]         hashed half lengths (-1: empty), later slot counts
+//   hval   [HS]         window of each hashed half length, later list starts
+//   bink   [kFrameBins] bins whose results are in res
+//   jidx   [6 S]        result index of each (slot, harmonic)
+//   misc   [4]          counters
+#define WAVE_LDS_FIELDS(X)                                                   \
+  X(double, f0, S) X(double, q, S) X(double, res, 2 * kFrameBins)            \
+  X(double, xwd, 2 * kStage) X(unsigned long long, bits, W) X(int, wpre, W)  \
+  X(int, h, S) X(int, nh, S) X(int, win, S) X(int, rank, S)                  \
+  X(int, wslots, S) X(int, win_h, S) X(int, hkey, HS) X(int, hval, HS)       \
+  X(int, bink, kFrameBins) X(unsigned char, jidx, 6 * S) X(int, misc, 4)
+
+struct WaveLds {
+#define X(T, name, count) LDS T* name;
+  WAVE_LDS_FIELDS(X)
+#undef X
 };
 
-constexpr int kTwLds = 1024;  // twiddle entries staged in LDS
-constexpr int kSplit = 8;     // lanes sharing one DFT bin
-
-// Walks the carve-up; with base == nullptr it only measures.
-__host__ __device__ inline size_t carve_refine_lds(void* base, int S, int BW, RefineLds* l) {
+__host__ __device__ inline size_t wave_lds_bytes(int S, int HS, int W) {
   size_t at = 0;
-  auto take = [&](size_t bytes) -> char* {
-    char* r = base ? static_cast<char*>(base) + at : nullptr;
-    at += (bytes + 15) & ~size_t(15);
-    return r;
-  };
-  RefineLds t;
-  t.t_res = (double*)take(sizeof(double) * 12 * S);
-  t.f0 = (double*)take(sizeof(double) * S);
-  t.tw = (double*)take(sizeof(double) * 2 * kTwLds);
-  t.wv = (double*)take(sizeof(double) * BW);
-  t.xw = (double*)take(sizeof(double) * 2 * BW);
-  t.keys = (unsigned long long*)take(sizeof(unsigned long long) * kHash);
-  t.vals = (int*)take(sizeof(int) * kHash);
-  t.h = (int*)take(sizeof(int) * S);
-  t.win = (int*)take(sizeof(int) * S);
-  t.tref = (int*)take(sizeof(int) * 6 * S);
-  t.w_h = (int*)take(sizeof(int) * S);
-  t.w_off = (int*)take(sizeof(int) * S);
-  t.w_round = (int*)take(sizeof(int) * S);
-  t.w_cnt = (int*)take(sizeof(int) * S);
-  t.w_first = (int*)take(sizeof(int) * S);
-  t.r_first = (int*)take(sizeof(int) * (S + 1));
-  t.t_win = (int*)take(sizeof(int) * 6 * S);
-  t.t_k = (int*)take(sizeof(int) * 6 * S);
-  t.t_rank = (int*)take(sizeof(int) * 6 * S);
-  t.t_order = (int*)take(sizeof(int) * 6 * S);
-  t.misc = (int*)take(sizeof(int) * 8);
-  if (l) *l = t;
+#define X(T, name, count) at += (sizeof(T) * (size_t)(count) + 15) & ~size_t(15);
+  WAVE_LDS_FIELDS(X)
+#undef X
   return at;
 }
 
-__device__ __forceinline__ double blackman(int basic, int m, double fs, double t, double wl) {
-  const double tmp = (basic + m - 1.0) / fs - t;
-  return 0.42 + 0.5 * cos(2.0 * kPi * tmp / wl) + 0.08 * cos(4.0 * kPi * tmp / wl);
+__device__ inline WaveLds carve_wave_lds(LDS char* base, int S, int HS, int W) {
+  WaveLds l;
+  size_t at = 0;
+#define X(T, name, count)                \
+  l.name = (LDS T*)(base + at);          \
+  at += (sizeof(T) * (size_t)(count) + 15) & ~size_t(15);
+  WAVE_LDS_FIELDS(X)
+#undef X
+  return l;
 }
 
-__global__ __launch_bounds__(kThreads) void k_refine_frames(
-    Config c, const Utt* utt, const long long* f_off, int BW, const double* y,
-    const double* official, const int* count, const double* twiddle, double* rf0,
-    double* rscore, unsigned long long* work) {
+// LDS atomics of one wave's region.
+__device__ __forceinline__ int lds_cas(LDS int* p, int expected, int desired) {
+  __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+  return expected;  // the previous value, whether or not it was replaced
+}
+__device__ __forceinline__ int lds_add(LDS int* p, int v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_or(LDS unsigned long long* p, unsigned long long v) {
+  __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Orders this wave's LDS accesses: writes by any lane before it are visible
+// to reads by any lane after it.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Value of v in lane `src` (uniform), without LDS traffic.
+__device__ __forceinline__ double lane_value(double v, int src) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, src);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), src);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// FFT size of GetMeanF0 for half window h: 2^(2 + floor(log2(2h + 1))).
+__host__ __device__ inline int window_fft(int h) {
+  int lg = 0;
+  while ((2 << lg) <= 2 * h + 1) ++lg;
+  return 1 << (lg + 2);
+}
+
+// Exclusive prefix sum of v over the wave; returns the wave total.
+__device__ __forceinline__ int wave_exclusive(int v, int lane, int* excl) {
+  int incl = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int o = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += o;
+  }
+  *excl = incl - v;
+  return __builtin_amdgcn_readlane(incl, 63);
+}
+
+// cos and sin of 2 pi p / n: the exact table entry (table of T >= n entries,
+// stride T / n) or sincospi when no table covers n.
+__device__ __forceinline__ void twiddle_at(const double* tw, int stride, int n, int p,
+                                           double& cs, double& sn) {
+  if (stride > 0) {
+    const double2 v = reinterpret_cast<const double2*>(tw)[(long long)p * stride];
+    cs = v.x;
+    sn = v.y;
+  } else {
+    sincospi(2.0 * (double)p / n, &sn, &cs);
+  }
+}
+
+__global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
+    Config c, const Utt* utt, const long long* f_off, long long total_frames, int HS, int W,
+    const double* y, const double* official, const int* count, const double* twiddle,
+    double* rf0, double* rscore, unsigned long long* work) {
   extern __shared__ double lds_raw[];
   const int S = c.slots;
-  RefineLds L;
-  carve_refine_lds(lds_raw, S, BW, &L);
-  const long long gf = blockIdx.x;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int grp = lane / kSplit, sub = lane % kSplit;
+  const WaveLds L =
+      carve_wave_lds((LDS char*)(char*)lds_raw + wave * wave_lds_bytes(S, HS, W), S, HS, W);
+
+  const long long gf = (long long)blockIdx.x * kWaves + wave;
+  if (gf >= total_frames) return;
   const int ui = find_segment(f_off, c.count, gf);
   const int cnt = count[ui];
   const int ncand = 7 * cnt;
@@ -870,265 +835,318 @@ __global__ __launch_bounds__(kThreads) void k_refine_frames(
   const double fs = c.fs_work;
   const double t = (double)i / 1000.0;
   const double* yu = y + u.y_off;
-  const int tid = threadIdx.x;
-  const bool tw_lds = c.twiddle_n > 0 && c.twiddle_n <= kTwLds;
+  double* out_f0 = rf0 + gf * S;
+  double* out_sc = rscore + gf * S;
+  unsigned long long wsamp = 0, bsamp = 0, nbins = 0;
+  bool fault = false;
 
-  for (int k = tid; k < kHash; k += kThreads) L.keys[k] = kEmpty;
-  for (int k = tid; k < S; k += kThreads) L.w_cnt[k] = 0;
-  if (tw_lds)
-    for (int k = tid; k < 2 * c.twiddle_n; k += kThreads) L.tw[k] = twiddle[k];
-  if (tid < 8) L.misc[tid] = 0;
-  // Slot candidates (OverlapF0Candidates layout) and half window lengths.
-  for (int s = tid; s < ncand; s += kThreads) {
-    const int grp = s / cnt, jj = s % cnt;
+  // Slots, their half lengths, harmonic counts and distinct windows.
+  for (int k = lane; k < HS; k += 64) L.hkey[k] = -1;
+  if (lane == 0) L.misc[0] = 0;
+  wave_sync();
+  for (int s = lane; s < ncand; s += 64) {
+    const int g = s / cnt, jj = s % cnt;
     long long src = i;
-    if (grp >= 1 && grp <= 3) src = i - grp;
-    else if (grp >= 4) src = i + (grp - 3);
+    if (g >= 1 && g <= 3) src = i - g;
+    else if (g >= 4) src = i + (g - 3);
     const double f0 = (src >= 0 && src < u.frames) ? official[(u.f_off + src) * c.capacity + jj]
                                                    : 0.0;
+    int hw = f0 > 0.0 ? (int)(1.5 * fs / f0 + 1.0) : 0;
+    if (hw > c.hmax) {  // cannot happen for candidates in [f0_floor, f0_ceil]
+      fault = true;
+      hw = 0;
+    }
     L.f0[s] = f0;
-    L.h[s] = f0 > 0.0 ? (int)(1.5 * fs / f0 + 1.0) : 0;
-  }
-  __syncthreads();
-  // Distinct windows: the first slot with a given h owns it.
-  for (int s = tid; s < ncand; s += kThreads) {
-    const int hs = L.h[s];
-    int owner = -1;
-    if (hs > 0) {
-      owner = s;
-      for (int q = 0; q < s; ++q)
-        if (L.h[q] == hs) {
-          owner = q;
-          break;
-        }
-    }
-    if (owner == s) {
-      const int w = atomicAdd(&L.misc[0], 1);
-      L.win[s] = w;
-      L.w_h[w] = hs;
-    }
-    L.tref[6 * s] = owner;  // temporary: owner slot
-  }
-  __syncthreads();
-  for (int s = tid; s < ncand; s += kThreads) {
-    const int owner = L.tref[6 * s];
-    if (owner >= 0 && owner != s) L.win[s] = L.win[owner];
-  }
-  __syncthreads();
-  // Distinct bins: insert (h, k) of every (slot, harmonic) into the hash.
-  for (int s = tid; s < ncand; s += kThreads) {
-    const double f0 = L.f0[s];
-    const int hs = L.h[s];
-    if (hs <= 0) continue;
-    const int n = 1 << (2 + (31 - __clz(2 * hs + 1)));
-    const int H = min((int)(fs / 2.0 / f0), 6);
-    for (int hh = 0; hh < H; ++hh) {
-      const int k = mround(f0 * n / fs * (hh + 1));
-      const unsigned long long key = ((unsigned long long)(unsigned)hs << 32) | (unsigned)k;
-      unsigned pos = (unsigned)((key * 0x9E3779B97F4A7C15ull) >> 54) & (kHash - 1);
+    L.h[s] = hw;
+    int pos = -1;
+    if (hw > 0) {
+      L.nh[s] = min((int)(fs / 2.0 / f0), 6);
+      L.q[s] = f0 * window_fft(hw) / fs;
+      pos = (int)(((unsigned)hw * 2654435761u) & (unsigned)(HS - 1));
       while (true) {
-        const unsigned long long prev = atomicCAS(&L.keys[pos], kEmpty, key);
-        if (prev == kEmpty) {
-          const int tix = atomicAdd(&L.misc[1], 1);
-          const int w = L.win[s];
-          L.vals[pos] = tix;
-          L.t_win[tix] = w;
-          L.t_k[tix] = k;
-          L.t_rank[tix] = atomicAdd(&L.w_cnt[w], 1);
+        const int prev = lds_cas(L.hkey + pos, -1, hw);
+        if (prev == -1) {
+          const int w = lds_add(L.misc, 1);
+          L.hval[pos] = w;
+          L.win_h[w] = hw;
           break;
         }
-        if (prev == key) break;
-        pos = (pos + 1) & (kHash - 1);
+        if (prev == hw) break;
+        pos = (pos + 1) & (HS - 1);
       }
-      L.tref[6 * s + hh] = (int)pos;
     }
+    L.win[s] = pos;
   }
-  __syncthreads();
-  const int nwin = L.misc[0], ntask = L.misc[1];
-  // Rounds of consecutive windows packed into the BW-sample buffer, and the
-  // bins ordered by window so that a round's bins are contiguous.
-  if (tid == 0) {
-    int round = 0, used = 0, first = 0;
-    for (int w = 0; w < nwin; ++w) {
-      L.w_first[w] = first;
-      first += L.w_cnt[w];
-      const int len = 2 * L.w_h[w] + 1;
-      if (len > BW) {
-        L.w_round[w] = -1;
-        continue;
-      }
-      if (used + len > BW) {
-        L.r_first[++round] = L.w_first[w];
-        used = 0;
-      } else if (used == 0 && round == 0) {
-        L.r_first[0] = L.w_first[w];
-      }
-      L.w_round[w] = round;
-      L.w_off[w] = used;
-      used += len;
+  wave_sync();
+  const int nwin = L.misc[0];
+  for (int s = lane; s < ncand; s += 64) L.win[s] = L.win[s] >= 0 ? L.hval[L.win[s]] : -1;
+  wave_sync();
+  // Per-window slot lists: counts (hkey), starts (hval), lists (wslots).
+  LDS int* wcnt = L.hkey;
+  LDS int* wbeg = L.hval;
+  for (int w = lane; w < nwin; w += 64) wcnt[w] = 0;
+  wave_sync();
+  for (int s = lane; s < ncand; s += 64)
+    if (L.win[s] >= 0) L.rank[s] = lds_add(wcnt + L.win[s], 1);
+  wave_sync();
+  {
+    int carry = 0;
+    for (int base = 0; base < nwin; base += 64) {
+      const int w = base + lane;
+      int ex;
+      const int tot = wave_exclusive(w < nwin ? wcnt[w] : 0, lane, &ex);
+      if (w < nwin) wbeg[w] = carry + ex;
+      carry += tot;
     }
-    const int rounds = used > 0 || round > 0 ? round + 1 : 0;
-    L.r_first[rounds] = first;
-    L.misc[2] = rounds;
+    if (lane == 0) wbeg[nwin] = carry;
   }
-  __syncthreads();
-  for (int tk = tid; tk < ntask; tk += kThreads)
-    L.t_order[L.w_first[L.t_win[tk]] + L.t_rank[tk]] = tk;
-  __syncthreads();
-  const int rounds = L.misc[2];
-  const int grp = tid / kSplit, sub = tid % kSplit;
-  for (int r = 0; r < rounds; ++r) {
-    // Window values, then windowed samples, of every window of this round.
-    for (int w = 0; w < nwin; ++w) {
-      if (L.w_round[w] != r) continue;
-      const int hw = L.w_h[w], len = 2 * hw + 1, off = L.w_off[w];
-      const double wl = (2.0 * hw + 1.0) / fs;
-      const int basic = mround((t + (-hw) / fs) * fs + 0.001);
-      for (int m = tid; m < len; m += kThreads) L.wv[off + m] = blackman(basic, m, fs, t, wl);
+  wave_sync();
+  for (int s = lane; s < ncand; s += 64)
+    if (L.win[s] >= 0) L.wslots[wbeg[L.win[s]] + L.rank[s]] = s;
+  wave_sync();
+
+  // FixF0 (harvest.cpp:507-536) and GetRefinedF0's gate (:610-614) of slot s
+  // from its harmonics' results res[jidx].
+  auto finish = [&](int s) {
+    const double f0 = L.f0[s], qv = L.q[s];
+    const int H = L.nh[s];
+    const double inv_n = 1.0 / window_fft(L.h[s]);  // exact: a power of two
+    double numer = 0.0, den = 0.0, dev = 0.0;
+    for (int hh = 0; hh < H; ++hh) {
+      const int k = mround(qv * (hh + 1));
+      const int j = L.jidx[6 * s + hh];
+      const double pw = L.res[2 * j], nu = L.res[2 * j + 1];
+      const double ifr = pw == 0.0 ? 0.0 : (double)k * fs * inv_n + nu / pw * fs * 0.5 / kPi;
+      const double amp = sqrt(pw);
+      numer += amp * ifr;
+      den += amp * (hh + 1.0);
+      dev += fabs((ifr / (hh + 1.0) - f0) / f0);
     }
-    __syncthreads();
-    for (int w = 0; w < nwin; ++w) {
-      if (L.w_round[w] != r) continue;
-      const int hw = L.w_h[w], len = 2 * hw + 1, off = L.w_off[w];
-      const int basic = mround((t + (-hw) / fs) * fs + 0.001);
-      for (int m = tid; m < len; m += kThreads) {
-        double d;
-        if (m == 0) d = -L.wv[off + 1] / 2.0;
-        else if (m == len - 1) d = L.wv[off + len - 2] / 2.0;
-        else d = -(L.wv[off + m + 1] - L.wv[off + m - 1]) / 2.0;
-        int idx = basic + m - 1;
-        idx = idx < 0 ? 0 : (idx > u.n - 1 ? u.n - 1 : idx);
-        const double xv = yu[idx];
-        L.xw[2 * (off + m)] = xv * L.wv[off + m];
-        L.xw[2 * (off + m) + 1] = xv * d;
-      }
+    double refined = numer / (den + kGuard);
+    double score = 1.0 / (dev / H + kGuard);
+    if (refined < c.f0_floor || refined > c.f0_ceil || score < 2.5) {
+      refined = 0.0;
+      score = 0.0;
     }
-    __syncthreads();
-    // Each distinct bin is summed by kSplit lanes (samples m = sub mod kSplit)
-    // and reduced by a fixed butterfly, so the result does not depend on the
-    // batch or on scheduling.
-    const int p_end = L.r_first[r + 1];
-    for (int p = L.r_first[r] + grp; p < p_end; p += kThreads / kSplit) {
-      const int tk = L.t_order[p];
-      const int w = L.t_win[tk];
-      if (L.w_round[w] != r) continue;
-      const int hw = L.w_h[w], len = 2 * hw + 1, off = L.w_off[w];
-      const int n = 1 << (2 + (31 - __clz(len)));
-      const int k = L.t_k[tk];
-      const int stride = c.twiddle_n / n;
-      const int step = (k * kSplit) & (n - 1);
-      int ph = (int)(((long long)k * sub) & (n - 1));
-      double mre = 0.0, mim = 0.0, dre = 0.0, dim = 0.0;
-      for (int m = sub; m < len; m += kSplit) {
-        const double xm = L.xw[2 * (off + m)], xd = L.xw[2 * (off + m) + 1];
-        double cs, sn;
-        if (stride > 0) {
-          const double* tw = (tw_lds ? L.tw : twiddle) + 2 * ph * stride;
-          cs = tw[0];
-          sn = tw[1];
-        } else {
-          sincospi(2.0 * (double)ph / n, &sn, &cs);
-        }
-        mre = fma(xm, cs, mre);
-        mim = fma(-xm, sn, mim);
-        dre = fma(xd, cs, dre);
-        dim = fma(-xd, sn, dim);
-        ph = (ph + step) & (n - 1);
-      }
-#pragma unroll
-      for (int d = kSplit / 2; d > 0; d >>= 1) {
-        mre += __shfl_xor(mre, d, kSplit);
-        mim += __shfl_xor(mim, d, kSplit);
-        dre += __shfl_xor(dre, d, kSplit);
-        dim += __shfl_xor(dim, d, kSplit);
-      }
-      if (sub == 0) {
-        L.t_res[2 * tk + 0] = mre * mre + mim * mim;
-        L.t_res[2 * tk + 1] = mre * dim - mim * dre;
-      }
-    }
-    __syncthreads();
-  }
-  // Windows longer than the LDS buffer: windowed samples on the fly.
-  for (int tk = tid; tk < ntask; tk += kThreads) {
-    const int w = L.t_win[tk];
-    if (L.w_round[w] >= 0) continue;
-    const int hw = L.w_h[w], len = 2 * hw + 1;
-    const int n = 1 << (2 + (31 - __clz(len)));
-    const int k = L.t_k[tk];
-    const double wl = (2.0 * hw + 1.0) / fs;
+    out_f0[s] = refined;
+    out_sc[s] = score;
+  };
+  // Finishes the slots of windows [w0, w1).
+  auto flush = [&](int w0, int w1) {
+    const int b = wbeg[w0], e = wbeg[w1];
+    for (int p = b + lane; p < e; p += 64) finish(L.wslots[p]);
+    wave_sync();
+  };
+
+  const int stride = c.twiddle_n > 0 ? c.twiddle_n : 0;  // divided by n per window
+  int rbase = 0, wflush = 0;
+  for (int w = 0; w < nwin; ++w) {
+    const int hw = L.win_h[w], len = 2 * hw + 1, n = window_fft(hw);
+    const int b = wbeg[w], e = wbeg[w + 1], pairs = 6 * (e - b);
+    const int tstride = stride / n;  // 0: no table covers n
+    // Blackman argument (harvest.cpp:446-456) in units of pi:
+    // 2 tmp / wl with tmp = (basic + m - 1) / fs - t and wl = len / fs.
     const int basic = mround((t + (-hw) / fs) * fs + 0.001);
-    const int stride = c.twiddle_n / n;
-    double mre = 0.0, mim = 0.0, dre = 0.0, dim = 0.0;
-    double wprev = 0.0, wcur = blackman(basic, 0, fs, t, wl);
-    int ph = 0;
-    for (int m = 0; m < len; ++m) {
-      const double wnext = (m + 1 < len) ? blackman(basic, m + 1, fs, t, wl) : 0.0;
-      double d;
-      if (m == 0) d = -wnext / 2.0;
-      else if (m == len - 1) d = wprev / 2.0;
-      else d = -(wnext - wprev) / 2.0;
-      int idx = basic + m - 1;
-      idx = idx < 0 ? 0 : (idx > u.n - 1 ? u.n - 1 : idx);
-      const double xv = yu[idx];
-      const double xm = xv * wcur, xd = xv * d;
-      double cs, sn;
-      if (stride > 0) {
-        const double* tw = twiddle + 2LL * ph * stride;
-        cs = tw[0];
-        sn = tw[1];
+    const double a = (double)(basic - 1) - t * fs;
+    double wc0, ws0, gc, gs;
+    sincospi(2.0 * (a + lane) / len, &ws0, &wc0);  // sample m = lane
+    sincospi(128.0 / len, &gs, &gc);               // advance by 64 samples
+    const int nstage = (len + kStage - 1) / kStage;
+    bool staged = false;
+
+    // (power, IF numerator) of bins bink[j0, j0 + nbt) into res[j0, ...).
+    auto dft_bins = [&](int j0, int nbt) {
+      for (int jb = 0; jb < nbt; jb += kSplit * kPer) {
+        double mre[kPer], mim[kPer], dre[kPer], dim[kPer], tc[kPer], ts[kPer], rc[kPer],
+            rs[kPer];
+        bool act[kPer];
+#pragma unroll
+        for (int qq = 0; qq < kPer; ++qq) {
+          const int j = jb + grp + 8 * qq;
+          act[qq] = j < nbt;
+          const int k = act[qq] ? L.bink[j0 + j] : 0;
+          // Twiddle of this lane's first sample m = sub, and the rotation
+          // for m -> m + 8.
+          twiddle_at(twiddle, tstride, n, (int)(((long long)k * sub) & (n - 1)), tc[qq], ts[qq]);
+          twiddle_at(twiddle, tstride, n, (k * kSplit) & (n - 1), rc[qq], rs[qq]);
+          mre[qq] = mim[qq] = dre[qq] = dim[qq] = 0.0;
+        }
+        const bool fill = !staged || nstage > 1;
+        double wc = wc0, ws = ws0, cur = 0.0, prev_last = 0.0;
+        auto blackman_cos = [](double cs) { return 0.42 + 0.5 * cs + 0.08 * (2.0 * cs * cs - 1.0); };
+        if (fill) cur = blackman_cos(wc);
+        for (int c0 = 0; c0 < len; c0 += kStage) {
+          const int cend = min(c0 + kStage, len);
+          if (fill) {
+            for (int s0 = c0; s0 < cend; s0 += 64) {
+              const int m = s0 + lane;
+              const double nc = fma(wc, gc, -(ws * gs)), ns = fma(ws, gc, wc * gs);
+              wc = nc;
+              ws = ns;
+              const double nxt = blackman_cos(wc);  // sample m + 64
+              const double next_first = lane_value(nxt, 0);
+              const double last = lane_value(cur, 63);
+              double wn = __shfl_down(cur, 1, 64);
+              double wp = __shfl_up(cur, 1, 64);
+              if (lane == 63) wn = next_first;
+              if (lane == 0) wp = prev_last;
+              prev_last = last;
+              if (m < len) {
+                double d;
+                if (m == 0) d = -wn / 2.0;
+                else if (m == len - 1) d = wp / 2.0;
+                else d = -(wn - wp) / 2.0;
+                int idx = basic + m - 1;
+                idx = idx < 0 ? 0 : (idx > u.n - 1 ? u.n - 1 : idx);
+                const double xv = yu[idx];
+                L.xwd[2 * (m - c0)] = xv * cur;
+                L.xwd[2 * (m - c0) + 1] = xv * d;
+              }
+              cur = nxt;
+            }
+            if (lane == 0) wsamp += (unsigned long long)(cend - c0);
+            wave_sync();
+          }
+#pragma unroll
+          for (int qq = 0; qq < kPer; ++qq) {
+            if (!act[qq]) continue;
+            double cs = tc[qq], sn = ts[qq];
+            const double ac = rc[qq], as = rs[qq];
+#pragma unroll 2
+            for (int m = c0 + sub; m < cend; m += kSplit) {
+              const double xm = L.xwd[2 * (m - c0)], xd = L.xwd[2 * (m - c0) + 1];
+              mre[qq] = fma(xm, cs, mre[qq]);
+              mim[qq] = fma(-xm, sn, mim[qq]);
+              dre[qq] = fma(xd, cs, dre[qq]);
+              dim[qq] = fma(-xd, sn, dim[qq]);
+              const double nc = fma(cs, ac, -(sn * as)), ns = fma(sn, ac, cs * as);
+              cs = nc;
+              sn = ns;
+            }
+            tc[qq] = cs;
+            ts[qq] = sn;
+          }
+          if (nstage > 1) wave_sync();
+        }
+        if (fill && nstage == 1) staged = true;
+#pragma unroll
+        for (int qq = 0; qq < kPer; ++qq) {
+#pragma unroll
+          for (int dd = kSplit / 2; dd > 0; dd >>= 1) {
+            mre[qq] += __shfl_xor(mre[qq], dd, kSplit);
+            mim[qq] += __shfl_xor(mim[qq], dd, kSplit);
+            dre[qq] += __shfl_xor(dre[qq], dd, kSplit);
+            dim[qq] += __shfl_xor(dim[qq], dd, kSplit);
+          }
+          const int j = jb + grp + 8 * qq;
+          if (sub == 0 && act[qq]) {
+            L.res[2 * (j0 + j)] = mre[qq] * mre[qq] + mim[qq] * mim[qq];
+            L.res[2 * (j0 + j) + 1] = mre[qq] * dim[qq] - mim[qq] * dre[qq];
+          }
+        }
+      }
+      if (lane == 0) {
+        bsamp += (unsigned long long)nbt * len;
+        nbins += (unsigned long long)nbt;
+      }
+      wave_sync();
+    };
+
+    // Distinct bins of the window (bins 0..n/2), one (slot, harmonic) pair
+    // per lane.
+    const int words = (n / 2 + 64) >> 6;
+    int nb = kFrameBins + 1;
+    if (W > 0) {
+      for (int k = lane; k < words; k += 64) L.bits[k] = 0ull;
+      wave_sync();
+      for (int p = lane; p < pairs; p += 64) {
+        const int s = L.wslots[b + p / 6], hh = p % 6;
+        if (hh < L.nh[s]) {
+          const int k = mround(L.q[s] * (hh + 1));
+          lds_or(L.bits + (k >> 6), 1ull << (k & 63));
+        }
+      }
+      wave_sync();
+      if (words <= kDirectWords) {
+        int ex = 0;
+        if (lane < words)
+          for (int v = 0; v < lane; ++v) ex += __popcll(L.bits[v]);
+        const int incl = ex + (lane < words ? __popcll(L.bits[lane]) : 0);
+        if (lane < words) L.wpre[lane] = ex;
+        nb = __builtin_amdgcn_readlane(incl, words - 1);
       } else {
-        sincospi(2.0 * (double)ph / n, &sn, &cs);
-      }
-      mre = fma(xm, cs, mre);
-      mim = fma(-xm, sn, mim);
-      dre = fma(xd, cs, dre);
-      dim = fma(-xd, sn, dim);
-      ph = (ph + k) & (n - 1);
-      wprev = wcur;
-      wcur = wnext;
-    }
-    L.t_res[2 * tk + 0] = mre * mre + mim * mim;
-    L.t_res[2 * tk + 1] = mre * dim - mim * dre;
-  }
-  __syncthreads();
-  // FixF0 per slot (harvest.cpp:507-536) and the gate of GetRefinedF0.
-  for (int s = tid; s < ncand; s += kThreads) {
-    const double f0 = L.f0[s];
-    const int hs = L.h[s];
-    double refined = 0.0, score = 0.0;
-    if (hs > 0) {
-      const int n = 1 << (2 + (31 - __clz(2 * hs + 1)));
-      const int H = min((int)(fs / 2.0 / f0), 6);
-      double numer = 0.0, den = 0.0, dev = 0.0;
-      for (int hh = 0; hh < H; ++hh) {
-        const int tk = L.vals[L.tref[6 * s + hh]];
-        const double pw = L.t_res[2 * tk], nu = L.t_res[2 * tk + 1];
-        const int k = L.t_k[tk];
-        const double ifr = pw == 0.0 ? 0.0 : (double)k * fs / n + nu / pw * fs / 2.0 / kPi;
-        const double amp = sqrt(pw);
-        numer += amp * ifr;
-        den += amp * (hh + 1.0);
-        dev += fabs((ifr / (hh + 1.0) - f0) / f0);
-      }
-      refined = numer / (den + kGuard);
-      score = 1.0 / (dev / H + kGuard);
-      if (refined < c.f0_floor || refined > c.f0_ceil || score < 2.5) {
-        refined = 0.0;
-        score = 0.0;
+        int carry = 0;
+        for (int base = 0; base < words; base += 64) {
+          const int wd = base + lane;
+          int ex;
+          const int tot = wave_exclusive(wd < words ? __popcll(L.bits[wd]) : 0, lane, &ex);
+          if (wd < words) L.wpre[wd] = carry + ex;
+          carry += tot;
+        }
+        nb = carry;
       }
     }
-    rf0[gf * c.slots + s] = refined;
-    rscore[gf * c.slots + s] = score;
+    wave_sync();
+    if (nb <= kFrameBins) {
+      if (rbase + nb > kFrameBins) {  // results buffer full: finish what is pending
+        flush(wflush, w);
+        wflush = w;
+        rbase = 0;
+      }
+      for (int wd = lane; wd < words; wd += 64) {
+        unsigned long long bb = L.bits[wd];
+        int r = rbase + L.wpre[wd];
+        while (bb) {
+          L.bink[r++] = wd * 64 + __ffsll((long long)bb) - 1;
+          bb &= bb - 1;
+        }
+      }
+      wave_sync();
+      dft_bins(rbase, nb);
+      for (int p = lane; p < pairs; p += 64) {
+        const int s = L.wslots[b + p / 6], hh = p % 6;
+        if (hh < L.nh[s]) {
+          const int k = mround(L.q[s] * (hh + 1));
+          const int wd = k >> 6;
+          L.jidx[6 * s + hh] = (unsigned char)(rbase + L.wpre[wd] +
+                                               __popcll(L.bits[wd] & ((1ull << (k & 63)) - 1ull)));
+        }
+      }
+      rbase += nb;
+      wave_sync();
+    } else {
+      // More distinct bins than the buffer holds (or no bitmap): finish what
+      // is pending, then slot by slot.
+      flush(wflush, w);
+      for (int p = b; p < e; ++p) {
+        const int s = L.wslots[p];
+        const int H = L.nh[s];
+        if (lane < H) {
+          L.bink[lane] = mround(L.q[s] * (lane + 1));
+          L.jidx[6 * s + lane] = (unsigned char)lane;
+        }
+        wave_sync();
+        dft_bins(0, H);
+        if (lane == 0) finish(s);
+        wave_sync();
+      }
+      wflush = w + 1;
+      rbase = 0;
+    }
   }
-  // Work actually performed: distinct window samples and bin-samples.
-  if (tid == 0 && work) {
-    unsigned long long wsamp = 0, bsamp = 0;
-    for (int w = 0; w < nwin; ++w) wsamp += 2ull * L.w_h[w] + 1;
-    for (int tk = 0; tk < ntask; ++tk) bsamp += 2ull * L.w_h[L.t_win[tk]] + 1;
-    atomicAdd(&work[0], wsamp);
-    atomicAdd(&work[1], bsamp);
-    atomicAdd(&work[2], (unsigned long long)ntask);
+  flush(wflush, nwin);
+  for (int s = lane; s < ncand; s += 64)
+    if (L.h[s] == 0) {
+      out_f0[s] = 0.0;
+      out_sc[s] = 0.0;
+    }
+  if (work) {
+    if (fault) atomicOr(&work[3], 1ull);
+    if (lane == 0) {
+      atomicAdd(&work[0], wsamp);
+      atomicAdd(&work[1], bsamp);
+      atomicAdd(&work[2], nbins);
+    }
   }
 }
 
@@ -1375,27 +1393,18 @@ void launch_refine(hipStream_t s, const Config& c, const Utt* utt,
                    const double* y, const double* official, const int* count,
                    const double* twiddle, double* rf0, double* rscore,
                    unsigned long long* work) {
-  // Frame-per-workgroup kernel when its LDS (at most 80 KiB, two
-  // workgroups per 160 KiB CU) leaves a useful window buffer and the (h, k)
-  // hash stays below 75 % load.
-  const size_t fixed = carve_refine_lds(nullptr, c.slots, 0, nullptr);
-  const long long limit = std::min(80 * 1024, c.lds_limit);
-  const long long budget = limit - (long long)fixed - 64;
-  const int bw = budget > 0 ? (int)(budget / (3 * sizeof(double))) : 0;
-  if (6 * c.slots <= (kHash * 3) / 4 && bw >= 512) {
-    const size_t bytes = carve_refine_lds(nullptr, c.slots, bw, nullptr);
-    k_refine_frames<<<(unsigned)total_frames, kThreads, bytes, s>>>(
-        c, utt, f_off, bw, y, official, count, twiddle, rf0, rscore, work);
-    check_launch("refine_frames");
-    return;
-  }
-  const long long n = total_frames * c.slots;
-  if (c.twiddle_n > 0)
-    k_refine<true><<<grid_for(n), kThreads, 0, s>>>(c, utt, f_off, total_frames, y,
-                                                     official, count, twiddle, rf0, rscore);
-  else
-    k_refine<false><<<grid_for(n), kThreads, 0, s>>>(c, utt, f_off, total_frames, y,
-                                                      official, count, twiddle, rf0, rscore);
+  int HS = 64;  // window hash: a power of two >= 2 slots keeps probing short
+  while (HS < 2 * c.slots) HS <<= 1;
+  int W = (window_fft(c.hmax) / 2 + 64) >> 6;
+  if (W > kMaxWords) W = 0;  // very low floors: every window takes the slot-by-slot path
+  const size_t bytes = kWaves * wave_lds_bytes(c.slots, HS, W);
+  if (bytes > (size_t)c.lds_limit)
+    throw std::runtime_error("refinement needs " + std::to_string(bytes) +
+                             " bytes of LDS per workgroup; the device allows " +
+                             std::to_string(c.lds_limit));
+  const unsigned grid = (unsigned)((total_frames + kWaves - 1) / kWaves);
+  k_refine_waves<<<grid, kThreads, bytes, s>>>(c, utt, f_off, total_frames, HS, W, y, official,
+                                               count, twiddle, rf0, rscore, work);
   check_launch("refine");
 }
 
