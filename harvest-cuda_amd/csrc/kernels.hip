@@ -373,10 +373,19 @@ __global__ __launch_bounds__(kThreads) void k_fill_int(int* p, long long n, int 
 // f[i] = sum_{m=0}^{2h} c[m] y[i + h + 1 - m] + Q(i + h + 1) / n reproduces
 // WORLD's filtered_signal[i] / n (harvest.cpp:115-142; c2r scaling is a
 // power of two and cancels in every crossing).  Each thread produces 8
-// consecutive outputs with a register-blocked FIR from LDS, then the four
-// crossing types of harvest.cpp:206-238 are detected per sample pair, their
-// sub-sample edges computed with ZeroCrossingEngine's formula
-// (harvest.cpp:184-186), compacted in order and written to the tile's slab.
+// consecutive outputs with a register-blocked FIR from LDS (a rolling
+// 16-sample window: 8 new samples per 64 FMAs), then the four crossing types
+// of harvest.cpp:206-238 are detected per sample pair, their sub-sample edges
+// computed with ZeroCrossingEngine's formula (harvest.cpp:184-186), compacted
+// in order and written to the tile's slab.
+
+// LDS position of sample s: one pad double after every 8, so that lanes 8
+// samples apart (the FIR's and crossing stage's access pattern) are 9 doubles
+// apart and hit distinct banks.
+__device__ __forceinline__ int pad8(int s) { return s + (s >> 3); }
+
+__host__ __device__ inline size_t pad8_size(int n) { return (size_t)n + (size_t)(n >> 3) + 1; }
+
 template <bool kLds>
 __global__ __launch_bounds__(kThreads) void k_filter_events(
     Config c, const Utt* utt, const long long* t_off, long long total_tiles,
@@ -400,29 +409,41 @@ __global__ __launch_bounds__(kThreads) void k_filter_events(
   if (kLds) {
     for (int s = threadIdx.x; s < span; s += kThreads) {
       long long idx = base + s;
-      smem[s] = (idx >= 0 && idx < N) ? yu[idx] : 0.0;
+      smem[pad8(s)] = (idx >= 0 && idx < N) ? yu[idx] : 0.0;
     }
     __syncthreads();
   }
   auto Y = [&](int s) -> double {
-    if (kLds) return smem[s];
+    if (kLds) return smem[pad8(s)];
     long long idx = base + s;
     return (idx >= 0 && idx < N) ? yu[idx] : 0.0;
   };
 
+  // acc[r] = sum over taps mm in increasing order of cr[mm] y[j0 + r + mm].
   const int j0 = threadIdx.x * 8;
-  double acc[8];
+  double acc[8], va[8], vb[8];
 #pragma unroll
-  for (int r = 0; r < 8; ++r) acc[r] = 0.0;
-  for (int mm = 0; mm < taps8; mm += 8) {
-    double v[16];
+  for (int r = 0; r < 8; ++r) {
+    acc[r] = 0.0;
+    va[r] = Y(j0 + r);
+  }
+  for (int mm = 0; mm < taps8; mm += 16) {
 #pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = Y(j0 + mm + q);
+    for (int q = 0; q < 8; ++q) vb[q] = Y(j0 + mm + 8 + q);
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const double cc = cr[mm + k];
 #pragma unroll
-      for (int r = 0; r < 8; ++r) acc[r] = fma(cc, v[r + k], acc[r]);
+      for (int r = 0; r < 8; ++r) acc[r] = fma(cc, r + k < 8 ? va[r + k] : vb[r + k - 8], acc[r]);
+    }
+    if (mm + 8 >= taps8) break;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) va[q] = Y(j0 + mm + 16 + q);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const double cc = cr[mm + 8 + k];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) acc[r] = fma(cc, r + k < 8 ? vb[r + k] : va[r + k - 8], acc[r]);
     }
   }
   // Two halo outputs (kTile, kTile + 1) for the derivative crossings.
@@ -431,41 +452,53 @@ __global__ __launch_bounds__(kThreads) void k_filter_events(
     const int j = kTile + threadIdx.x;
     for (int mm = 0; mm < taps8; ++mm) halo = fma(cr[mm], Y(j + mm), halo);
   }
-  // Two-bin correction and scale.
+  // Two-bin correction (-1)^k [2 Re(D1 e^{-2 pi i k / n}) + D2] / n at
+  // k = i0 + j + h + 1: the exact twiddle of the thread's first output, then
+  // rotation by 2 pi / n per output.
   const Quirk qk = quirk[(long long)ui * c.bands + b];
   const double inv_n = 1.0 / u.fft;
   const long long nmask = u.fft - 1;
-  auto correction = [&](long long k) -> double {
-    double sn, cs;
-    sincospi(2.0 * (double)(k & nmask) / u.fft, &sn, &cs);
-    double q = 2.0 * (qk.d1re * cs + qk.d1im * sn) + qk.d2;
+  const long long k0 = i0 + j0 + h + 1;
+  double tsn, tcs, rsn, rcs;
+  sincospi(2.0 * (double)(k0 & nmask) / u.fft, &tsn, &tcs);
+  sincospi(2.0 / u.fft, &rsn, &rcs);
+  auto correction_at = [&](long long k, double cs, double sn) -> double {
+    const double q = 2.0 * (qk.d1re * cs + qk.d1im * sn) + qk.d2;
     return (k & 1) ? -q : q;
   };
   if (kLds) __syncthreads();
   double* f = smem;
 #pragma unroll
-  for (int r = 0; r < 8; ++r)
-    f[j0 + r] = acc[r] + correction(i0 + j0 + r + h + 1) * inv_n;
+  for (int r = 0; r < 8; ++r) {
+    f[pad8(j0 + r)] = acc[r] + correction_at(k0 + r, tcs, tsn) * inv_n;
+    const double nc = fma(tcs, rcs, -(tsn * rsn)), ns = fma(tsn, rcs, tcs * rsn);
+    tcs = nc;
+    tsn = ns;
+  }
   if (threadIdx.x < 2) {
     const int j = kTile + threadIdx.x;
-    f[j] = halo + correction(i0 + j + h + 1) * inv_n;
+    const long long k = i0 + j + h + 1;
+    double sn, cs;
+    sincospi(2.0 * (double)(k & nmask) / u.fft, &sn, &cs);
+    f[pad8(j)] = halo + correction_at(k, cs, sn) * inv_n;
   }
   __syncthreads();
 
   // Crossing flags: bit (type * 8 + r).  Types: 0 negative-going, 1
   // positive-going, 2 peak, 3 dip; WORLD's order (harvest.cpp:218-237).
+  auto F = [&](int j) -> double { return f[pad8(j)]; };
   unsigned flags = 0;
 #pragma unroll
   for (int r = 0; r < 8; ++r) {
     const int j = j0 + r;
     const long long i = i0 + j;
     if (i <= N - 2) {
-      const double a = f[j], bb = f[j + 1];
+      const double a = F(j), bb = F(j + 1);
       if (0.0 < a && bb <= 0.0) flags |= 1u << (0 * 8 + r);
       if (a < 0.0 && bb >= 0.0) flags |= 1u << (1 * 8 + r);
     }
     if (i <= N - 3) {
-      const double d0 = f[j + 1] - f[j], d1 = f[j + 2] - f[j + 1];
+      const double d0 = F(j + 1) - F(j), d1 = F(j + 2) - F(j + 1);
       if (0.0 < d0 && d1 <= 0.0) flags |= 1u << (2 * 8 + r);
       if (d0 < 0.0 && d1 >= 0.0) flags |= 1u << (3 * 8 + r);
     }
@@ -488,9 +521,9 @@ __global__ __launch_bounds__(kThreads) void k_filter_events(
       const double e = (double)(i0 + j + 1);
       double fe;
       if (t < 2) {
-        fe = e - f[j] / (f[j + 1] - f[j]);
+        fe = e - F(j) / (F(j + 1) - F(j));
       } else {
-        const double d0 = f[j + 1] - f[j], d1 = f[j + 2] - f[j + 1];
+        const double d0 = F(j + 1) - F(j), d1 = F(j + 2) - F(j + 1);
         fe = e - d0 / (d1 - d0);
       }
       out[pos++] = fe;
@@ -1348,13 +1381,13 @@ void launch_filter_events(hipStream_t s, const Config& c, const Utt* utt,
                           const int* half, const Quirk* quirk, double* slab,
                           int* counts) {
   const int taps8 = (2 * max_half + 1 + 7) & ~7;
-  const size_t lds_bytes = sizeof(double) * (size_t)(kTile + taps8 + 10);
+  const size_t lds_bytes = sizeof(double) * pad8_size(kTile + taps8 + 10);
   dim3 grid((unsigned)total_tiles, (unsigned)group);
   if (lds_bytes <= 64 * 1024) {
     k_filter_events<true><<<grid, kThreads, lds_bytes, s>>>(
         c, utt, t_off, total_tiles, band0, y, coef, coef_off, half, quirk, slab, counts);
   } else {
-    k_filter_events<false><<<grid, kThreads, sizeof(double) * (kTile + 2), s>>>(
+    k_filter_events<false><<<grid, kThreads, sizeof(double) * pad8_size(kTile + 2), s>>>(
         c, utt, t_off, total_tiles, band0, y, coef, coef_off, half, quirk, slab, counts);
   }
   check_launch("filter_events");
