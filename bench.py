@@ -57,6 +57,21 @@ def fir_flops(lengths, fs, floor=71.0, ceil=800.0):
     return 2.0 * taps * n_total
 
 
+def algorithmic_bytes(lengths, fs, floor=71.0, ceil=800.0):
+    """Implementation-independent HBM bytes of the path per call, summed over
+    utterances: A = 8 L + 8 N + 16 (n/2 + 1)(2B + 1) + 16 B F + 24 F C (read x,
+    write y, the signal spectrum and one round trip of each band's product
+    spectrum at WORLD's global FFT size n, the B x F raw candidate matrix, and
+    the F x C candidate slots read and refined; C = 7 round(B / 10))."""
+    from harvest_cuda import _core
+    total = 0.0
+    for L in lengths:
+        p = _core._params(L, fs, floor, ceil, 5.0)
+        n, B, F, C = p["fft_size"], p["bands"], p["frames"], 7 * p["capacity"]
+        total += 8 * L + 8 * p["y_length"] + 16 * (n // 2 + 1) * (2 * B + 1) + 16 * B * F + 24 * F * C
+    return total
+
+
 def roofline(kernels_ms, work, lengths, fs, steps):
     """Roofline object of the kernel with the largest average duration."""
     avg = {k: v / steps for k, v in kernels_ms.items()}
@@ -74,7 +89,7 @@ def roofline(kernels_ms, work, lengths, fs, steps):
         achieved = flops / sec / 1e12
         return name, {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                       "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                      "kernel": "k_refine_frames", "avg_ms": avg[name],
+                      "kernel": "k_refine_waves", "avg_ms": avg[name],
                       "model": "8 FLOP per bin-sample (4 FMA) + 17 FLOP per window sample, "
                                "cosines not counted"}
     # HBM-bound stages: algorithmic bytes of the decimation scan (read x
@@ -189,6 +204,7 @@ def main():
                    "kernel_prune_compact"]
     kernels = {k: 0.0 for k in kernel_keys}
     work = {"window_samples": 0.0, "bin_samples": 0.0}
+    device_ms = 0.0  # hipEvent span of the device launch sequence
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -197,6 +213,7 @@ def main():
             kernels[k] += tm[k]
         work["window_samples"] += tm["work_window_samples"]
         work["bin_samples"] += tm["work_bin_samples"]
+        device_ms += tm["preprocess"] + tm["filter_events"] + tm["refine"]
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed / args.steps * 1000.0
@@ -216,6 +233,14 @@ def main():
         "roofline": roof,
         "kernels_ms_per_step": {k: v / args.steps for k, v in kernels.items()},
     }
+    a_bytes = algorithmic_bytes([len(x)], fs)
+    dev_s = device_ms / args.steps / 1000.0
+    result["algorithmic_hbm"] = {
+        "bytes_per_step": a_bytes, "device_ms": dev_s * 1000.0,
+        "gbps_device_span": a_bytes / dev_s / 1e9, "gbps_wall": a_bytes / (ms_per_step / 1000.0) / 1e9,
+        "frac_of_peak_device_span": a_bytes / dev_s / 1e9 / HBM_PEAK_GBS,
+        "model": "8L + 8N + 16(n/2+1)(2B+1) + 16BF + 24FC (global-n spectra; this design "
+                 "never materialises them, so the figure is a normalised throughput)"}
     if args.cpu_baseline:
         ref_f0, dt, kind = world_baseline(x, fs)
         result["cpu_baseline"] = {"value": seconds / dt, "unit": "audio-sec/sec", "cores": 1,

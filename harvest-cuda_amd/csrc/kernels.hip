@@ -729,7 +729,7 @@ constexpr int kPer = 2;          // bins per lane group per pass
 constexpr int kFrameBins = 128;  // bin results of one frame kept in LDS (< 256)
 constexpr int kStage = 128;      // windowed samples staged per pass (multiple of 64)
 constexpr int kMaxWords = 128;   // bitmap words (bins 0..8191)
-constexpr int kDirectWords = 16; // bitmaps up to this size take a direct prefix
+constexpr int kXcds = 8;         // XCDs that workgroups are dispatched to in turn
 
 // LDS-qualified pointers: accesses through them compile to ds_* operations
 // however the pointers are passed around.
@@ -814,6 +814,13 @@ __device__ __forceinline__ double lane_value(double v, int src) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// Workgroup b of G is dispatched to XCD b mod 8; renumbered so that each XCD
+// takes one contiguous range (its L2 then holds one slice of the signal).
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned G) {
+  const unsigned x = b % kXcds, per = G / kXcds, rem = G % kXcds;
+  return x * per + min(x, rem) + b / kXcds;
+}
+
 // FFT size of GetMeanF0 for half window h: 2^(2 + floor(log2(2h + 1))).
 __host__ __device__ inline int window_fft(int h) {
   int lg = 0;
@@ -857,7 +864,7 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
   const WaveLds L =
       carve_wave_lds((LDS char*)(char*)lds_raw + wave * wave_lds_bytes(S, HS, W), S, HS, W);
 
-  const long long gf = (long long)blockIdx.x * kWaves + wave;
+  const long long gf = (long long)xcd_block(blockIdx.x, gridDim.x) * kWaves + wave;
   if (gf >= total_frames) return;
   const int ui = find_segment(f_off, c.count, gf);
   const int cnt = count[ui];
@@ -868,6 +875,9 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
   const double fs = c.fs_work;
   const double t = (double)i / 1000.0;
   const double* yu = y + u.y_off;
+  auto y_at = [&](int idx) -> double {  // GetBaseIndex's clamping (harvest.cpp:434-441)
+    return yu[idx < 0 ? 0 : (idx > u.n - 1 ? u.n - 1 : idx)];
+  };
   double* out_f0 = rf0 + gf * S;
   double* out_sc = rscore + gf * S;
   unsigned long long wsamp = 0, bsamp = 0, nbins = 0;
@@ -913,6 +923,31 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
   wave_sync();
   const int nwin = L.misc[0];
   for (int s = lane; s < ncand; s += 64) L.win[s] = L.win[s] >= 0 ? L.hval[L.win[s]] : -1;
+  // Per-window constants, lane w holding those of window w < 64: the first
+  // sample index (GetBaseIndex), the Blackman phase offset a and the rotation
+  // by 64 samples; windows from 64 on compute them in the loop.
+  int pw_basic = 0;
+  double pw_a = 0.0, pw_gc = 1.0, pw_gs = 0.0;
+  if (lane < nwin) {
+    const int hw = L.win_h[lane];
+    pw_basic = mround((t + (-hw) / fs) * fs + 0.001);
+    pw_a = (double)(pw_basic - 1) - t * fs;
+    sincospi(128.0 / (2 * hw + 1), &pw_gs, &pw_gc);
+  }
+  auto window_basic = [&](int w) -> int {
+    if (w < 64) return __builtin_amdgcn_readlane(pw_basic, w);
+    return mround((t + (-L.win_h[w]) / fs) * fs + 0.001);
+  };
+  // The first 128 windowed samples of the next window are loaded ahead into
+  // registers, overlapping their latency with the current window's work.
+  double ypre0 = 0.0, ypre1 = 0.0;
+  auto prefetch = [&](int w) {
+    if (w >= nwin) return;
+    const int bs = window_basic(w), len = 2 * L.win_h[w] + 1;
+    ypre0 = lane < len ? y_at(bs + lane - 1) : 0.0;
+    ypre1 = lane + 64 < len ? y_at(bs + lane + 63) : 0.0;
+  };
+  prefetch(0);
   wave_sync();
   // Per-window slot lists: counts (hkey), starts (hval), lists (wslots).
   LDS int* wcnt = L.hkey;
@@ -979,11 +1014,20 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
     const int tstride = stride / n;  // 0: no table covers n
     // Blackman argument (harvest.cpp:446-456) in units of pi:
     // 2 tmp / wl with tmp = (basic + m - 1) / fs - t and wl = len / fs.
-    const int basic = mround((t + (-hw) / fs) * fs + 0.001);
-    const double a = (double)(basic - 1) - t * fs;
-    double wc0, ws0, gc, gs;
+    int basic;
+    double a, gc, gs;
+    if (w < 64) {
+      basic = __builtin_amdgcn_readlane(pw_basic, w);
+      a = lane_value(pw_a, w);
+      gc = lane_value(pw_gc, w);
+      gs = lane_value(pw_gs, w);
+    } else {
+      basic = window_basic(w);
+      a = (double)(basic - 1) - t * fs;
+      sincospi(128.0 / len, &gs, &gc);  // advance by 64 samples
+    }
+    double wc0, ws0;
     sincospi(2.0 * (a + lane) / len, &ws0, &wc0);  // sample m = lane
-    sincospi(128.0 / len, &gs, &gc);               // advance by 64 samples
     const int nstage = (len + kStage - 1) / kStage;
     bool staged = false;
 
@@ -1029,9 +1073,8 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
                 if (m == 0) d = -wn / 2.0;
                 else if (m == len - 1) d = wp / 2.0;
                 else d = -(wn - wp) / 2.0;
-                int idx = basic + m - 1;
-                idx = idx < 0 ? 0 : (idx > u.n - 1 ? u.n - 1 : idx);
-                const double xv = yu[idx];
+                const double xv =
+                    s0 == 0 ? ypre0 : (s0 == 64 ? ypre1 : y_at(basic + m - 1));
                 L.xwd[2 * (m - c0)] = xv * cur;
                 L.xwd[2 * (m - c0) + 1] = xv * d;
               }
@@ -1100,13 +1143,15 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
         }
       }
       wave_sync();
-      if (words <= kDirectWords) {
-        int ex = 0;
-        if (lane < words)
-          for (int v = 0; v < lane; ++v) ex += __popcll(L.bits[v]);
-        const int incl = ex + (lane < words ? __popcll(L.bits[lane]) : 0);
+      if (words <= 64) {
+        const int pc = lane < words ? __popcll(L.bits[lane]) : 0;
+        int run = 0, ex = 0;
+        for (int v = 0; v < words; ++v) {
+          if (lane == v) ex = run;
+          run += __builtin_amdgcn_readlane(pc, v);
+        }
         if (lane < words) L.wpre[lane] = ex;
-        nb = __builtin_amdgcn_readlane(incl, words - 1);
+        nb = run;
       } else {
         int carry = 0;
         for (int base = 0; base < words; base += 64) {
@@ -1136,6 +1181,7 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
       }
       wave_sync();
       dft_bins(rbase, nb);
+      prefetch(w + 1);
       for (int p = lane; p < pairs; p += 64) {
         const int s = L.wslots[b + p / 6], hh = p % 6;
         if (hh < L.nh[s]) {
@@ -1163,6 +1209,7 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
         if (lane == 0) finish(s);
         wave_sync();
       }
+      prefetch(w + 1);
       wflush = w + 1;
       rbase = 0;
     }
