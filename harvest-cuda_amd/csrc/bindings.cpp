@@ -79,7 +79,7 @@ PYBIND11_MODULE(_core, m) {
     {
       py::gil_scoped_release release;
       Cache::Lease ws(cache(), device);
-      r = analyze(ws.get(), p, n, fs, floor, ceil, period);
+      r = analyze(ws.get(), p, n, fs, floor, ceil, period, host_threads());
     }
     return py::make_tuple(to_numpy(std::move(r.f0)), to_numpy(std::move(r.times)));
   }, py::arg("x").noconvert(), py::arg("fs"), py::arg("f0_floor") = 71.0,
@@ -154,7 +154,7 @@ PYBIND11_MODULE(_core, m) {
     {
       py::gil_scoped_release release;
       Cache::Lease ws(cache(), device);
-      r = analyze(ws.get(), p, n, fs, floor, ceil, period, &t);
+      r = analyze(ws.get(), p, n, fs, floor, ceil, period, host_threads(), &t);
     }
     return py::make_tuple(to_numpy(std::move(r.f0)), to_numpy(std::move(r.times)), t);
   }, py::arg("x").noconvert(), py::arg("fs"), py::arg("f0_floor") = 71.0,
@@ -214,19 +214,31 @@ PYBIND11_MODULE(_core, m) {
 
   // ---- host contour stages (CPU only) for bit-exactness tests ----
   m.def("_fix_contour", [](py::array_t<int32_t, py::array::c_style> offsets, Input f0,
-                           Input scores, Input base) {
+                           Input scores, Input base, int threads) {
     const int frames = (int)base.size();
     if ((int)offsets.size() != frames + 1) throw std::invalid_argument("offsets must have frames + 1 entries");
+    if (f0.size() != scores.size()) throw std::invalid_argument("f0 and scores differ in length");
+    const int32_t* o = offsets.data();
+    if (o[0] != 0 || o[frames] != (int64_t)f0.size())
+      throw std::invalid_argument("offsets must start at 0 and end at the candidate count");
+    for (int i = 0; i < frames; ++i)
+      if (o[i + 1] < o[i]) throw std::invalid_argument("offsets must be non-decreasing");
     std::vector<double> out(frames);
-    CandidateRows rows{offsets.data(), f0.data(), scores.data()};
-    fix_contour(rows, frames, base.data(), out.data());
+    CandidateRows rows{o, f0.data(), scores.data()};
+    {
+      py::gil_scoped_release release;
+      fix_contour(rows, frames, base.data(), out.data(), std::max(1, threads));
+    }
     return to_numpy(std::move(out));
-  });
-  m.def("_smooth", [](Input f0) {
+  }, py::arg("offsets"), py::arg("f0"), py::arg("scores"), py::arg("base"), py::arg("threads") = 1);
+  m.def("_smooth", [](Input f0, int threads) {
     std::vector<double> out(f0.size());
-    smooth_contour(f0.data(), (int)f0.size(), out.data());
+    {
+      py::gil_scoped_release release;
+      smooth_contour(f0.data(), (int)f0.size(), out.data(), std::max(1, threads));
+    }
     return to_numpy(std::move(out));
-  });
+  }, py::arg("f0"), py::arg("threads") = 1);
   m.def("_params", [](int64_t length, int fs, double floor, double ceil, double period) {
     Params p = derive_params(length, fs, floor, ceil, period);
     py::dict d;

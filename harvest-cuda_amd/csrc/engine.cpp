@@ -1,5 +1,5 @@
 // Host engine: WORLD's parameter derivation, the backend call, and the
-// sequential contour stages in O(F).  Reference counterpart: src/engine.cpp
+// contour stages in O(F) (voiced sections on several threads for long inputs).  Reference counterpart: src/engine.cpp
 // (analyze :240-333, contour adapters :55-214).  This file is built with
 // -ffp-contract=off; every floating-point expression below keeps WORLD's
 // evaluation order because each one feeds a discrete decision.
@@ -11,6 +11,8 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <exception>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <thread>
@@ -21,6 +23,44 @@ namespace {
 constexpr double kLog2 = 0.69314718055994529;  // constantnumbers.h:24
 
 int mround(double x) { return x > 0 ? (int)(x + 0.5) : (int)(x - 0.5); }
+
+// Contours shorter than this stay on the calling thread: spawning threads
+// costs more than the per-section work they would share.
+constexpr int kParallelFrames = 16384;
+// Threads per utterance for its sections; more gain nothing on ~100 sections.
+constexpr int kSectionThreads = 8;
+
+// Runs f(i) for i in [0, n) on up to `threads` threads, the caller included.
+// The first exception thrown by f is rethrown once every thread has joined;
+// if the system refuses a thread, the remaining ones share the work.
+template <class F> void parallel_for(int n, int threads, F&& f) {
+  threads = std::max(1, std::min(threads, n));
+  if (threads == 1) {
+    for (int i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::atomic<int> next{0};
+  std::exception_ptr error;
+  std::mutex error_mutex;
+  auto work = [&] {
+    try {
+      for (int i = next++; i < n; i = next++) f(i);
+    } catch (...) {
+      std::lock_guard<std::mutex> guard(error_mutex);
+      if (!error) error = std::current_exception();
+      next = n;
+    }
+  };
+  std::vector<std::thread> pool;
+  pool.reserve(threads - 1);
+  try {
+    for (int t = 1; t < threads; ++t) pool.emplace_back(work);
+  } catch (...) {
+  }
+  work();
+  for (auto& t : pool) t.join();
+  if (error) std::rethrow_exception(error);
+}
 
 // Sections of a contour as GetBoundaryList (harvest.cpp:727-743) reports
 // them: entries alternate start (first voiced frame) and end (last voiced
@@ -69,12 +109,19 @@ struct Section {
 };
 
 // SelectBestF0 (harvest.cpp:636-650) over the nonzero candidates of a frame;
-// zero candidates have error 1 and never beat the 0.18 allowance.
+// zero candidates have error 1 and never beat the 0.18 allowance.  A
+// candidate with |reference - c| > error * reference * (1 + 1e-14) has a
+// rounded quotient above `error` (the margin exceeds every rounding involved),
+// so it is rejected without the division; all others take WORLD's expression.
 double select_best(double reference, const CandidateRows& rows, int frame) {
   double best = 0.0, error = 0.18;
+  double bound = error * reference * (1.0 + 1e-14);
   for (int k = rows.offsets[frame]; k < rows.offsets[frame + 1]; ++k) {
-    const double e = std::fabs(reference - rows.f0[k]) / reference;
+    const double d = std::fabs(reference - rows.f0[k]);
+    if (d > bound) continue;
+    const double e = d / reference;
     if (e > error) continue;
+    bound = e * reference * (1.0 + 1e-14);
     best = rows.f0[k];
     error = e;
   }
@@ -126,12 +173,13 @@ void sorted_order(const std::vector<int>& list, int n, std::vector<int>& order) 
 // FixStep3 (harvest.cpp:968-995) in O(F): each section keeps only its own
 // neighbourhood instead of a full-length channel.
 void fix_step3(const double* in, int frames, const CandidateRows& rows, double* out,
-               std::vector<int>& list) {
+               std::vector<int>& list, int threads) {
   std::copy(in, in + frames, out);
   boundary_list(in, frames, list);
   const int count = (int)list.size() / 2;
   std::vector<Section> sections(count);
-  for (int i = 0; i < count; ++i) {
+  // Each section's ExtendF0 reads only the input and the candidates.
+  parallel_for(count, frames >= kParallelFrames ? threads : 1, [&](int i) {
     Section& s = sections[i];
     const int st = list[2 * i], ed = list[2 * i + 1];
     s.offset = std::max(0, st - 101);
@@ -139,7 +187,7 @@ void fix_step3(const double* in, int frames, const CandidateRows& rows, double* 
     std::copy(in + st, in + ed + 1, s.values.begin() + (st - s.offset));
     s.end = extend(s, ed, std::min(frames - 2, ed + 100), 1, rows);
     s.start = extend(s, st, std::max(1, st - 100), -1, rows);
-  }
+  });
   // ExtendSub (harvest.cpp:840-856): the running mean is carried across
   // sections, and selected sections move to the front in order.
   int selected = 0;
@@ -354,13 +402,14 @@ void validate(const double* x, int64_t length, int fs, double f0_floor, double f
   if (!finite) throw std::invalid_argument("x must contain only finite samples");
 }
 
-void fix_contour(const CandidateRows& rows, int frames, const double* base, double* out) {
+void fix_contour(const CandidateRows& rows, int frames, const double* base, double* out,
+                 int threads) {
   std::vector<double> a(frames), b(frames);
   std::vector<int> list;
   list.reserve(64);
   fix_step1(base, frames, a.data());
   fix_step2(a.data(), frames, b.data(), list);
-  fix_step3(b.data(), frames, rows, a.data(), list);
+  fix_step3(b.data(), frames, rows, a.data(), list, threads);
   fix_step4(a.data(), frames, out, list);
 }
 
@@ -371,22 +420,27 @@ void fix_contour(const CandidateRows& rows, int frames, const double* base, doub
 // in the backward pass, eventually periodic) input, so those stretches are
 // advanced exactly with iterate() instead of sample by sample: every output
 // stays bit-identical to WORLD's, in O(section + transient) per section.
-void smooth_contour(const double* f0, int frames, double* out) {
+void smooth_contour(const double* f0, int frames, double* out, int threads) {
   constexpr int lag = 300;
   const long n = (long)frames + 2 * lag;
   auto padded = [&](long i) { return i >= lag && i < lag + frames ? f0[i - lag] : 0.0; };
   for (int i = 0; i < frames; ++i) out[i] = 0.0;
-  std::vector<double> section_out, tail;
-  long i = 1;
-  while (i < n - 1) {
+  // Voiced sections [st, ed] of the padded contour; each writes only its own
+  // outputs, so they run independently.
+  std::vector<std::pair<long, long>> sections;
+  for (long i = 1; i < n - 1;) {
     if (!(padded(i) > 0)) {
       ++i;
       continue;
     }
-    const long st = i;
     long ed = i;
     while (ed + 1 < n - 1 && padded(ed + 1) > 0) ++ed;
+    sections.emplace_back(i, ed);
     i = ed + 1;
+  }
+  parallel_for((int)sections.size(), frames >= kParallelFrames ? threads : 1, [&](int si) {
+    const long st = sections[si].first, ed = sections[si].second;
+    std::vector<double> section_out, tail;
     // Forward pass: st steps of the constant x[st] from rest, then the section.
     const double head = padded(st);
     Smoother fwd = iterate(Smoother{}, st, [&](Smoother& s) { s.step(head); });
@@ -431,7 +485,7 @@ void smooth_contour(const double* f0, int frames, double* out) {
     }
     for (; k >= 0; --k) bwd.step(suffix_out(k));
     for (long q = ed; q >= st; --q) out[q - lag] = bwd.step(section_out[q - st]);
-  }
+  });
 }
 
 void resample(const double* basic, int frames, double frame_period, int out_frames, double* f0,
@@ -470,13 +524,13 @@ double ms_since(Clock::time_point& mark) {
 
 void finish_utterance(const CandidateRows& rows, int frames, const double* best,
                       double frame_period, int out_frames, Result& r, HostArena* arena,
-                      Timings* timings, Clock::time_point* mark) {
+                      Timings* timings, Clock::time_point* mark, int threads) {
   HostArray<double> contour(arena), basic(arena);
   contour.allocate(frames);
   basic.allocate(frames);
-  fix_contour(rows, frames, best, contour.data());
+  fix_contour(rows, frames, best, contour.data(), threads);
   if (timings) (*timings)["contour"] += ms_since(*mark);
-  smooth_contour(contour.data(), frames, basic.data());
+  smooth_contour(contour.data(), frames, basic.data(), threads);
   if (timings) (*timings)["smooth"] += ms_since(*mark);
   r.f0.resize(out_frames);
   r.times.resize(out_frames);
@@ -501,7 +555,7 @@ View result_view(hv_workspace* ws) {
 }  // namespace
 
 Result analyze(hv_workspace* ws, const double* x, int64_t length, int fs, double f0_floor,
-               double f0_ceil, double frame_period, Timings* timings) {
+               double f0_ceil, double frame_period, int host_threads, Timings* timings) {
   Clock::time_point mark = Clock::now();
   if (timings) timings->clear();
   const Params p = derive_params(length, fs, f0_floor, f0_ceil, frame_period);
@@ -521,7 +575,7 @@ Result analyze(hv_workspace* ws, const double* x, int64_t length, int fs, double
   Result r;
   CandidateRows rows{v.offsets, v.f0, v.scores};
   finish_utterance(rows, p.frames, v.best, frame_period, p.out_frames, r, lease.arena(), timings,
-                   &mark);
+                   &mark, std::min(host_threads, kSectionThreads));
   return r;
 }
 
@@ -555,20 +609,15 @@ std::vector<Result> batch_common(hv_workspace* ws, Call&& call,
   std::iota(order.begin(), order.end(), 0);
   std::sort(order.begin(), order.end(),
             [&](int a, int b) { return params[a].frames > params[b].frames; });
-  std::atomic<int> next{0};
-  auto worker = [&] {
-    for (int k = next++; k < count; k = next++) {
-      const int u = order[k];
-      CandidateRows rows{v.offsets + frame_off[u], v.f0, v.scores};
-      finish_utterance(rows, params[u].frames, v.best + frame_off[u], frame_period,
-                       params[u].out_frames, results[u], nullptr, nullptr, nullptr);
-    }
-  };
-  const int threads = std::max(1, std::min(host_threads, count));
-  std::vector<std::thread> pool;
-  for (int t = 1; t < threads; ++t) pool.emplace_back(worker);
-  worker();
-  for (auto& t : pool) t.join();
+  // Threads left over when there are fewer utterances than threads work
+  // inside each utterance (per voiced section).
+  const int inner = std::max(1, std::min(kSectionThreads, host_threads / std::max(1, count)));
+  parallel_for(count, host_threads, [&](int k) {
+    const int u = order[k];
+    CandidateRows rows{v.offsets + frame_off[u], v.f0, v.scores};
+    finish_utterance(rows, params[u].frames, v.best + frame_off[u], frame_period,
+                     params[u].out_frames, results[u], nullptr, nullptr, nullptr, inner);
+  });
   if (timings) (*timings)["contour"] = ms_since(mark);
   return results;
 }

@@ -43,7 +43,8 @@ void validate(const double* x, int64_t length, int fs, double f0_floor, double f
 
 // Plain analysis of one waveform on the leased workspace.
 Result analyze(hv_workspace* ws, const double* x, int64_t length, int fs, double f0_floor,
-               double f0_ceil, double frame_period, Timings* timings = nullptr);
+               double f0_ceil, double frame_period, int host_threads = 1,
+               Timings* timings = nullptr);
 // Batched analysis: one device launch sequence for all utterances, then the
 // host contour of every utterance on a pool of host threads.
 std::vector<Result> analyze_batch(hv_workspace* ws, const std::vector<const double*>& xs,
@@ -59,9 +60,12 @@ std::vector<Result> analyze_batch_device(hv_workspace* ws, const double* x_devic
 
 // Contour stages, exposed for the bit-exactness tests against WORLD.
 // FixF0Contour (harvest.cpp:1027-1044) from SearchF0Base's output `base`.
-void fix_contour(const CandidateRows& rows, int frames, const double* base, double* out);
+// Voiced sections are extended on up to `threads` threads.
+void fix_contour(const CandidateRows& rows, int frames, const double* base, double* out,
+                 int threads = 1);
 // SmoothF0Contour (harvest.cpp:1079-1113), bit-exact, O(F + sections * settle).
-void smooth_contour(const double* f0, int frames, double* out);
+// Sections are smoothed on up to `threads` threads.
+void smooth_contour(const double* f0, int frames, double* out, int threads = 1);
 // Resampling to the output frame period (harvest.cpp:1246-1251).
 void resample(const double* basic, int frames, double frame_period, int out_frames,
               double* f0, double* times);

@@ -116,6 +116,26 @@ def test_smoothing_long_sections_settle_exactly(core, oracle):
     np.testing.assert_array_equal(core._smooth(f0), oracle.smooth(f0))
 
 
+@pytest.mark.parametrize("threads", [2, 3, 8])
+def test_threaded_sections_are_bit_identical(core, oracle, threads):
+    """Long contours (>= 16384 frames) extend and smooth their voiced sections
+    on several threads; every output equals the single-thread result and the
+    oracle's."""
+    rng = np.random.default_rng(4242 + threads)
+    for trial in range(3):
+        f0, sc = random_matrix(rng, trial + 1)
+        reps = -(-20000 // f0.shape[0])
+        f0, sc = np.tile(f0, (reps, 1)), np.tile(sc, (reps, 1))
+        off, cf, cs = csr(f0, sc)
+        base = oracle.base(f0, sc, f0.shape[1])
+        one = core._fix_contour(off, cf, cs, base)
+        many = core._fix_contour(off, cf, cs, base, threads=threads)
+        np.testing.assert_array_equal(many, one)
+        np.testing.assert_array_equal(many, oracle.fix_contour(f0, sc, f0.shape[1]))
+        np.testing.assert_array_equal(core._smooth(many, threads=threads), core._smooth(one))
+        np.testing.assert_array_equal(core._smooth(many, threads=threads), oracle.smooth(one))
+
+
 needs_ref = pytest.mark.skipif("not __import__('oracle.oracle').oracle.ref_available()",
                                reason="WORLD tap (oracle/_ref) not built here")
 
