@@ -63,7 +63,7 @@ def build_module() -> Path:
         "g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-pthread", "-ffp-contract=off",
         "-fvisibility=hidden",
         f"-I{pybind11.get_include()}", f"-I{sysconfig.get_paths()['include']}",
-        str(CSRC / "engine.cpp"), str(CSRC / "bindings.cpp"),
+        str(CSRC / "engine.cpp"), str(CSRC / "adjoint.cpp"), str(CSRC / "bindings.cpp"),
         f"-L{PKG}", "-lharvest_hip", "-Wl,-rpath,$ORIGIN", "-o", str(out),
     ]
     _run(cmd)

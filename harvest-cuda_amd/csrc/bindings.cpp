@@ -13,6 +13,7 @@
 #include <thread>
 #include <vector>
 
+#include "adjoint.h"
 #include "engine.h"
 #include "workspace_cache.h"
 
@@ -211,6 +212,141 @@ PYBIND11_MODULE(_core, m) {
   }, py::arg("xs"), py::arg("fs"), py::arg("f0_floor") = 71.0, py::arg("f0_ceil") = 800.0,
      py::kw_only(), py::arg("device") = 0,
      "Device CSR of pruned refined candidates and the best f0 per 1 ms frame.");
+
+  // ---- differentiable form (frozen decisions) ----
+  py::class_<DiffRecord>(m, "_DiffRecord",
+                         "Decisions of a recorded forward pass (frozen-decision mode).")
+      .def_property_readonly("entries", [](const DiffRecord& r) { return r.entry_frame.size(); })
+      .def_property_readonly("lengths", [](const DiffRecord& r) { return r.lengths; })
+      .def_property_readonly("fs", [](const DiffRecord& r) { return r.fs; })
+      .def_property_readonly("out_frames", [](const DiffRecord& r) {
+        std::vector<int> v;
+        for (const auto& u : r.utts) v.push_back(u.out_frames);
+        return v;
+      })
+      .def_property_readonly("frame_period", [](const DiffRecord& r) { return r.frame_period; })
+      .def_property_readonly("f0_floor", [](const DiffRecord& r) { return r.f0_floor; })
+      .def_property_readonly("f0_ceil", [](const DiffRecord& r) { return r.f0_ceil; })
+      .def_property_readonly("entry_frame", [](const DiffRecord& r) { return r.entry_frame; })
+      .def_property_readonly("entry_init", [](const DiffRecord& r) { return r.entry_init; })
+      .def_property_readonly("sources", [](const DiffRecord& r) {
+        py::list out;
+        for (const auto& u : r.utts) out.append(py::cast(u.source));
+        return out;
+      })
+      .def_property_readonly("step3", [](const DiffRecord& r) {
+        py::list out;
+        for (const auto& u : r.utts) out.append(py::cast(u.step3));
+        return out;
+      })
+      .def_property_readonly("frame_base", [](const DiffRecord& r) {
+        std::vector<int64_t> v;
+        for (const auto& u : r.utts) v.push_back(u.frame_base);
+        return v;
+      });
+
+  auto collect = [](py::list xs, std::vector<Input>& keep, std::vector<const double*>& ptrs,
+                    std::vector<int64_t>& lengths) {
+    for (py::handle h : xs) {
+      if (!Input::check_(h)) throw py::type_error("every utterance must be a 1-D C-contiguous float64 array");
+      Input a = py::reinterpret_borrow<Input>(h);
+      keep.push_back(a);
+      ptrs.push_back(data_1d(a));
+      lengths.push_back((int64_t)a.size());
+    }
+  };
+  auto check_lengths = [](const DiffRecord& rec, const std::vector<int64_t>& lengths) {
+    if (lengths != rec.lengths)
+      throw std::invalid_argument("waveform lengths differ from the recorded forward pass");
+  };
+
+  m.def("_harvest_record", [collect](py::list xs, int fs, double floor, double ceil, double period,
+                                     int device) {
+    std::vector<Input> keep;
+    std::vector<const double*> ptrs;
+    std::vector<int64_t> lengths;
+    collect(xs, keep, ptrs, lengths);
+    if (ptrs.empty()) throw std::invalid_argument("no waveforms");
+    for (size_t u = 0; u < ptrs.size(); ++u) validate(ptrs[u], lengths[u], fs, floor, ceil, period);
+    auto rec = std::make_unique<DiffRecord>();
+    std::vector<Result> rs;
+    {
+      py::gil_scoped_release release;
+      Cache::Lease ws(cache(), device);
+      rs = analyze_record(ws.get(), ptrs, lengths, fs, floor, ceil, period, host_threads(), *rec);
+    }
+    py::list out;
+    for (auto& r : rs) out.append(py::make_tuple(to_numpy(std::move(r.f0)), to_numpy(std::move(r.times))));
+    return py::make_tuple(out, std::move(rec));
+  }, py::arg("xs"), py::arg("fs"), py::arg("f0_floor") = 71.0, py::arg("f0_ceil") = 800.0,
+     py::arg("frame_period") = 5.0, py::kw_only(), py::arg("device") = 0,
+     "Batched forward that also records its decisions: ([(f0, t)], record).");
+
+  m.def("_harvest_frozen", [collect, check_lengths](const DiffRecord& rec, py::list xs, int device) {
+    std::vector<Input> keep;
+    std::vector<const double*> ptrs;
+    std::vector<int64_t> lengths;
+    collect(xs, keep, ptrs, lengths);
+    check_lengths(rec, lengths);
+    for (size_t u = 0; u < ptrs.size(); ++u)
+      validate(ptrs[u], lengths[u], rec.fs, rec.f0_floor, rec.f0_ceil, rec.frame_period);
+    std::vector<std::vector<double>> f0s;
+    {
+      py::gil_scoped_release release;
+      Cache::Lease ws(cache(), device);
+      f0s = frozen_forward(ws.get(), ptrs, rec);
+    }
+    py::list out;
+    for (auto& f : f0s) out.append(to_numpy(std::move(f)));
+    return out;
+  }, py::arg("record"), py::arg("xs"), py::kw_only(), py::arg("device") = 0,
+     "Output F0 of (perturbed) waveforms with the record's decisions held.");
+
+  m.def("_harvest_adjoint", [collect, check_lengths](const DiffRecord& rec, py::list xs,
+                                                     py::list g_f0, int device) {
+    std::vector<Input> keep, gkeep;
+    std::vector<const double*> ptrs, gptrs;
+    std::vector<int64_t> lengths, glengths;
+    collect(xs, keep, ptrs, lengths);
+    collect(g_f0, gkeep, gptrs, glengths);
+    check_lengths(rec, lengths);
+    if (glengths.size() != rec.utts.size())
+      throw std::invalid_argument("one output gradient per waveform");
+    for (size_t u = 0; u < glengths.size(); ++u)
+      if (glengths[u] != rec.utts[u].out_frames)
+        throw std::invalid_argument("output gradient length differs from the output frames");
+    for (size_t u = 0; u < ptrs.size(); ++u)
+      validate(ptrs[u], lengths[u], rec.fs, rec.f0_floor, rec.f0_ceil, rec.frame_period);
+    std::vector<std::vector<double>> gx;
+    {
+      py::gil_scoped_release release;
+      Cache::Lease ws(cache(), device);
+      gx = adjoint(ws.get(), ptrs, rec, gptrs);
+    }
+    py::list out;
+    for (auto& g : gx) out.append(to_numpy(std::move(g)));
+    return out;
+  }, py::arg("record"), py::arg("xs"), py::arg("g_f0"), py::kw_only(), py::arg("device") = 0,
+     "Gradient of sum(g_f0 * f0) with respect to the waveforms, decisions held.");
+
+  // ---- host contour adjoints (CPU only) for tests ----
+  m.def("_smooth_adjoint", [](Input f0, Input g) {
+    if (g.size() != f0.size()) throw std::invalid_argument("length mismatch");
+    std::vector<double> out(f0.size());
+    smooth_adjoint(f0.data(), (int)f0.size(), g.data(), out.data());
+    return to_numpy(std::move(out));
+  });
+  m.def("_fix_step4", [](Input step3) {
+    std::vector<double> out(step3.size());
+    fix_step4_apply(step3.data(), (int)step3.size(), out.data());
+    return to_numpy(std::move(out));
+  });
+  m.def("_fix_step4_adjoint", [](Input step3, Input g) {
+    if (g.size() != step3.size()) throw std::invalid_argument("length mismatch");
+    std::vector<double> out(step3.size());
+    fix_step4_adjoint(step3.data(), (int)step3.size(), g.data(), out.data());
+    return to_numpy(std::move(out));
+  });
 
   // ---- host contour stages (CPU only) for bit-exactness tests ----
   m.def("_fix_contour", [](py::array_t<int32_t, py::array::c_style> offsets, Input f0,

@@ -178,9 +178,11 @@ struct hv_workspace {
   DevBuf x, pass, states, carries, y, partial, mean, partial3, sums3, quirk;
   DevBuf slab, counts, offs, totals, raw, official, count;
   DevBuf rf0, rsc, pf0, psc, best, row_nnz, offsets, scan_tmp;
-  DevBuf utt, c_off, t_off, f_off, p_off, y_off;
+  DevBuf utt, c_off, t_off, f_off, p_off, y_off, x_off;
+  DevBuf cinit;                                        // initial f0 of each CSR entry
+  DevBuf full, gy, gx, ent_frame, ent_f0, ent_w, ent_out;  // frozen-decision adjoint
   DevBuf coef, coef_off, half, bounds, twiddle, work;
-  PinnedBuf h_x, h_meta, h_offsets, h_best, h_cand, h_work;
+  PinnedBuf h_x, h_meta, h_offsets, h_best, h_cand, h_work, h_ent, h_gx;
   // Filter-bank cache key (working rate, floor, ceil).
   double key_fs = -1.0, key_floor = -1.0, key_ceil = -1.0;
   std::vector<double> boundaries;
@@ -214,7 +216,7 @@ namespace {
 struct Plan {
   Config cfg{};
   std::vector<Utt> utts;
-  std::vector<long long> c_off, t_off, f_off, p_off, y_off;
+  std::vector<long long> c_off, t_off, f_off, p_off, y_off, x_off;
   long long total_x = 0, total_m = 0, total_y = 0, total_frames = 0;
   long long total_tiles = 0, total_chunks = 0, total_pblocks = 0;
 };
@@ -371,6 +373,7 @@ void layout(Plan& p, const std::vector<long long>& lengths, const std::vector<in
   p.f_off.assign(U + 1, 0);
   p.p_off.assign(U + 1, 0);
   p.y_off.assign(U + 1, 0);
+  p.x_off.assign(U + 1, 0);
   long long x = 0, m = 0;
   for (int u = 0; u < U; ++u) {
     Utt& t = p.utts[u];
@@ -402,6 +405,7 @@ void layout(Plan& p, const std::vector<long long>& lengths, const std::vector<in
     p.p_off[u + 1] = p.p_off[u] + t.pblocks;
     x += t.length;
     m += t.m;
+    p.x_off[u + 1] = x;
   }
   p.total_x = x;
   p.total_m = m;
@@ -417,15 +421,18 @@ void layout(Plan& p, const std::vector<long long>& lengths, const std::vector<in
 // Upload the plan's descriptor arrays through one pinned staging buffer.
 void upload_plan(hv_workspace* ws, const Plan& p) {
   const int U = p.cfg.count;
-  const size_t bytes = U * sizeof(Utt) + 5 * (U + 1) * sizeof(long long);
+  constexpr int kArrays = 6;
+  const size_t bytes = U * sizeof(Utt) + kArrays * (U + 1) * sizeof(long long);
   char* h = ws->h_meta.ensure<char>(bytes);
   std::memcpy(h, p.utts.data(), U * sizeof(Utt));
   size_t at = U * sizeof(Utt);
-  const std::vector<long long>* arrays[5] = {&p.c_off, &p.t_off, &p.f_off, &p.p_off, &p.y_off};
-  DevBuf* dst[5] = {&ws->c_off, &ws->t_off, &ws->f_off, &ws->p_off, &ws->y_off};
+  const std::vector<long long>* arrays[kArrays] = {&p.c_off, &p.t_off, &p.f_off,
+                                                   &p.p_off, &p.y_off, &p.x_off};
+  DevBuf* dst[kArrays] = {&ws->c_off, &ws->t_off, &ws->f_off,
+                          &ws->p_off, &ws->y_off, &ws->x_off};
   check(hipMemcpyAsync(ws->utt.ensure<Utt>(U), h, U * sizeof(Utt), hipMemcpyHostToDevice,
                        ws->stream), "upload plan");
-  for (int k = 0; k < 5; ++k) {
+  for (int k = 0; k < kArrays; ++k) {
     std::memcpy(h + at, arrays[k]->data(), (U + 1) * sizeof(long long));
     check(hipMemcpyAsync(dst[k]->ensure<long long>(U + 1), h + at, (U + 1) * sizeof(long long),
                          hipMemcpyHostToDevice, ws->stream), "upload plan");
@@ -501,8 +508,8 @@ void analyze_device(hv_workspace* ws, const Plan& p, int32_t* offsets, double* b
                       pf0, psc, dbest, row_nnz);
     launch_exclusive_scan(s, row_nnz, doffsets, p.total_frames, scan_tmp, nb + 1);
     // Compacted candidates reuse the refinement buffers (no longer needed).
-    launch_compact(s, c, dutt, ws->f_off.get<long long>(), p.total_frames, count, pf0, psc,
-                   doffsets, rf0, rsc);
+    launch_compact(s, c, dutt, ws->f_off.get<long long>(), p.total_frames, count, official,
+                   pf0, psc, doffsets, rf0, rsc, ws->cinit.ensure<double>(slots));
   });
   int32_t* hoff = ws->h_offsets.ensure<int32_t>(p.total_frames + 1);
   double* hbest = ws->h_best.ensure<double>(p.total_frames);
@@ -581,13 +588,13 @@ void check_common(int fs, double f0_floor, double f0_ceil) {
 
 // Shared body of the waveform entry points; `device_x` is true when xs[0]
 // already points to the concatenated batch in device memory.
-void run_batch(hv_workspace* ws, int count, const double* const* xs, const int64_t* lengths,
-               int fs, double f0_floor, double f0_ceil, int32_t* offsets, double* best,
-               int64_t* nnz, double* timings, bool device_x) {
-  if (!ws) invalid("null workspace");
+// Plan, upload and preprocessing shared by the analysis and the frozen-decision
+// entry point: on return the decimated, mean-removed y is on the device.
+Plan preprocess_batch(hv_workspace* ws, int count, const double* const* xs,
+                      const int64_t* lengths, int fs, double f0_floor, double f0_ceil,
+                      double* timings, bool device_x) {
   if (count < 1) invalid("count must be positive");
   check_common(fs, f0_floor, f0_ceil);
-  DeviceGuard guard(ws->device);
   const int ratio = std::max(std::min(host_round(fs / 8000.0), 12), 1);
   const double fs_work = (double)fs / ratio;
   if (f0_ceil >= fs_work / 2) invalid("f0_ceil must be below the internal Nyquist frequency");
@@ -638,7 +645,98 @@ void run_batch(hv_workspace* ws, int count, const double* const* xs, const int64
                      ws->y_off.get<long long>(), p.total_y, y,
                      ws->partial.ensure<double>(p.total_pblocks), ws->mean.ensure<double>(count));
   check(hipEventRecord(ws->ev[1], s), "hipEventRecord");
+  return p;
+}
+
+void run_batch(hv_workspace* ws, int count, const double* const* xs, const int64_t* lengths,
+               int fs, double f0_floor, double f0_ceil, int32_t* offsets, double* best,
+               int64_t* nnz, double* timings, bool device_x) {
+  if (!ws) invalid("null workspace");
+  DeviceGuard guard(ws->device);
+  const Plan p = preprocess_batch(ws, count, xs, lengths, fs, f0_floor, f0_ceil, timings,
+                                  device_x);
   analyze_device(ws, p, offsets, best, nnz, timings);
+}
+
+// Frozen-decision refinement of entries (value and/or adjoint), then the
+// adjoint of mean removal and decimation.
+void refine_entries(hv_workspace* ws, int count, const double* const* xs,
+                    const int64_t* lengths, int fs, double f0_floor, double f0_ceil,
+                    int64_t n_entries, const int64_t* frame, const double* f0_initial,
+                    const double* weight, double* refined, double* grad_x) {
+  if (!ws) invalid("null workspace");
+  DeviceGuard guard(ws->device);
+  if (n_entries < 0) invalid("entry count must be nonnegative");
+  if (n_entries > 0 && (!frame || !f0_initial)) invalid("null entry arrays");
+  if (grad_x && n_entries > 0 && !weight) invalid("grad_x needs weights");
+  const Plan p = preprocess_batch(ws, count, xs, lengths, fs, f0_floor, f0_ceil, nullptr, false);
+  const Config& c = p.cfg;
+  hipStream_t s = ws->stream;
+  const Utt* dutt = ws->utt.get<Utt>();
+  for (int64_t e = 0; e < n_entries; ++e) {
+    if (frame[e] < 0 || frame[e] >= p.total_frames)
+      invalid("entry frame " + std::to_string(frame[e]) + " is outside the batch's frames");
+    if (!(f0_initial[e] >= c.f0_floor * (1.0 - 1e-9) && f0_initial[e] <= c.f0_ceil * (1.0 + 1e-9)))
+      invalid("entry initial f0 outside [f0_floor, f0_ceil]");
+  }
+  // Entry arrays through one pinned staging buffer: frames, f0, weights.
+  const size_t ne = (size_t)n_entries;
+  char* h = ws->h_ent.ensure<char>(std::max<size_t>(1, ne * (sizeof(long long) + 2 * sizeof(double))));
+  long long* hf = reinterpret_cast<long long*>(h);
+  double* hi = reinterpret_cast<double*>(h + ne * sizeof(long long));
+  double* hw = hi + ne;
+  for (size_t e = 0; e < ne; ++e) {
+    hf[e] = frame[e];
+    hi[e] = f0_initial[e];
+    hw[e] = weight ? weight[e] : 0.0;
+  }
+  long long* dframe = ws->ent_frame.ensure<long long>(std::max<size_t>(1, ne));
+  double* dinit = ws->ent_f0.ensure<double>(std::max<size_t>(1, ne));
+  double* dw = ws->ent_w.ensure<double>(std::max<size_t>(1, ne));
+  double* dout = ws->ent_out.ensure<double>(std::max<size_t>(1, ne));
+  if (ne > 0) {
+    check(hipMemcpyAsync(dframe, hf, ne * sizeof(long long), hipMemcpyHostToDevice, s), "upload entries");
+    check(hipMemcpyAsync(dinit, hi, ne * sizeof(double), hipMemcpyHostToDevice, s), "upload entries");
+    check(hipMemcpyAsync(dw, hw, ne * sizeof(double), hipMemcpyHostToDevice, s), "upload entries");
+  }
+  double* gy = nullptr;
+  if (grad_x) {
+    gy = ws->gy.ensure<double>(p.total_y);
+    launch_fill_int(s, reinterpret_cast<int*>(gy), 2 * p.total_y, 0);
+  }
+  launch_refine_entries(s, c, dutt, ws->f_off.get<long long>(), n_entries, dframe, dinit,
+                        grad_x ? dw : nullptr, ws->y.get<double>(), refined ? dout : nullptr, gy);
+  if (refined && ne > 0) {
+    check(hipMemcpyAsync(hw, dout, ne * sizeof(double), hipMemcpyDeviceToHost, s), "download refined");
+  }
+  if (grad_x) {
+    // Mean removal is symmetric: y_bar minus its mean.
+    launch_remove_mean(s, c, dutt, ws->p_off.get<long long>(), p.total_pblocks,
+                       ws->y_off.get<long long>(), p.total_y, gy,
+                       ws->partial.ensure<double>(p.total_pblocks), ws->mean.ensure<double>(count));
+    double* gx = ws->gx.ensure<double>(p.total_x);
+    if (c.ratio > 1) {
+      launch_decimate_transpose(s, c, dutt, ws->c_off.get<long long>(), p.total_chunks,
+                                ws->x_off.get<long long>(), p.total_x, gy,
+                                ws->pass.ensure<double>(p.total_m),
+                                ws->states.ensure<double>(3 * p.total_chunks),
+                                ws->carries.ensure<double>(3 * p.total_chunks),
+                                ws->full.ensure<double>(p.total_m), gx);
+    } else {
+      check(hipMemcpyAsync(gx, gy, p.total_x * sizeof(double), hipMemcpyDeviceToDevice, s),
+            "copy gradient");
+    }
+    double* hgx = ws->h_gx.ensure<double>(p.total_x);
+    check(hipMemcpyAsync(hgx, gx, p.total_x * sizeof(double), hipMemcpyDeviceToHost, s),
+          "download gradient");
+    check(hipStreamSynchronize(s), "adjoint");
+    std::memcpy(grad_x, hgx, p.total_x * sizeof(double));
+  } else {
+    check(hipStreamSynchronize(s), "refine entries");
+  }
+  if (refined) std::memcpy(refined, hw, ne * sizeof(double));
+  // The analysis result of this workspace is no longer current.
+  ws->last_nnz = -1;
 }
 
 }  // namespace
@@ -785,6 +883,32 @@ hv_status hv_analyze_batch_device(hv_workspace* ws, int count, const double* x_d
     if (!x_device || !lengths || !nnz) invalid("null pointer");
     const double* xs[1] = {x_device};
     run_batch(ws, count, xs, lengths, fs, f0_floor, f0_ceil, offsets, best, nnz, timings_ms, true);
+  });
+}
+
+hv_status hv_fetch_initial(hv_workspace* ws, double* f0_initial) {
+  return guarded([&] {
+    if (!ws) invalid("null workspace");
+    if (ws->last_nnz < 0) invalid("no analysis result to fetch");
+    if (ws->last_nnz == 0) return;
+    if (!f0_initial) invalid("null pointer");
+    DeviceGuard guard(ws->device);
+    check(hipMemcpyAsync(f0_initial, ws->cinit.get<double>(), ws->last_nnz * sizeof(double),
+                         hipMemcpyDeviceToHost, ws->stream), "download initial f0");
+    check(hipStreamSynchronize(ws->stream), "download initial f0");
+  });
+}
+
+hv_status hv_refine_entries(hv_workspace* ws, int count, const double* const* xs,
+                            const int64_t* lengths, int fs, double f0_floor, double f0_ceil,
+                            int64_t n_entries, const int64_t* frame, const double* f0_initial,
+                            const double* weight, double* refined, double* grad_x) {
+  return guarded([&] {
+    if (!xs || !lengths) invalid("null input arrays");
+    for (int u = 0; u < count; ++u)
+      if (!xs[u]) invalid("null utterance pointer");
+    refine_entries(ws, count, xs, lengths, fs, f0_floor, f0_ceil, n_entries, frame, f0_initial,
+                   weight, refined, grad_x);
   });
 }
 

@@ -488,12 +488,111 @@ void smooth_contour(const double* f0, int frames, double* out, int threads) {
   });
 }
 
+void fix_contour_split(const CandidateRows& rows, int frames, const double* base, double* step3,
+                       double* out, int threads) {
+  std::vector<double> a(frames), b(frames);
+  std::vector<int> list;
+  fix_step1(base, frames, a.data());
+  fix_step2(a.data(), frames, b.data(), list);
+  fix_step3(b.data(), frames, rows, step3, list, threads);
+  fix_step4(step3, frames, out, list);
+}
+
+void fix_step4_apply(const double* step3, int frames, double* out) {
+  std::vector<int> list;
+  fix_step4(step3, frames, out, list);
+}
+
+// FixStep4 is the identity except on bridged gaps, where inner frame j = a +
+// c (c = 1..d, d = b - a - 1) gets (in[a] + 1) + ((in[b] - 1) - (in[a] + 1))
+// c / (d + 1): affine in in[a] and in[b].
+void fix_step4_adjoint(const double* step3, int frames, const double* g_out, double* g_in) {
+  std::vector<int> list;
+  boundary_list(step3, frames, list);
+  std::copy(g_out, g_out + frames, g_in);
+  const int sections = (int)list.size() / 2;
+  for (int i = 0; i < sections - 1; ++i) {
+    const int a = list[i * 2 + 1], b = list[(i + 1) * 2];
+    const int distance = b - a - 1;
+    if (distance >= 9) continue;
+    for (int j = a + 1, c = 1; j <= b - 1; ++j, ++c) {
+      const double t = c / (distance + 1.0);
+      g_in[a] += g_out[j] * (1.0 - t);
+      g_in[b] += g_out[j] * t;
+      g_in[j] = 0.0;
+    }
+  }
+}
+
+// Transpose of SmoothF0Contour.  Per voiced section [st, ed] of the padded
+// contour the forward map is Sel R F R F E: E extends the section by its end
+// values over the padded length, F is the recursion, R reverses, Sel keeps
+// [st, ed].  Since F^T = R F R, the transpose is E^T R F R F Sel^T: scatter the
+// section's output gradient, run the same two passes, and fold the extension
+// back onto the end values.  Zero inputs leave a zero state at rest, and a
+// decaying state reaches exact zero, so each pass runs only while nonzero.
+void smooth_adjoint(const double* f0, int frames, const double* g_out, double* g_in) {
+  constexpr int lag = 300;
+  const long n = (long)frames + 2 * lag;
+  auto padded = [&](long i) { return i >= lag && i < lag + frames ? f0[i - lag] : 0.0; };
+  for (int i = 0; i < frames; ++i) g_in[i] = 0.0;
+  std::vector<double> u(n);
+  // One causal pass of the recursion over u[lo, n) (input zero below lo),
+  // in place; returns the index after the last nonzero output.
+  auto pass = [&](long lo, long hi_input) {
+    Smoother s;
+    long last = lo;
+    for (long k = lo; k < n; ++k) {
+      if (k >= hi_input && s.w0 == 0.0 && s.w1 == 0.0) break;
+      const double in = k < hi_input ? u[k] : 0.0;
+      u[k] = s.step(in);
+      if (u[k] != 0.0) last = k + 1;
+    }
+    for (long k = last; k < n; ++k) u[k] = 0.0;
+    return last;
+  };
+  long i = 1;
+  while (i < n - 1) {
+    if (!(padded(i) > 0)) {
+      ++i;
+      continue;
+    }
+    const long st = i;
+    long ed = i;
+    while (ed + 1 < n - 1 && padded(ed + 1) > 0) ++ed;
+    i = ed + 1;
+    std::fill(u.begin(), u.end(), 0.0);
+    for (long k = st; k <= ed; ++k) u[k] = g_out[k - lag];
+    const long end1 = pass(st, ed + 1);           // F over [st, ...)
+    std::reverse(u.begin(), u.end());            // R
+    const long end2 = pass(n - end1, n - st);    // F; nonzero inputs end at n - 1 - st
+    std::reverse(u.begin(), u.end());            // R
+    (void)end2;
+    // E^T: the extension copies x[st] to [0, st) and x[ed] to (ed, n).
+    double head = 0.0, tail = 0.0;
+    for (long k = 0; k <= st; ++k) head += u[k];
+    for (long k = ed; k < n; ++k) tail += u[k];
+    if (st == ed) {
+      g_in[st - lag] += head + tail - u[st];
+    } else {
+      g_in[st - lag] += head;
+      g_in[ed - lag] += tail;
+      for (long k = st + 1; k < ed; ++k) g_in[k - lag] += u[k];
+    }
+  }
+}
+
 void resample(const double* basic, int frames, double frame_period, int out_frames, double* f0,
               double* times) {
   for (int i = 0; i < out_frames; ++i) {
     times[i] = i * frame_period / 1000.0;
     f0[i] = basic[std::min(frames - 1, mround(times[i] * 1000.0))];
   }
+}
+
+void resample_indices(int frames, double frame_period, int out_frames, int* index) {
+  for (int i = 0; i < out_frames; ++i)
+    index[i] = std::min(frames - 1, mround(i * frame_period / 1000.0 * 1000.0));
 }
 
 namespace {

@@ -66,9 +66,21 @@ void fix_contour(const CandidateRows& rows, int frames, const double* base, doub
 // SmoothF0Contour (harvest.cpp:1079-1113), bit-exact, O(F + sections * settle).
 // Sections are smoothed on up to `threads` threads.
 void smooth_contour(const double* f0, int frames, double* out, int threads = 1);
+// FixF0Contour split after FixStep3 (`step3`) for the differentiable form;
+// `out` equals fix_contour's output.
+void fix_contour_split(const CandidateRows& rows, int frames, const double* base, double* step3,
+                       double* out, int threads = 1);
+// FixStep4 (harvest.cpp:1000-1022) alone, and its transpose at `step3`.
+void fix_step4_apply(const double* step3, int frames, double* out);
+void fix_step4_adjoint(const double* step3, int frames, const double* g_out, double* g_in);
+// Transpose of smooth_contour at contour f0 (the map is linear for a fixed
+// voiced/unvoiced pattern).
+void smooth_adjoint(const double* f0, int frames, const double* g_out, double* g_in);
 // Resampling to the output frame period (harvest.cpp:1246-1251).
 void resample(const double* basic, int frames, double frame_period, int out_frames,
               double* f0, double* times);
+// The 1 ms frame each output frame of resample reads.
+void resample_indices(int frames, double frame_period, int out_frames, int* index);
 
 // Throws std::invalid_argument / std::runtime_error for a failed backend call.
 void throw_status(hv_status status);

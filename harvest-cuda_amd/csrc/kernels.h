@@ -67,7 +67,21 @@ void launch_exclusive_scan(hipStream_t s, const int* in, int* out, long long n,
 long long scan_block_count(long long n);
 void launch_compact(hipStream_t s, const Config& c, const Utt* utt,
                     const long long* f_off, long long total_frames,
-                    const int* count, const double* pf0, const double* pscore,
-                    const int* offsets, double* out_f0, double* out_score);
+                    const int* count, const double* official, const double* pf0,
+                    const double* pscore, const int* offsets, double* out_f0,
+                    double* out_score, double* out_init);
+// Transpose of launch_decimate (gradient of y -> gradient of x); `full` holds
+// one pass-length buffer per utterance.
+void launch_decimate_transpose(hipStream_t s, const Config& c, const Utt* utt,
+                               const long long* c_off, long long total_chunks,
+                               const long long* x_off, long long total_x, const double* gy,
+                               double* pass_buffer, double* states, double* carries,
+                               double* full, double* gx);
+// Frozen-decision refined F0 of (frame, initial f0) entries and/or the
+// weighted adjoint added into gy.
+void launch_refine_entries(hipStream_t s, const Config& c, const Utt* utt,
+                           const long long* f_off, long long n_entries, const long long* frame,
+                           const double* f0_init, const double* weight, const double* y,
+                           double* refined, double* gy);
 
 }  // namespace hv

@@ -81,6 +81,19 @@ __device__ unsigned long long block_scan_u64(unsigned long long v,
 // end samples and decimate() adds 9 odd reflections that, on a constant run,
 // equal the end samples again (2a - a = a exactly).  Pass 1 reads pass 0's
 // output backwards.
+//
+// The transpose (for the adjoint): decimation is y = S F R F P x with P the
+// padding, F the causal IIR from rest, R the reversal and S the selection of
+// every r-th sample.  F^T = R F R exactly, so x_bar = P^T R F R F R S^T y_bar:
+// pass 2 reads S^T y_bar reversed (the gradient scattered to pass-1 indices),
+// pass 3 reads pass 2's output backwards, and k_iir_fold applies R and P^T.
+__device__ __forceinline__ long long select_base(const Config& c, const Utt& u) {
+  // decimate(): y[count] = tmp1[nbeg + count r + 8] of the twice-reversed
+  // output, i.e. pass-1 index m - 9 - nbeg - count r; harvest.cpp:62 keeps
+  // count = lag / r + j.
+  return u.m - 9 - u.nbeg - (long long)(c.lag / c.ratio) * c.ratio;
+}
+
 template <int PASS>
 __device__ __forceinline__ double iir_input(const Config& c, const Utt& u,
                                             const double* x, const double* w,
@@ -89,6 +102,11 @@ __device__ __forceinline__ double iir_input(const Config& c, const Utt& u,
     long long j = i - 9 - c.lag;
     j = j < 0 ? 0 : (j >= u.length ? u.length - 1 : j);
     return x[u.x_off + j];
+  }
+  if (PASS == 2) {  // x is the y-gradient here
+    const long long d = select_base(c, u) - (u.m - 1 - i);
+    if (d >= 0 && d % c.ratio == 0 && d / c.ratio < u.n) return x[u.y_off + d / c.ratio];
+    return 0.0;
   }
   return w[u.m_off + (u.m - 1 - i)];
 }
@@ -205,19 +223,17 @@ __global__ __launch_bounds__(kThreads) void k_iir_replay(
   long long start = (cg - u.c_off) * kChunk;
   long long end = min(start + (long long)kChunk, u.m);
   double w0 = carries[cg * 3 + 0], w1 = carries[cg * 3 + 1], w2 = carries[cg * 3 + 2];
-  // decimate(): y[count] = tmp1[nbeg + count r + 8] of the twice-reversed
-  // output, i.e. pass-1 index m - 9 - nbeg - count r; harvest.cpp:62 keeps
-  // count = lag / r + j.
-  const long long out_base =
-      u.m - 9 - u.nbeg - (long long)(c.lag / c.ratio) * c.ratio;
+  const long long out_base = select_base(c, u);
   for (long long i = start; i < end; ++i) {
     double wt = iir_input<PASS>(c, u, x, w, i) + c.a0 * w0 + c.a1 * w1 + c.a2 * w2;
     double out = c.b0 * wt + c.b1 * w0 + c.b1 * w1 + c.b0 * w2;
     w2 = w1;
     w1 = w0;
     w0 = wt;
-    if (PASS == 0) {
+    if (PASS == 0 || PASS == 2) {
       w[u.m_off + i] = out;
+    } else if (PASS == 3) {  // y is the full-length output buffer here
+      y[u.m_off + i] = out;
     } else {
       long long d = out_base - i;
       if (d >= 0 && d % c.ratio == 0) {
@@ -226,6 +242,30 @@ __global__ __launch_bounds__(kThreads) void k_iir_replay(
       }
     }
   }
+}
+
+// x_bar = P^T R z for z = pass 3's output: interior samples take one value,
+// the two end samples collect their padding copies (lag + 10 each).
+__global__ __launch_bounds__(kThreads) void k_iir_fold(Config c, const Utt* utt,
+                                                      const long long* x_off, long long total,
+                                                      const double* z, double* gx) {
+  const long long g = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (g >= total) return;
+  const Utt u = utt[find_segment(x_off, c.count, g)];
+  const long long k = g - u.x_off;
+  const long long shift = 9 + c.lag;  // padded index of x[0]
+  auto v = [&](long long t) { return z[u.m_off + (u.m - 1 - t)]; };
+  double s = 0.0;
+  if (u.length == 1) {
+    for (long long t = 0; t < u.m; ++t) s += v(t);
+  } else if (k == 0) {
+    for (long long t = 0; t <= shift; ++t) s += v(t);
+  } else if (k == u.length - 1) {
+    for (long long t = k + shift; t < u.m; ++t) s += v(t);
+  } else {
+    s = v(k + shift);
+  }
+  gx[g] = s;
 }
 
 __global__ __launch_bounds__(kThreads) void k_copy_input(
@@ -1230,6 +1270,120 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Frozen-decision refinement of given entries and its adjoint.  Entry e is a
+// 1 ms frame and the official candidate its slot was refined from; that fixes
+// the half window h, the FFT size n, the harmonic count H and the bins k_h
+// (GetMeanF0 harvest.cpp:541-584, FixF0 :507-536).  With those held, the
+// refined F0
+//   r = sum_h a_h IF_h / (sum_h a_h (h+1) + 1e-12),   a_h = sqrt(P_h),
+//   IF_h = k_h fs / n + Nu_h / P_h * fs / (2 pi),
+//   P_h = |M_h|^2,  Nu_h = Re M_h Im D_h - Im M_h Re D_h,
+//   M_h = sum_m y_m w_m e^{-2 pi i k_h m / n},  D_h likewise with d_m,
+// is a smooth function of the windowed samples.  One wavefront per entry:
+// lanes own samples, the 4H sums are reduced by a fixed butterfly, r is
+// written to `refined` (if given), and weight_e * dr/dy is added into gy (if
+// given) at the clamped sample indices.  The gradient is accumulated with
+// atomics, so its last bits depend on the order of accumulation.
+__global__ __launch_bounds__(64) void k_refine_entries(
+    Config c, const Utt* utt, const long long* f_off, long long n_entries,
+    const long long* frame, const double* f0_init, const double* weight, const double* y,
+    double* refined, double* gy) {
+  const long long e = blockIdx.x;
+  if (e >= n_entries) return;
+  const int lane = threadIdx.x;
+  const long long gf = frame[e];
+  const Utt u = utt[find_segment(f_off, c.count, gf)];
+  const double fs = c.fs_work;
+  const double t = (double)(gf - u.f_off) / 1000.0;
+  const double f0 = f0_init[e];
+  const int hw = (int)(1.5 * fs / f0 + 1.0), len = 2 * hw + 1;
+  int lg = 0;
+  while ((2 << lg) <= len) ++lg;
+  const int n = 1 << (lg + 2);
+  const int basic = mround((t + (-hw) / fs) * fs + 0.001);
+  const double a = (double)(basic - 1) - t * fs;
+  const int H = min((int)(fs / 2.0 / f0), 6);
+  int kb[6];
+  for (int hh = 0; hh < 6; ++hh) kb[hh] = hh < H ? mround(f0 * n / fs * (hh + 1)) : 0;
+  const double* yu = y + u.y_off;
+  auto idx_of = [&](int m) {
+    const int idx = basic + m - 1;
+    return idx < 0 ? 0 : (idx > u.n - 1 ? u.n - 1 : idx);
+  };
+  // Blackman window (GetMainWindow) and its central difference (GetDiffWindow).
+  auto win = [&](int m) {
+    const double cs = cospi(2.0 * (a + m) / len);
+    return 0.42 + 0.5 * cs + 0.08 * (2.0 * cs * cs - 1.0);
+  };
+  auto dwin = [&](int m) {
+    if (m == 0) return -win(1) / 2.0;
+    if (m == len - 1) return win(len - 2) / 2.0;
+    return -(win(m + 1) - win(m - 1)) / 2.0;
+  };
+  double mr[6], mi[6], dr[6], di[6];
+  for (int hh = 0; hh < 6; ++hh) mr[hh] = mi[hh] = dr[hh] = di[hh] = 0.0;
+  for (int m = lane; m < len; m += 64) {
+    const double xv = yu[idx_of(m)];
+    const double xw = xv * win(m), xd = xv * dwin(m);
+    for (int hh = 0; hh < H; ++hh) {
+      double sn, cs;
+      sincospi(2.0 * (double)(((long long)kb[hh] * m) & (n - 1)) / n, &sn, &cs);
+      mr[hh] = fma(xw, cs, mr[hh]);
+      mi[hh] = fma(-xw, sn, mi[hh]);
+      dr[hh] = fma(xd, cs, dr[hh]);
+      di[hh] = fma(-xd, sn, di[hh]);
+    }
+  }
+  for (int hh = 0; hh < 6; ++hh)
+    for (int d = 32; d > 0; d >>= 1) {
+      mr[hh] += __shfl_xor(mr[hh], d, 64);
+      mi[hh] += __shfl_xor(mi[hh], d, 64);
+      dr[hh] += __shfl_xor(dr[hh], d, 64);
+      di[hh] += __shfl_xor(di[hh], d, 64);
+    }
+  const double cif = fs / 2.0 / kPi;
+  double pw[6], nu[6], amp[6], ifr[6], numer = 0.0, den = 0.0;
+  for (int hh = 0; hh < H; ++hh) {
+    pw[hh] = mr[hh] * mr[hh] + mi[hh] * mi[hh];
+    nu[hh] = mr[hh] * di[hh] - mi[hh] * dr[hh];
+    ifr[hh] = pw[hh] == 0.0 ? 0.0 : (double)kb[hh] * fs / n + nu[hh] / pw[hh] * fs / 2.0 / kPi;
+    amp[hh] = sqrt(pw[hh]);
+    numer += amp[hh] * ifr[hh];
+    den += amp[hh] * (hh + 1.0);
+  }
+  const double dg = den + kGuard;
+  const double r = numer / dg;
+  if (refined && lane == 0) refined[e] = r;
+  if (!gy || !weight) return;
+  const double g = weight[e];
+  // Reverse mode: r -> (a_h, IF_h) -> (P_h, Nu_h) -> (M_h, D_h).
+  double gmr[6], gmi[6], gdr[6], gdi[6];
+  for (int hh = 0; hh < H; ++hh) {
+    gmr[hh] = gmi[hh] = gdr[hh] = gdi[hh] = 0.0;
+    if (pw[hh] == 0.0) continue;  // IF_h = 0 and a_h = 0 there: no smooth dependence
+    const double g_amp = g * (ifr[hh] - r * (hh + 1.0)) / dg;
+    const double g_if = g * amp[hh] / dg;
+    const double g_pw = g_amp / (2.0 * amp[hh]) - g_if * cif * nu[hh] / (pw[hh] * pw[hh]);
+    const double g_nu = g_if * cif / pw[hh];
+    gmr[hh] = 2.0 * mr[hh] * g_pw + di[hh] * g_nu;
+    gmi[hh] = 2.0 * mi[hh] * g_pw - dr[hh] * g_nu;
+    gdr[hh] = -mi[hh] * g_nu;
+    gdi[hh] = mr[hh] * g_nu;
+  }
+  for (int m = lane; m < len; m += 64) {
+    double gxw = 0.0, gxd = 0.0;
+    for (int hh = 0; hh < H; ++hh) {
+      double sn, cs;
+      sincospi(2.0 * (double)(((long long)kb[hh] * m) & (n - 1)) / n, &sn, &cs);
+      gxw += gmr[hh] * cs - gmi[hh] * sn;
+      gxd += gdr[hh] * cs - gdi[hh] * sn;
+    }
+    const double gx = win(m) * gxw + dwin(m) * gxd;
+    if (gx != 0.0) atomicAdd(gy + u.y_off + idx_of(m), gx);
+  }
+}
+
 // RemoveUnreliableCandidates (harvest.cpp:652-688) against the pre-prune
 // values of frames i +- 1.  min_k fl(|r - c_k| / r) = fl(min_k |r - c_k| / r)
 // because rounding is monotone, so one division per neighbour frame suffices.
@@ -1291,21 +1445,34 @@ __global__ __launch_bounds__(kThreads) void k_best(Config c, const long long* f_
   row_nnz[gf] = nz;
 }
 
-__global__ __launch_bounds__(kThreads) void k_compact(Config c, const long long* f_off,
+// Order-preserving compaction of the nonzero pruned candidates, each with the
+// official candidate its slot was refined from (OverlapF0Candidates layout),
+// which fixes the slot's window and bins for the frozen-decision adjoint.
+__global__ __launch_bounds__(kThreads) void k_compact(Config c, const Utt* utt,
+                                                     const long long* f_off,
                                                      long long total_frames,
-                                                     const int* count, const double* pf0,
-                                                     const double* pscore,
+                                                     const int* count, const double* official,
+                                                     const double* pf0, const double* pscore,
                                                      const int* offsets, double* of0,
-                                                     double* oscore) {
+                                                     double* oscore, double* oinit) {
   const long long gf = (long long)blockIdx.x * kThreads + threadIdx.x;
   if (gf >= total_frames) return;
-  const int ncand = 7 * count[find_segment(f_off, c.count, gf)];
+  const int ui = find_segment(f_off, c.count, gf);
+  const int cnt = count[ui];
+  const int ncand = 7 * cnt;
+  const Utt u = utt[ui];
+  const long long i = gf - u.f_off;
   int pos = offsets[gf];
   for (int k = 0; k < ncand; ++k) {
     const double f = pf0[gf * c.slots + k];
     if (f != 0.0) {
       of0[pos] = f;
       oscore[pos] = pscore[gf * c.slots + k];
+      const int g = k / cnt, jj = k % cnt;
+      long long src = i;
+      if (g >= 1 && g <= 3) src = i - g;
+      else if (g >= 4) src = i + (g - 3);
+      oinit[pos] = official[(u.f_off + src) * c.capacity + jj];
       ++pos;
     }
   }
@@ -1519,12 +1686,43 @@ void launch_exclusive_scan(hipStream_t s, const int* in, int* out, long long n,
 
 void launch_compact(hipStream_t s, const Config& c, const Utt* utt,
                     const long long* f_off, long long total_frames,
-                    const int* count, const double* pf0, const double* pscore,
-                    const int* offsets, double* out_f0, double* out_score) {
-  (void)utt;
-  k_compact<<<grid_for(total_frames), kThreads, 0, s>>>(c, f_off, total_frames, count, pf0,
-                                                       pscore, offsets, out_f0, out_score);
+                    const int* count, const double* official, const double* pf0,
+                    const double* pscore, const int* offsets, double* out_f0,
+                    double* out_score, double* out_init) {
+  k_compact<<<grid_for(total_frames), kThreads, 0, s>>>(c, utt, f_off, total_frames, count,
+                                                       official, pf0, pscore, offsets, out_f0,
+                                                       out_score, out_init);
   check_launch("compact");
+}
+
+void launch_decimate_transpose(hipStream_t s, const Config& c, const Utt* utt,
+                               const long long* c_off, long long total_chunks,
+                               const long long* x_off, long long total_x, const double* gy,
+                               double* pass_buffer, double* states, double* carries,
+                               double* full, double* gx) {
+  const unsigned grid = grid_for(total_chunks);
+  k_iir_chunk_state<2><<<grid, kThreads, 0, s>>>(c, utt, c_off, total_chunks, gy, pass_buffer,
+                                                  states);
+  k_iir_carry<<<c.count, kThreads, 0, s>>>(c, utt, states, carries);
+  k_iir_replay<2><<<grid, kThreads, 0, s>>>(c, utt, c_off, total_chunks, gy, pass_buffer,
+                                             carries, full);
+  k_iir_chunk_state<3><<<grid, kThreads, 0, s>>>(c, utt, c_off, total_chunks, gy, pass_buffer,
+                                                  states);
+  k_iir_carry<<<c.count, kThreads, 0, s>>>(c, utt, states, carries);
+  k_iir_replay<3><<<grid, kThreads, 0, s>>>(c, utt, c_off, total_chunks, gy, pass_buffer,
+                                             carries, full);
+  k_iir_fold<<<grid_for(total_x), kThreads, 0, s>>>(c, utt, x_off, total_x, full, gx);
+  check_launch("decimate_transpose");
+}
+
+void launch_refine_entries(hipStream_t s, const Config& c, const Utt* utt,
+                           const long long* f_off, long long n_entries, const long long* frame,
+                           const double* f0_init, const double* weight, const double* y,
+                           double* refined, double* gy) {
+  if (n_entries <= 0) return;
+  k_refine_entries<<<(unsigned)n_entries, 64, 0, s>>>(c, utt, f_off, n_entries, frame, f0_init,
+                                                      weight, y, refined, gy);
+  check_launch("refine_entries");
 }
 
 }  // namespace hv

@@ -127,6 +127,13 @@ def config4(seed: int = 1004, count: int = 32):
     return xs, fs
 
 
+def config4_target(f0s, seed: int = 1004, sigma: float = 0.02):
+    """Per-frame targets f0 * exp(N(0, sigma^2)) for the forward F0 tracks f0s
+    (the config-4 loss), from a stream independent of the waveforms'."""
+    rng = np.random.default_rng(np.random.SeedSequence(seed).spawn(len(f0s) + 1)[len(f0s)])
+    return [np.asarray(f) * np.exp(rng.normal(0.0, sigma, len(f))) for f in f0s]
+
+
 def config5_recording(index: int, seed: int = 1005, seconds: float = 1800.0) -> tuple[np.ndarray, int]:
     fs = 44100
     child = np.random.SeedSequence(seed).spawn(index + 1)[index]

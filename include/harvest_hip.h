@@ -140,6 +140,31 @@ hv_status hv_fetch_candidates(hv_workspace* ws, double* f0, double* scores);
 hv_status hv_result_view(hv_workspace* ws, const int32_t** offsets, const double** best,
                          const double** f0, const double** scores, int64_t* nnz);
 
+/* Initial f0 of each candidate of the last analysis, in CSR order: the
+ * official candidate (harvest.cpp:348-412) its slot was refined from, which
+ * fixes the slot's window length, FFT size and harmonic bins.  No reference
+ * counterpart: the input of the frozen-decision adjoint below. */
+hv_status hv_fetch_initial(hv_workspace* ws, double* f0_initial);
+
+/* Frozen-decision refinement and its adjoint, for the differentiable form
+ * (the adjoint entry point proposed beside hv_analyze_batch; no reference
+ * counterpart).  The waveforms are preprocessed exactly as in
+ * hv_analyze_batch; then for each entry e (a 1 ms frame of the batch and the
+ * initial f0 of a candidate there) the refined F0 of GetRefinedF0
+ * (harvest.cpp:589-617) is evaluated with the window, bins and harmonic count
+ * that the initial f0 fixes:
+ *   refined[e]  (if refined != NULL) the refined F0;
+ *   grad_x[ΣL]  (if grad_x != NULL, weight required) sum_e weight[e] times
+ *               the gradient of refined[e] with respect to the waveforms,
+ *               through refinement, mean removal and decimation.
+ * frame[e] indexes the batch's concatenated 1 ms frames (utterance u's
+ * frames start after those of utterances 0..u-1).  The gradient is summed
+ * with atomics: its last bits may differ between runs. */
+hv_status hv_refine_entries(hv_workspace* ws, int count, const double* const* xs,
+                            const int64_t* lengths, int fs, double f0_floor, double f0_ceil,
+                            int64_t n_entries, const int64_t* frame, const double* f0_initial,
+                            const double* weight, double* refined, double* grad_x);
+
 #ifdef __cplusplus
 }
 #endif
