@@ -72,6 +72,18 @@ def algorithmic_bytes(lengths, fs, floor=71.0, ceil=800.0):
     return total
 
 
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` on this workload, from the committed
+    FETCH_SIZE/WRITE_SIZE counter passes calibrated by a copy of known size
+    (profiles/config2_traffic.md); None when not profiled."""
+    import re
+    path = ROOT / "profiles" / "config2_traffic.md"
+    if not path.exists():
+        return None
+    m = re.search(r"^traffic_bytes_per_launch %s: (\d+)$" % re.escape(kernel), path.read_text(), re.M)
+    return float(m.group(1)) if m else None
+
+
 def roofline(kernels_ms, work, lengths, fs, steps):
     """Roofline object of the kernel with the largest average duration."""
     avg = {k: v / steps for k, v in kernels_ms.items()}
@@ -81,14 +93,16 @@ def roofline(kernels_ms, work, lengths, fs, steps):
         flops = fir_flops(lengths, fs)
         achieved = flops / sec / 1e12
         return name, {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                      "traffic": measured_traffic("k_filter_events"),
                       "kernel": "k_filter_events", "avg_ms": avg[name],
                       "model": "2 FLOP per tap per output sample per band (FP64 FMA)"}
     if name == "kernel_refine":
         flops = 8.0 * work["bin_samples"] / steps + 17.0 * work["window_samples"] / steps
         achieved = flops / sec / 1e12
         return name, {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                      "traffic": measured_traffic("k_refine_waves"),
                       "kernel": "k_refine_waves", "avg_ms": avg[name],
                       "model": "8 FLOP per bin-sample (4 FMA) + 17 FLOP per window sample, "
                                "cosines not counted"}
@@ -172,7 +186,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batched", type=int, default=1, help="also run the config-3 batched leg")
     ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-MiB float64 tensor 5 times on the GPU (a kernel of known "
+                         "traffic, to calibrate the profiler's FETCH_SIZE/WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 8
+        src = torch.rand(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "launches": 5}))
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "

@@ -15,6 +15,7 @@ except ImportError:  # pragma: no cover - torch is optional for the NumPy API
     _torch = None
 
 from ._core import clear_cache, cuda_available, harvest, harvest_batch  # noqa: E402
+from .stream import harvest_stream  # noqa: E402
 
-__all__ = ["harvest", "harvest_batch", "cuda_available", "clear_cache"]
+__all__ = ["harvest", "harvest_batch", "harvest_stream", "cuda_available", "clear_cache"]
 __version__ = "0.1.0"
