@@ -711,34 +711,59 @@ __global__ __launch_bounds__(kThreads) void k_official(Config c, const long long
                                                       long long total_frames,
                                                       const double* raw,
                                                       double* official, int* count) {
-  long long gf = (long long)blockIdx.x * kThreads + threadIdx.x;
-  if (gf >= total_frames) return;
-  const int ui = find_segment(f_off, c.count, gf);
+  const long long gf = (long long)blockIdx.x * kThreads + threadIdx.x;
+  // Lanes past the end stay active for the wave-level reduction below.
+  const bool valid = gf < total_frames;
+  const int ui = find_segment(f_off, c.count, valid ? gf : total_frames - 1);
   double* o = official + gf * c.capacity;
   int n = 0, st = 0;
   bool prev = false;
   double sum = 0.0;
-  for (int b = 0; b < c.bands; ++b) {
-    const double v = raw[(long long)b * total_frames + gf];
-    const bool on = b > 0 && b < c.bands - 1 && v > 0;
-    if (on) {
-      if (!prev) {
-        st = b;
-        sum = 0.0;
+  // Eight bands are loaded ahead of their use: one thread per frame leaves few
+  // waves per CU, so the loads' latency is hidden by having them in flight.
+  constexpr int kAhead = 8;
+  for (int b0 = 0; valid && b0 < c.bands; b0 += kAhead) {
+    double vals[kAhead];
+#pragma unroll
+    for (int q = 0; q < kAhead; ++q)
+      vals[q] = b0 + q < c.bands ? raw[(long long)(b0 + q) * total_frames + gf] : 0.0;
+#pragma unroll
+    for (int q = 0; q < kAhead; ++q) {
+      const int b = b0 + q;
+      if (b >= c.bands) break;
+      const double v = vals[q];
+      const bool on = b > 0 && b < c.bands - 1 && v > 0;
+      if (on) {
+        if (!prev) {
+          st = b;
+          sum = 0.0;
+        }
+        sum += v;
+      } else if (prev) {
+        const int len = b - st;
+        if (len >= 10) {
+          if (n < c.capacity) o[n] = sum / len;
+          ++n;
+        }
       }
-      sum += v;
-    } else if (prev) {
-      const int len = b - st;
-      if (len >= 10) {
-        if (n < c.capacity) o[n] = sum / len;
-        ++n;
-      }
+      prev = on;
     }
-    prev = on;
   }
   n = min(n, c.capacity);
-  for (int k = n; k < c.capacity; ++k) o[k] = 0.0;
-  atomicMax(count + ui, n);
+  if (valid)
+    for (int k = n; k < c.capacity; ++k) o[k] = 0.0;
+  // count[u] = max over the utterance's frames: one atomic per wave when the
+  // wave lies in one utterance (the usual case), not one per frame on a
+  // single contended address.
+  const int ui0 = __shfl(ui, 0, 64);
+  if (__all(ui == ui0)) {
+    int m = n;
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) m = max(m, __shfl_xor(m, d, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(count + ui0, m);
+  } else if (valid) {
+    atomicMax(count + ui, n);
+  }
 }
 
 // ---------------------------------------------------------------------------
