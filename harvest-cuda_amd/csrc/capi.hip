@@ -489,9 +489,11 @@ void analyze_device(hv_workspace* ws, const Plan& p, int32_t* offsets, double* b
   double* rf0 = ws->rf0.ensure<double>(slots);
   double* rsc = ws->rsc.ensure<double>(slots);
   // HV_STAT_SLOTS work counters, then a flag the refinement sets if a
-  // candidate's window exceeds the bound its LDS was sized for.
-  unsigned long long* work = ws->work.ensure<unsigned long long>(HV_STAT_SLOTS + 1);
-  launch_fill_int(s, reinterpret_cast<int*>(work), 2 * (HV_STAT_SLOTS + 1), 0);
+  // candidate's window exceeds the bound its LDS was sized for; each spread
+  // over kWorkSpread addresses.
+  constexpr int kWorkWords = (HV_STAT_SLOTS + 1) * kWorkSpread;
+  unsigned long long* work = ws->work.ensure<unsigned long long>(kWorkWords);
+  launch_fill_int(s, reinterpret_cast<int*>(work), 2 * kWorkWords, 0);
   timed(ws, 6, [&] {
     launch_refine(s, c, dutt, ws->f_off.get<long long>(), p.total_frames, y, official, count,
                   ws->twiddle.get<double>(), rf0, rsc, work);
@@ -517,11 +519,16 @@ void analyze_device(hv_workspace* ws, const Plan& p, int32_t* offsets, double* b
                        hipMemcpyDeviceToHost, s), "download offsets");
   check(hipMemcpyAsync(hbest, dbest, p.total_frames * sizeof(double), hipMemcpyDeviceToHost, s),
         "download best");
-  unsigned long long* hwork = ws->h_work.ensure<unsigned long long>(HV_STAT_SLOTS + 1);
-  check(hipMemcpyAsync(hwork, work, (HV_STAT_SLOTS + 1) * sizeof(unsigned long long),
+  unsigned long long* hwork = ws->h_work.ensure<unsigned long long>(kWorkWords);
+  check(hipMemcpyAsync(hwork, work, kWorkWords * sizeof(unsigned long long),
                        hipMemcpyDeviceToHost, s), "download counters");
   check(hipEventRecord(ws->ev[3], s), "hipEventRecord");
   check(hipStreamSynchronize(s), "analysis");
+  for (int k = 0; k <= HV_STAT_SLOTS; ++k) {  // fold to one entry per counter
+    unsigned long long total = 0;
+    for (int q = 0; q < kWorkSpread; ++q) total += hwork[k * kWorkSpread + q];
+    hwork[k] = total;
+  }
   if (hwork[HV_STAT_SLOTS])
     throw std::runtime_error("refinement window beyond its precomputed bound");
   // The candidates go straight to the pinned staging buffer, where the host

@@ -15,7 +15,10 @@ constexpr int kTile = 2048;          // filterbank output samples per workgroup
 constexpr int kSlabCap = kTile / 2;  // max crossings of one type per tile
 constexpr int kChunk = 128;          // IIR samples per scan chunk
 constexpr int kMeanBlock = 4096;     // samples per deterministic partial sum
-constexpr int kWindowMargin = 4;     // crossings kept either side of a tile
+constexpr int kWindowMargin = 4;
+// Work counters are spread over this many addresses (by workgroup) so that
+// the per-wave atomics of a launch do not queue on one address; the host sums.
+constexpr int kWorkSpread = 64;     // crossings kept either side of a tile
 
 // Per-utterance layout of a batch.  All offsets index the concatenated
 // device arrays of the batch.

@@ -1286,11 +1286,12 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
       out_sc[s] = 0.0;
     }
   if (work) {
-    if (fault) atomicOr(&work[3], 1ull);
+    const int spread = blockIdx.x % kWorkSpread;
+    if (fault) atomicOr(&work[3 * kWorkSpread + spread], 1ull);
     if (lane == 0) {
-      atomicAdd(&work[0], wsamp);
-      atomicAdd(&work[1], bsamp);
-      atomicAdd(&work[2], nbins);
+      atomicAdd(&work[0 * kWorkSpread + spread], wsamp);
+      atomicAdd(&work[1 * kWorkSpread + spread], bsamp);
+      atomicAdd(&work[2 * kWorkSpread + spread], nbins);
     }
   }
 }
