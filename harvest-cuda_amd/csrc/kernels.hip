@@ -781,16 +781,18 @@ __global__ __launch_bounds__(kThreads) void k_official(Config c, const long long
 //     LDS, kStage samples per pass;
 //   * marks the window's distinct bins k in an LDS bitmap, whose rank
 //     (popcount) gives each (slot, harmonic) its result index;
-//   * sums each distinct bin with 8 lanes (samples m = sub mod 8, increasing
-//     m) and a fixed butterfly; each lane's twiddle starts from the exact
-//     table value for its first sample and advances by complex rotation;
+//   * sums each distinct bin with 8 lanes and a fixed butterfly: lane sub runs
+//     a Goertzel recurrence at frequency 8w over its samples m = sub + 8i
+//     (two operations per sample and signal, no twiddle per sample) and
+//     turns the final state into its partial sum with the exact table
+//     values of 8w and w m_last;
 //   * keeps the frame's bin results in LDS and finishes all of its slots
 //     (FixF0 and the gate) in one lane-parallel pass at the end.
 // The order of operations per bin is fixed, so results do not depend on the
 // batch, on the staging passes or on which path served the bin.
 constexpr int kWaves = 4;        // frames per workgroup
 constexpr int kSplit = 8;        // lanes sharing one DFT bin
-constexpr int kPer = 2;          // bins per lane group per pass
+constexpr int kPer = 1;          // bins per lane group per pass
 constexpr int kFrameBins = 128;  // bin results of one frame kept in LDS (< 256)
 constexpr int kStage = 128;      // windowed samples staged per pass (multiple of 64)
 constexpr int kMaxWords = 128;   // bitmap words (bins 0..8191)
@@ -1099,19 +1101,21 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
     // (power, IF numerator) of bins bink[j0, j0 + nbt) into res[j0, ...).
     auto dft_bins = [&](int j0, int nbt) {
       for (int jb = 0; jb < nbt; jb += kSplit * kPer) {
-        double mre[kPer], mim[kPer], dre[kPer], dim[kPer], tc[kPer], ts[kPer], rc[kPer],
-            rs[kPer];
+        // Goertzel on this lane's samples m = sub + 8 i (frequency 8 w):
+        // states (p1, p2) for x w and (q1, q2) for x d, carried across the
+        // staging passes; (gc, gs) = (cos, sin) 8 w and (pc, ps) = (cos, sin)
+        // of w m_last, both exact table entries.
+        double p1[kPer], p2[kPer], q1[kPer], q2[kPer], gc8[kPer], gs8[kPer], pc[kPer], ps[kPer];
         bool act[kPer];
+        const int mlast = sub < len ? sub + kSplit * ((len - 1 - sub) / kSplit) : 0;
 #pragma unroll
         for (int qq = 0; qq < kPer; ++qq) {
           const int j = jb + grp + 8 * qq;
           act[qq] = j < nbt;
           const int k = act[qq] ? L.bink[j0 + j] : 0;
-          // Twiddle of this lane's first sample m = sub, and the rotation
-          // for m -> m + 8.
-          twiddle_at(twiddle, tstride, n, (int)(((long long)k * sub) & (n - 1)), tc[qq], ts[qq]);
-          twiddle_at(twiddle, tstride, n, (k * kSplit) & (n - 1), rc[qq], rs[qq]);
-          mre[qq] = mim[qq] = dre[qq] = dim[qq] = 0.0;
+          twiddle_at(twiddle, tstride, n, (k * kSplit) & (n - 1), gc8[qq], gs8[qq]);
+          twiddle_at(twiddle, tstride, n, (int)(((long long)k * mlast) & (n - 1)), pc[qq], ps[qq]);
+          p1[qq] = p2[qq] = q1[qq] = q2[qq] = 0.0;
         }
         const bool fill = !staged || nstage > 1;
         double wc = wc0, ws = ws0, cur = 0.0, prev_last = 0.0;
@@ -1151,38 +1155,44 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
 #pragma unroll
           for (int qq = 0; qq < kPer; ++qq) {
             if (!act[qq]) continue;
-            double cs = tc[qq], sn = ts[qq];
-            const double ac = rc[qq], as = rs[qq];
-#pragma unroll 2
+            const double c2 = 2.0 * gc8[qq];
+            double a1 = p1[qq], a2 = p2[qq], b1 = q1[qq], b2 = q2[qq];
+#pragma unroll 1
             for (int m = c0 + sub; m < cend; m += kSplit) {
               const double xm = L.xwd[2 * (m - c0)], xd = L.xwd[2 * (m - c0) + 1];
-              mre[qq] = fma(xm, cs, mre[qq]);
-              mim[qq] = fma(-xm, sn, mim[qq]);
-              dre[qq] = fma(xd, cs, dre[qq]);
-              dim[qq] = fma(-xd, sn, dim[qq]);
-              const double nc = fma(cs, ac, -(sn * as)), ns = fma(sn, ac, cs * as);
-              cs = nc;
-              sn = ns;
+              const double a0 = fma(c2, a1, xm) - a2;
+              const double b0 = fma(c2, b1, xd) - b2;
+              a2 = a1;
+              a1 = a0;
+              b2 = b1;
+              b1 = b0;
             }
-            tc[qq] = cs;
-            ts[qq] = sn;
+            p1[qq] = a1;
+            p2[qq] = a2;
+            q1[qq] = b1;
+            q2[qq] = b2;
           }
           if (nstage > 1) wave_sync();
         }
         if (fill && nstage == 1) staged = true;
 #pragma unroll
         for (int qq = 0; qq < kPer; ++qq) {
+          // sum_i x_i e^{-i w (m0 + 8 i)} = e^{-i w m_last} (s_{N-1} - e^{-8 i w} s_{N-2}).
+          const double yr = p1[qq] - gc8[qq] * p2[qq], yi = gs8[qq] * p2[qq];
+          const double zr = q1[qq] - gc8[qq] * q2[qq], zi = gs8[qq] * q2[qq];
+          double mre = pc[qq] * yr + ps[qq] * yi, mim = pc[qq] * yi - ps[qq] * yr;
+          double dre = pc[qq] * zr + ps[qq] * zi, dim = pc[qq] * zi - ps[qq] * zr;
 #pragma unroll
           for (int dd = kSplit / 2; dd > 0; dd >>= 1) {
-            mre[qq] += __shfl_xor(mre[qq], dd, kSplit);
-            mim[qq] += __shfl_xor(mim[qq], dd, kSplit);
-            dre[qq] += __shfl_xor(dre[qq], dd, kSplit);
-            dim[qq] += __shfl_xor(dim[qq], dd, kSplit);
+            mre += __shfl_xor(mre, dd, kSplit);
+            mim += __shfl_xor(mim, dd, kSplit);
+            dre += __shfl_xor(dre, dd, kSplit);
+            dim += __shfl_xor(dim, dd, kSplit);
           }
           const int j = jb + grp + 8 * qq;
           if (sub == 0 && act[qq]) {
-            L.res[2 * (j0 + j)] = mre[qq] * mre[qq] + mim[qq] * mim[qq];
-            L.res[2 * (j0 + j) + 1] = mre[qq] * dim[qq] - mim[qq] * dre[qq];
+            L.res[2 * (j0 + j)] = mre * mre + mim * mim;
+            L.res[2 * (j0 + j) + 1] = mre * dim - mim * dre;
           }
         }
       }
