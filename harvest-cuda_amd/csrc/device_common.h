@@ -16,6 +16,9 @@ constexpr int kSlabCap = kTile / 2;  // max crossings of one type per tile
 constexpr int kChunk = 128;          // IIR samples per scan chunk
 constexpr int kMeanBlock = 4096;     // samples per deterministic partial sum
 constexpr int kWindowMargin = 4;
+// Frames per thread in k_interp: a tile spans kTile r 1000 / fs frames, at
+// most 385 for the smallest working rate (fs / r >= 5333 Hz); launches check it.
+constexpr int kInterpFramesPerThread = 2;
 // Work counters are spread over this many addresses (by workgroup) so that
 // the per-wave atomics of a launch do not queue on one address; the host sums.
 constexpr int kWorkSpread = 64;     // crossings kept either side of a tile

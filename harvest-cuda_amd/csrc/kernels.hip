@@ -613,9 +613,14 @@ __global__ __launch_bounds__(kThreads) void k_interp(
     long long total_frames, int band0, const double* boundaries,
     const double* slab, const int* counts, const int* offs, const int* totals,
     double* raw) {
+  // One crossing type at a time: its edges of ranks [wlo, whi] are staged,
+  // then the interval locations loc_k = (e_k + e_{k+1}) / 2 / fs and values
+  // val_k = fs / (e_{k+1} - e_k) (harvest.cpp:222-236) are computed once, so
+  // histc's search only compares; each frame keeps its running sum of the
+  // four types' interp1 values in registers, in type order.
   constexpr int kWin = kSlabCap + 2 * kWindowMargin + 2;
-  __shared__ double win[4][kWin];
-  __shared__ int wlo_s[4], whi_s[4], E_s[4];
+  constexpr int kReps = kInterpFramesPerThread;
+  __shared__ double edge[kWin], loc[kWin], val[kWin];
   const long long g = blockIdx.x;
   const int bg = blockIdx.y;
   const int b = band0 + bg;
@@ -637,14 +642,17 @@ __global__ __launch_bounds__(kThreads) void k_interp(
     for (long long i = i_lo + threadIdx.x; i <= i_hi; i += kThreads) out[i] = 0.0;
     return;
   }
-  for (int t = 0; t < 4; ++t) {
-    const long long q = bg * 4 + t;
+  const double fs = c.fs_work;
+  double sum[kReps];
+  for (int ty = 0; ty < 4; ++ty) {
+    const long long q = bg * 4 + ty;
     const int* qcnt = counts + q * total_tiles;
     const int* qoff = offs + q * total_tiles;
     const int E = totals[q * c.count + ui];
     const int r_lo = qoff[g], n_in = qcnt[g];
     const int wlo = max(0, r_lo - kWindowMargin);
     const int whi = min(E - 1, r_lo + n_in - 1 + kWindowMargin);
+    if (ty > 0) __syncthreads();  // the previous type's readers are done
     for (int k = wlo + threadIdx.x; k <= whi; k += kThreads) {
       long long gt;
       int local;
@@ -660,46 +668,46 @@ __global__ __launch_bounds__(kThreads) void k_interp(
         gt = lo;
         local = k - qoff[gt];
       }
-      win[t][k - wlo] = slab[(q * total_tiles + gt) * kSlabCap + local];
+      edge[k - wlo] = slab[(q * total_tiles + gt) * kSlabCap + local];
     }
-    if (threadIdx.x == 0) {
-      wlo_s[t] = wlo;
-      whi_s[t] = whi;
-      E_s[t] = E;
+    __syncthreads();
+    for (int k = wlo + threadIdx.x; k < whi; k += kThreads) {
+      const double e0 = edge[k - wlo], e1 = edge[k + 1 - wlo];
+      loc[k - wlo] = (e0 + e1) / 2.0 / fs;
+      val[k - wlo] = fs / (e1 - e0);
     }
-  }
-  __syncthreads();
-
-  const double fs = c.fs_work;
-  const double bound = boundaries[b];
-  const double upper = bound * 1.1, lower = bound * 0.9;
-  for (long long i = i_lo + threadIdx.x; i <= i_hi; i += kThreads) {
-    const double t = (double)i / 1000.0;
-    double v[4];
+    __syncthreads();
 #pragma unroll
-    for (int ty = 0; ty < 4; ++ty) {
-      const double* fe = win[ty];
-      const int wlo = wlo_s[ty], E = E_s[ty];
+    for (int rep = 0; rep < kReps; ++rep) {
+      const long long i = i_lo + threadIdx.x + (long long)rep * kThreads;
+      if (i > i_hi) continue;
+      const double t = (double)i / 1000.0;
       // k* = number of interval locations <= t (histc); loc_k needs edges
       // k and k + 1, so k ranges over [wlo, min(whi, E - 1)).
       int lo = wlo;
-      int hi = min(whi_s[ty], E - 1);
+      int hi = min(whi, E - 1);
       while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        double loc = (fe[mid - wlo] + fe[mid + 1 - wlo]) / 2.0 / fs;
-        if (loc <= t) lo = mid + 1; else hi = mid;
+        const int mid = (lo + hi) >> 1;
+        if (loc[mid - wlo] <= t) lo = mid + 1; else hi = mid;
       }
       // histc clamps to [1, E - 2]; the window bounds hold by construction
       // (ranks either side of the tile) and are enforced here as well.
       int kc = min(max(lo, 1), E - 2);
-      kc = min(max(kc, wlo + 1), whi_s[ty] - 1);
-      const double e0 = fe[kc - 1 - wlo], e1 = fe[kc - wlo], e2 = fe[kc + 1 - wlo];
-      const double x0 = (e0 + e1) / 2.0 / fs, x1 = (e1 + e2) / 2.0 / fs;
-      const double y0 = fs / (e1 - e0), y1 = fs / (e2 - e1);
+      kc = min(max(kc, wlo + 1), whi - 1);
+      const double x0 = loc[kc - 1 - wlo], x1 = loc[kc - wlo];
+      const double y0 = val[kc - 1 - wlo], y1 = val[kc - wlo];
       const double s = (t - x0) / (x1 - x0);
-      v[ty] = y0 + s * (y1 - y0);
+      const double v = y0 + s * (y1 - y0);
+      sum[rep] = ty == 0 ? v : sum[rep] + v;
     }
-    double cand = (v[0] + v[1] + v[2] + v[3]) / 4.0;
+  }
+  const double bound = boundaries[b];
+  const double upper = bound * 1.1, lower = bound * 0.9;
+#pragma unroll
+  for (int rep = 0; rep < kReps; ++rep) {
+    const long long i = i_lo + threadIdx.x + (long long)rep * kThreads;
+    if (i > i_hi) continue;
+    double cand = sum[rep] / 4.0;
     if (cand > upper || cand < lower || cand > c.f0_ceil || cand < c.f0_floor) cand = 0.0;
     out[i] = cand;
   }
@@ -1657,6 +1665,11 @@ void launch_interp(hipStream_t s, const Config& c, const Utt* utt,
                    const double* boundaries, const double* slab,
                    const int* counts, const int* offs, const int* totals,
                    double* raw) {
+  // Frames per tile: at most ceil(kTile r 1000 / fs) + 1.
+  const long long per_tile = ((long long)kTile * 1000LL * c.ratio + c.fs - 1) / c.fs + 1;
+  if (per_tile > (long long)kInterpFramesPerThread * kThreads)
+    throw std::runtime_error("interp: " + std::to_string(per_tile) +
+                             " frames per tile exceed the kernel's threads");
   dim3 grid((unsigned)total_tiles, (unsigned)group);
   k_interp<<<grid, kThreads, 0, s>>>(c, utt, t_off, total_tiles, total_frames, band0,
                                      boundaries, slab, counts, offs, totals, raw);
