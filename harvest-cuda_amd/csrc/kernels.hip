@@ -1428,65 +1428,89 @@ __global__ __launch_bounds__(64) void k_refine_entries(
   }
 }
 
-// RemoveUnreliableCandidates (harvest.cpp:652-688) against the pre-prune
-// values of frames i +- 1.  min_k fl(|r - c_k| / r) = fl(min_k |r - c_k| / r)
-// because rounding is monotone, so one division per neighbour frame suffices.
-__global__ __launch_bounds__(kThreads) void k_prune(
+// RemoveUnreliableCandidates (harvest.cpp:652-688) and SearchF0Base
+// (:693-705), one wavefront per frame.  The neighbour frames' nonzero
+// pre-prune values are compacted into LDS first; a zero neighbour value only
+// adds the error min(|r - 0| / r, 1) = 1, which the clamp already implies, and
+// is taken into account for fidelity.  min_k fl(|r - c_k| / r) =
+// fl(min_k |r - c_k| / r) because rounding is monotone, so one division per
+// neighbour frame suffices.  The best F0 is that of the first slot with the
+// largest score (strictly greater than 0), found by a wave reduction that
+// breaks score ties by the lower slot index.
+__global__ __launch_bounds__(kThreads) void k_prune_best(
     Config c, const Utt* utt, const long long* f_off, long long total_frames,
     const int* count, const double* rf0, const double* rscore, double* pf0,
-    double* pscore) {
-  const long long id = (long long)blockIdx.x * kThreads + threadIdx.x;
-  const long long gf = id / c.slots;
-  const int s = (int)(id % c.slots);
+    double* pscore, double* best, int* row_nnz) {
+  extern __shared__ double prune_raw[];
+  const int S = c.slots;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  LDS double* nxt = (LDS double*)(LDS char*)(char*)prune_raw + (size_t)wave * 2 * S;
+  LDS double* prv = nxt + S;
+  const long long gf = (long long)blockIdx.x * kWaves + wave;
   if (gf >= total_frames) return;
   const int ui = find_segment(f_off, c.count, gf);
   const int ncand = 7 * count[ui];
-  if (s >= ncand) return;
   const Utt u = utt[ui];
   const long long i = gf - u.f_off;
-  double r = rf0[gf * c.slots + s], sc = rscore[gf * c.slots + s];
-  if (r != 0.0 && i >= 1 && i <= u.frames - 2) {
-    double m1 = INFINITY, m2 = INFINITY;
-    const double* nxt = rf0 + (gf + 1) * c.slots;
-    const double* prv = rf0 + (gf - 1) * c.slots;
-    for (int k = 0; k < ncand; ++k) {
-      m1 = fmin(m1, fabs(r - nxt[k]));
-      m2 = fmin(m2, fabs(r - prv[k]));
-    }
-    double e1 = m1 / r, e2 = m2 / r;
-    if (e1 > 1.0) e1 = 1.0;
-    if (e2 > 1.0) e2 = 1.0;
-    const double e = e1 < e2 ? e1 : e2;
-    if (!(e <= 0.05)) {
-      r = 0.0;
-      sc = 0.0;
+  const bool interior = i >= 1 && i <= u.frames - 2;
+  int n1 = 0, n2 = 0;
+  if (interior) {
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int base = 0; base < ncand; base += 64) {
+      const int k = base + lane;
+      const double a = k < ncand ? rf0[(gf + 1) * S + k] : 0.0;
+      const double p = k < ncand ? rf0[(gf - 1) * S + k] : 0.0;
+      const unsigned long long ma = __ballot(a != 0.0), mp = __ballot(p != 0.0);
+      if (a != 0.0) nxt[n1 + __popcll(ma & below)] = a;
+      if (p != 0.0) prv[n2 + __popcll(mp & below)] = p;
+      n1 += __popcll(ma);
+      n2 += __popcll(mp);
     }
   }
-  pf0[gf * c.slots + s] = r;
-  pscore[gf * c.slots + s] = sc;
-}
-
-// SearchF0Base (harvest.cpp:693-705) and the nonzero count of each frame.
-__global__ __launch_bounds__(kThreads) void k_best(Config c, const long long* f_off,
-                                                  long long total_frames,
-                                                  const int* count, const double* pf0,
-                                                  const double* pscore, double* best,
-                                                  int* row_nnz) {
-  const long long gf = (long long)blockIdx.x * kThreads + threadIdx.x;
-  if (gf >= total_frames) return;
-  const int ncand = 7 * count[find_segment(f_off, c.count, gf)];
-  double b = 0.0, bs = 0.0;
-  int nz = 0;
-  for (int k = 0; k < ncand; ++k) {
-    const double sc = pscore[gf * c.slots + k], f = pf0[gf * c.slots + k];
+  wave_sync();
+  double bs = 0.0, bf = 0.0;
+  int bk = 0x7fffffff, nz = 0;
+  for (int s = lane; s < ncand; s += 64) {
+    double r = rf0[gf * S + s], sc = rscore[gf * S + s];
+    if (r != 0.0 && interior) {
+      double m1 = INFINITY, m2 = INFINITY;
+      for (int k = 0; k < n1; ++k) m1 = fmin(m1, fabs(r - nxt[k]));
+      for (int k = 0; k < n2; ++k) m2 = fmin(m2, fabs(r - prv[k]));
+      if (n1 < ncand) m1 = fmin(m1, fabs(r));
+      if (n2 < ncand) m2 = fmin(m2, fabs(r));
+      double e1 = m1 / r, e2 = m2 / r;
+      if (e1 > 1.0) e1 = 1.0;
+      if (e2 > 1.0) e2 = 1.0;
+      const double e = e1 < e2 ? e1 : e2;
+      if (!(e <= 0.05)) {
+        r = 0.0;
+        sc = 0.0;
+      }
+    }
+    pf0[gf * S + s] = r;
+    pscore[gf * S + s] = sc;
     if (sc > bs) {
-      b = f;
       bs = sc;
+      bf = r;
+      bk = s;
     }
-    nz += f != 0.0;
+    nz += r != 0.0;
   }
-  best[gf] = b;
-  row_nnz[gf] = nz;
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    const double obs = __shfl_xor(bs, d, 64), obf = __shfl_xor(bf, d, 64);
+    const int obk = __shfl_xor(bk, d, 64);
+    if (obs > bs || (obs == bs && obk < bk)) {
+      bs = obs;
+      bf = obf;
+      bk = obk;
+    }
+    nz += __shfl_xor(nz, d, 64);
+  }
+  if (lane == 0) {
+    best[gf] = bs > 0.0 ? bf : 0.0;
+    row_nnz[gf] = nz;
+  }
 }
 
 // Order-preserving compaction of the nonzero pruned candidates, each with the
@@ -1709,10 +1733,11 @@ void launch_prune_best(hipStream_t s, const Config& c, const Utt* utt,
                        const int* count, const double* rf0,
                        const double* rscore, double* pf0, double* pscore,
                        double* best, int* row_nnz) {
-  k_prune<<<grid_for(total_frames * c.slots), kThreads, 0, s>>>(
-      c, utt, f_off, total_frames, count, rf0, rscore, pf0, pscore);
-  k_best<<<grid_for(total_frames), kThreads, 0, s>>>(c, f_off, total_frames, count, pf0,
-                                                    pscore, best, row_nnz);
+  const size_t bytes = (size_t)kWaves * 2 * c.slots * sizeof(double);
+  if (bytes > (size_t)c.lds_limit) throw std::runtime_error("prune: too many slots for LDS");
+  const unsigned grid = (unsigned)((total_frames + kWaves - 1) / kWaves);
+  k_prune_best<<<grid, kThreads, bytes, s>>>(c, utt, f_off, total_frames, count, rf0, rscore,
+                                             pf0, pscore, best, row_nnz);
   check_launch("prune_best");
 }
 
