@@ -94,6 +94,10 @@ __device__ __forceinline__ long long select_base(const Config& c, const Utt& u) 
   return u.m - 9 - u.nbeg - (long long)(c.lag / c.ratio) * c.ratio;
 }
 
+// Sequential loops load this many inputs ahead, so that the loads' latency
+// overlaps instead of stalling every step of the recursion.
+constexpr int kIirAhead = 8;
+
 template <int PASS>
 __device__ __forceinline__ double iir_input(const Config& c, const Utt& u,
                                             const double* x, const double* w,
@@ -121,11 +125,20 @@ __global__ __launch_bounds__(kThreads) void k_iir_chunk_state(
   long long start = (cg - u.c_off) * kChunk;
   long long end = min(start + (long long)kChunk, u.m);
   double w0 = 0.0, w1 = 0.0, w2 = 0.0;
-  for (long long i = start; i < end; ++i) {
-    double wt = iir_input<PASS>(c, u, x, w, i) + c.a0 * w0 + c.a1 * w1 + c.a2 * w2;
-    w2 = w1;
-    w1 = w0;
-    w0 = wt;
+  // Inputs are loaded kAhead at a time so that their latency overlaps.
+  for (long long i0 = start; i0 < end; i0 += kIirAhead) {
+    double in[kIirAhead];
+#pragma unroll
+    for (int q = 0; q < kIirAhead; ++q)
+      in[q] = i0 + q < end ? iir_input<PASS>(c, u, x, w, i0 + q) : 0.0;
+#pragma unroll
+    for (int q = 0; q < kIirAhead; ++q) {
+      if (i0 + q >= end) break;
+      double wt = in[q] + c.a0 * w0 + c.a1 * w1 + c.a2 * w2;
+      w2 = w1;
+      w1 = w0;
+      w0 = wt;
+    }
   }
   states[cg * 3 + 0] = w0;
   states[cg * 3 + 1] = w1;
@@ -179,7 +192,17 @@ __global__ __launch_bounds__(kThreads) void k_iir_carry(Config c, const Utt* utt
   const long long last = min(first + (long long)K, (long long)C);
   const double* z = states + u.c_off * 3;
   double agg[3] = {0.0, 0.0, 0.0};
-  for (long long k = first; k < last; ++k) mat3_apply(P, agg, z + k * 3, agg);
+  for (long long k0 = first; k0 < last; k0 += kIirAhead) {
+    double zz[kIirAhead][3];
+#pragma unroll
+    for (int q = 0; q < kIirAhead; ++q)
+      for (int r = 0; r < 3; ++r) zz[q][r] = k0 + q < last ? z[(k0 + q) * 3 + r] : 0.0;
+#pragma unroll
+    for (int q = 0; q < kIirAhead; ++q) {
+      if (k0 + q >= last) break;
+      mat3_apply(P, agg, zz[q], agg);
+    }
+  }
   scan[threadIdx.x][0] = agg[0];
   scan[threadIdx.x][1] = agg[1];
   scan[threadIdx.x][2] = agg[2];
@@ -205,11 +228,20 @@ __global__ __launch_bounds__(kThreads) void k_iir_carry(Config c, const Utt* utt
     carry[2] = scan[threadIdx.x - 1][2];
   }
   double* out = carries + u.c_off * 3;
-  for (long long k = first; k < last; ++k) {
-    out[k * 3 + 0] = carry[0];
-    out[k * 3 + 1] = carry[1];
-    out[k * 3 + 2] = carry[2];
-    mat3_apply(P, carry, z + k * 3, carry);
+  for (long long k0 = first; k0 < last; k0 += kIirAhead) {
+    double zz[kIirAhead][3];
+#pragma unroll
+    for (int q = 0; q < kIirAhead; ++q)
+      for (int r = 0; r < 3; ++r) zz[q][r] = k0 + q < last ? z[(k0 + q) * 3 + r] : 0.0;
+#pragma unroll
+    for (int q = 0; q < kIirAhead; ++q) {
+      const long long k = k0 + q;
+      if (k >= last) break;
+      out[k * 3 + 0] = carry[0];
+      out[k * 3 + 1] = carry[1];
+      out[k * 3 + 2] = carry[2];
+      mat3_apply(P, carry, zz[q], carry);
+    }
   }
 }
 
@@ -224,21 +256,30 @@ __global__ __launch_bounds__(kThreads) void k_iir_replay(
   long long end = min(start + (long long)kChunk, u.m);
   double w0 = carries[cg * 3 + 0], w1 = carries[cg * 3 + 1], w2 = carries[cg * 3 + 2];
   const long long out_base = select_base(c, u);
-  for (long long i = start; i < end; ++i) {
-    double wt = iir_input<PASS>(c, u, x, w, i) + c.a0 * w0 + c.a1 * w1 + c.a2 * w2;
-    double out = c.b0 * wt + c.b1 * w0 + c.b1 * w1 + c.b0 * w2;
-    w2 = w1;
-    w1 = w0;
-    w0 = wt;
-    if (PASS == 0 || PASS == 2) {
-      w[u.m_off + i] = out;
-    } else if (PASS == 3) {  // y is the full-length output buffer here
-      y[u.m_off + i] = out;
-    } else {
-      long long d = out_base - i;
-      if (d >= 0 && d % c.ratio == 0) {
-        long long j = d / c.ratio;
-        if (j < u.n) y[u.y_off + j] = out;
+  for (long long i0 = start; i0 < end; i0 += kIirAhead) {
+    double in[kIirAhead];
+#pragma unroll
+    for (int q = 0; q < kIirAhead; ++q)
+      in[q] = i0 + q < end ? iir_input<PASS>(c, u, x, w, i0 + q) : 0.0;
+#pragma unroll
+    for (int q = 0; q < kIirAhead; ++q) {
+      const long long i = i0 + q;
+      if (i >= end) break;
+      double wt = in[q] + c.a0 * w0 + c.a1 * w1 + c.a2 * w2;
+      double out = c.b0 * wt + c.b1 * w0 + c.b1 * w1 + c.b0 * w2;
+      w2 = w1;
+      w1 = w0;
+      w0 = wt;
+      if (PASS == 0 || PASS == 2) {
+        w[u.m_off + i] = out;
+      } else if (PASS == 3) {  // y is the full-length output buffer here
+        y[u.m_off + i] = out;
+      } else {
+        long long d = out_base - i;
+        if (d >= 0 && d % c.ratio == 0) {
+          long long j = d / c.ratio;
+          if (j < u.n) y[u.y_off + j] = out;
+        }
       }
     }
   }
