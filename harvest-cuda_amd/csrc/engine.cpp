@@ -11,11 +11,15 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <condition_variable>
 #include <exception>
+#include <functional>
 #include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <thread>
+
+#include <unistd.h>
 
 namespace harvest {
 namespace {
@@ -30,36 +34,124 @@ constexpr int kParallelFrames = 16384;
 // Threads per utterance for its sections; more gain nothing on ~100 sections.
 constexpr int kSectionThreads = 8;
 
-// Runs f(i) for i in [0, n) on up to `threads` threads, the caller included.
-// The first exception thrown by f is rethrown once every thread has joined;
-// if the system refuses a thread, the remaining ones share the work.
+// A persistent pool of host threads for the per-section work of long
+// contours (thread creation per call would cost as much as the work).  One
+// job runs at a time: a caller that finds the pool busy (another call, or a
+// job nested inside a job) runs its loop on its own thread instead, so no
+// call ever waits for workers that are waiting for it.
+class HostPool {
+ public:
+  static HostPool& instance() {
+    static HostPool pool;
+    return pool;
+  }
+
+  // Runs body(i) for i in [0, n) on up to `threads` threads, the caller
+  // included.  The first exception thrown by body is rethrown once every
+  // participant has finished.
+  void run(int n, int threads, const std::function<void(int)>& body) {
+    threads = std::max(1, std::min(threads, n));
+    std::unique_lock<std::mutex> job_lock(job_mutex_, std::try_to_lock);
+    if (threads == 1 || !job_lock.owns_lock()) {
+      for (int i = 0; i < n; ++i) body(i);
+      return;
+    }
+    {
+      std::unique_lock<std::mutex> lk(mutex_);
+      forget_after_fork();
+      while ((int)workers_.size() < threads - 1) {
+        try {
+          workers_.emplace_back([this] { work(); });
+        } catch (...) {
+          break;  // the system refused a thread: fewer participants
+        }
+      }
+      body_ = &body;
+      n_ = n;
+      next_ = 0;
+      error_ = nullptr;
+      wanted_ = std::min(threads - 1, (int)workers_.size());
+      running_ = wanted_;
+      ++generation_;
+    }
+    wake_.notify_all();
+    participate();
+    std::unique_lock<std::mutex> lk(mutex_);
+    done_.wait(lk, [this] { return running_ == 0; });
+    body_ = nullptr;
+    if (error_) std::rethrow_exception(error_);
+  }
+
+  ~HostPool() {
+    {
+      std::lock_guard<std::mutex> lk(mutex_);
+      forget_after_fork();
+      stop_ = true;
+    }
+    wake_.notify_all();
+    for (auto& t : workers_) t.join();
+  }
+
+ private:
+  // A forked child inherits the thread objects but not the threads: they are
+  // released without joining, and the child starts its own.
+  void forget_after_fork() {
+    if (owner_ == getpid()) return;
+    if (!workers_.empty()) new std::vector<std::thread>(std::move(workers_));  // never joined
+    workers_.clear();
+    owner_ = getpid();
+  }
+
+  void participate() {
+    try {
+      for (int i = next_++; i < n_; i = next_++) (*body_)(i);
+    } catch (...) {
+      std::lock_guard<std::mutex> lk(mutex_);
+      if (!error_) error_ = std::current_exception();
+      next_ = n_;
+    }
+  }
+
+  void work() {
+    unsigned long long seen = 0;
+    while (true) {
+      {
+        std::unique_lock<std::mutex> lk(mutex_);
+        wake_.wait(lk, [&] { return stop_ || (generation_ != seen && wanted_ > 0); });
+        if (stop_) return;
+        seen = generation_;
+        --wanted_;
+      }
+      participate();
+      {
+        std::lock_guard<std::mutex> lk(mutex_);
+        --running_;
+      }
+      done_.notify_all();
+    }
+  }
+
+  std::mutex job_mutex_;  // held by the caller for the whole job
+  std::mutex mutex_;
+  std::condition_variable wake_, done_;
+  std::vector<std::thread> workers_;
+  const std::function<void(int)>* body_ = nullptr;
+  std::atomic<int> next_{0};
+  int n_ = 0, wanted_ = 0, running_ = 0;
+  unsigned long long generation_ = 0;
+  std::exception_ptr error_;
+  bool stop_ = false;
+  pid_t owner_ = getpid();
+};
+
 template <class F> void parallel_for(int n, int threads, F&& f) {
-  threads = std::max(1, std::min(threads, n));
-  if (threads == 1) {
+  if (n <= 0) return;
+  if (threads <= 1 || n == 1) {
     for (int i = 0; i < n; ++i) f(i);
     return;
   }
-  std::atomic<int> next{0};
-  std::exception_ptr error;
-  std::mutex error_mutex;
-  auto work = [&] {
-    try {
-      for (int i = next++; i < n; i = next++) f(i);
-    } catch (...) {
-      std::lock_guard<std::mutex> guard(error_mutex);
-      if (!error) error = std::current_exception();
-      next = n;
-    }
-  };
-  std::vector<std::thread> pool;
-  pool.reserve(threads - 1);
-  try {
-    for (int t = 1; t < threads; ++t) pool.emplace_back(work);
-  } catch (...) {
-  }
-  work();
-  for (auto& t : pool) t.join();
-  if (error) std::rethrow_exception(error);
+  const std::function<void(int)> body = [&](int i) { f(i); };
+  HostPool::instance().run(n, threads, body);
 }
 
 // Sections of a contour as GetBoundaryList (harvest.cpp:727-743) reports

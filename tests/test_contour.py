@@ -149,3 +149,51 @@ def test_contour_matches_world_live(core, oracle):
         got = engine_contour(core, oracle, f0, sc)
         np.testing.assert_array_equal(got, expect, err_msg=f"trial {trial}")
         np.testing.assert_array_equal(core._smooth(got), oracle.ref_smooth(got))
+
+
+def _long_contour(seed=99):
+    rng = np.random.default_rng(seed)
+    f0, sc = random_matrix(rng, 1)
+    reps = -(-30000 // f0.shape[0])
+    return np.tile(f0, (reps, 1)), np.tile(sc, (reps, 1))
+
+
+def test_threaded_sections_from_concurrent_callers(core, oracle):
+    """Several host threads use the section pool at once (the busy pool runs a
+    caller's loop on its own thread); every result stays bit-identical."""
+    from concurrent.futures import ThreadPoolExecutor
+    f0, sc = _long_contour()
+    off, cf, cs = csr(f0, sc)
+    base = oracle.base(f0, sc, f0.shape[1])
+    want = core._fix_contour(off, cf, cs, base)
+    want_s = core._smooth(want)
+    with ThreadPoolExecutor(max_workers=4) as pool:
+        futs = [pool.submit(core._fix_contour, off, cf, cs, base, threads=4) for _ in range(6)]
+        futs += [pool.submit(core._smooth, want, threads=4) for _ in range(6)]
+        res = [f.result() for f in futs]
+    for r in res[:6]:
+        np.testing.assert_array_equal(r, want)
+    for r in res[6:]:
+        np.testing.assert_array_equal(r, want_s)
+
+
+def _smooth_in_child(args):
+    import sys
+    from pathlib import Path
+    sys.path.insert(0, str(Path(__file__).resolve().parents[1] / "harvest-cuda_amd"))
+    from harvest_cuda import _core
+    f0, threads = args
+    return _core._smooth(f0, threads=threads)
+
+
+def test_section_pool_survives_fork(core):
+    """A forked child (e.g. a data-loader worker) starts its own pool threads
+    instead of waiting for the parent's."""
+    import multiprocessing as mp
+    f0, sc = _long_contour(7)
+    contour = np.where(f0[:, 0] > 0, f0[:, 0], 0.0)
+    want = core._smooth(contour, threads=4)  # the parent's pool now has workers
+    ctx = mp.get_context("fork")
+    with ctx.Pool(1) as pool:
+        got = pool.apply_async(_smooth_in_child, ((contour, 4),)).get(timeout=120)
+    np.testing.assert_array_equal(got, want)
