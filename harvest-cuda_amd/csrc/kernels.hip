@@ -930,6 +930,27 @@ __device__ __forceinline__ double lane_value(double v, int src) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// Whole-wave DPP shifts (CDNA row masks all set): lane l receives v of lane
+// l + 1 (from_next_lane) or l - 1 (from_prev_lane); the lane without a source
+// receives `edge`.
+template <int kCtrl>
+__device__ __forceinline__ double wave_shift(double v, double edge) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned long long o = (unsigned long long)__double_as_longlong(edge);
+  const unsigned lo =
+      (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)o, (int)(unsigned)b, kCtrl, 0xF, 0xF, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32),
+                                                            (int)(unsigned)(b >> 32), kCtrl, 0xF,
+                                                            0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ double from_next_lane(double v, double edge) {
+  return wave_shift<0x130>(v, edge);  // wave_shl:1
+}
+__device__ __forceinline__ double from_prev_lane(double v, double edge) {
+  return wave_shift<0x138>(v, edge);  // wave_shr:1
+}
+
 // Workgroup b of G is dispatched to XCD b mod 8; renumbered so that each XCD
 // takes one contiguous range (its L2 then holds one slice of the signal).
 __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned G) {
@@ -975,7 +996,9 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
     double* rf0, double* rscore, unsigned long long* work) {
   extern __shared__ double lds_raw[];
   const int S = c.slots;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // The wave index is uniform: readfirstlane lets the compiler keep the
+  // frame's scalars (frame, utterance, counts) in scalar registers.
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int grp = lane / kSplit, sub = lane % kSplit;
   const WaveLds L =
       carve_wave_lds((LDS char*)(char*)lds_raw + wave * wave_lds_bytes(S, HS, W), S, HS, W);
@@ -1037,7 +1060,7 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
     L.win[s] = pos;
   }
   wave_sync();
-  const int nwin = L.misc[0];
+  const int nwin = __builtin_amdgcn_readfirstlane(L.misc[0]);
   for (int s = lane; s < ncand; s += 64) L.win[s] = L.win[s] >= 0 ? L.hval[L.win[s]] : -1;
   // Per-window constants, lane w holding those of window w < 64: the first
   // sample index (GetBaseIndex), the Blackman phase offset a and the rotation
@@ -1052,14 +1075,14 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
   }
   auto window_basic = [&](int w) -> int {
     if (w < 64) return __builtin_amdgcn_readlane(pw_basic, w);
-    return mround((t + (-L.win_h[w]) / fs) * fs + 0.001);
+    return mround((t + (-__builtin_amdgcn_readfirstlane(L.win_h[w])) / fs) * fs + 0.001);
   };
   // The first 128 windowed samples of the next window are loaded ahead into
   // registers, overlapping their latency with the current window's work.
   double ypre0 = 0.0, ypre1 = 0.0;
   auto prefetch = [&](int w) {
     if (w >= nwin) return;
-    const int bs = window_basic(w), len = 2 * L.win_h[w] + 1;
+    const int bs = window_basic(w), len = 2 * __builtin_amdgcn_readfirstlane(L.win_h[w]) + 1;
     ypre0 = lane < len ? y_at(bs + lane - 1) : 0.0;
     ypre1 = lane + 64 < len ? y_at(bs + lane + 63) : 0.0;
   };
@@ -1117,7 +1140,8 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
   };
   // Finishes the slots of windows [w0, w1).
   auto flush = [&](int w0, int w1) {
-    const int b = wbeg[w0], e = wbeg[w1];
+    const int b = __builtin_amdgcn_readfirstlane(wbeg[w0]);
+    const int e = __builtin_amdgcn_readfirstlane(wbeg[w1]);
     for (int p = b + lane; p < e; p += 64) finish(L.wslots[p]);
     wave_sync();
   };
@@ -1125,8 +1149,9 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
   const int stride = c.twiddle_n > 0 ? c.twiddle_n : 0;  // divided by n per window
   int rbase = 0, wflush = 0;
   for (int w = 0; w < nwin; ++w) {
-    const int hw = L.win_h[w], len = 2 * hw + 1, n = window_fft(hw);
-    const int b = wbeg[w], e = wbeg[w + 1], pairs = 6 * (e - b);
+    const int hw = __builtin_amdgcn_readfirstlane(L.win_h[w]), len = 2 * hw + 1, n = window_fft(hw);
+    const int b = __builtin_amdgcn_readfirstlane(wbeg[w]);
+    const int e = __builtin_amdgcn_readfirstlane(wbeg[w + 1]), pairs = 6 * (e - b);
     const int tstride = stride / n;  // 0: no table covers n
     // Blackman argument (harvest.cpp:446-456) in units of pi:
     // 2 tmp / wl with tmp = (basic + m - 1) / fs - t and wl = len / fs.
@@ -1179,18 +1204,14 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
               wc = nc;
               ws = ns;
               const double nxt = blackman_cos(wc);  // sample m + 64
-              const double next_first = lane_value(nxt, 0);
-              const double last = lane_value(cur, 63);
-              double wn = __shfl_down(cur, 1, 64);
-              double wp = __shfl_up(cur, 1, 64);
-              if (lane == 63) wn = next_first;
-              if (lane == 0) wp = prev_last;
-              prev_last = last;
+              // Neighbours w[m + 1] and w[m - 1]; w[-1] = w[len] = 0 gives
+              // d[0] = -w[1] / 2 and d[len - 1] = w[len - 2] / 2.
+              double wn = from_next_lane(cur, lane_value(nxt, 0));
+              const double wp = from_prev_lane(cur, prev_last);
+              prev_last = lane_value(cur, 63);
+              if (m == len - 1) wn = 0.0;
               if (m < len) {
-                double d;
-                if (m == 0) d = -wn / 2.0;
-                else if (m == len - 1) d = wp / 2.0;
-                else d = -(wn - wp) / 2.0;
+                const double d = -(wn - wp) / 2.0;
                 const double xv =
                     s0 == 0 ? ypre0 : (s0 == 64 ? ypre1 : y_at(basic + m - 1));
                 L.xwd[2 * (m - c0)] = xv * cur;
@@ -1201,21 +1222,42 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
             if (lane == 0) wsamp += (unsigned long long)(cend - c0);
             wave_sync();
           }
+          // Lane sub takes staged samples sub + 8 r: `full` of them on every
+          // lane, one more on lanes sub < rem (uniform trip counts).
+          const int full = (cend - c0) / kSplit, rem = (cend - c0) % kSplit;
 #pragma unroll
           for (int qq = 0; qq < kPer; ++qq) {
-            if (!act[qq]) continue;
             const double c2 = 2.0 * gc8[qq];
             double a1 = p1[qq], a2 = p2[qq], b1 = q1[qq], b2 = q2[qq];
+            const LDS double* px = L.xwd + 2 * sub;
+            int r = 0;
 #pragma unroll 1
-            for (int m = c0 + sub; m < cend; m += kSplit) {
-              const double xm = L.xwd[2 * (m - c0)], xd = L.xwd[2 * (m - c0) + 1];
-              const double a0 = fma(c2, a1, xm) - a2;
-              const double b0 = fma(c2, b1, xd) - b2;
+            for (; r + 4 <= full; r += 4, px += 8 * kSplit) {  // four steps, registers alternate
+              double xs[8];
+#pragma unroll
+              for (int v = 0; v < 4; ++v) {
+                xs[2 * v] = px[2 * kSplit * v];
+                xs[2 * v + 1] = px[2 * kSplit * v + 1];
+              }
+              a2 = fma(c2, a1, xs[0]) - a2;
+              b2 = fma(c2, b1, xs[1]) - b2;
+              a1 = fma(c2, a2, xs[2]) - a1;
+              b1 = fma(c2, b2, xs[3]) - b1;
+              a2 = fma(c2, a1, xs[4]) - a2;
+              b2 = fma(c2, b1, xs[5]) - b2;
+              a1 = fma(c2, a2, xs[6]) - a1;
+              b1 = fma(c2, b2, xs[7]) - b1;
+            }
+            auto step = [&](const LDS double* q) {
+              const double a0 = fma(c2, a1, q[0]) - a2;
+              const double b0 = fma(c2, b1, q[1]) - b2;
               a2 = a1;
               a1 = a0;
               b2 = b1;
               b1 = b0;
-            }
+            };
+            for (; r < full; ++r, px += 2 * kSplit) step(px);
+            if (sub < rem) step(px);
             p1[qq] = a1;
             p2[qq] = a2;
             q1[qq] = b1;
@@ -1322,8 +1364,8 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
       // is pending, then slot by slot.
       flush(wflush, w);
       for (int p = b; p < e; ++p) {
-        const int s = L.wslots[p];
-        const int H = L.nh[s];
+        const int s = __builtin_amdgcn_readfirstlane(L.wslots[p]);
+        const int H = __builtin_amdgcn_readfirstlane(L.nh[s]);
         if (lane < H) {
           L.bink[lane] = mround(L.q[s] * (lane + 1));
           L.jidx[6 * s + lane] = (unsigned char)lane;
