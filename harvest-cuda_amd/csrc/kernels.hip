@@ -846,6 +846,7 @@ constexpr int kFrameBins = 128;  // bin results of one frame kept in LDS (< 256)
 constexpr int kStage = 128;      // windowed samples staged per pass (multiple of 64)
 constexpr int kMaxWords = 128;   // bitmap words (bins 0..8191)
 constexpr int kXcds = 8;         // XCDs that workgroups are dispatched to in turn
+constexpr int kYPre = 6;         // 64-sample chunks of the next window loaded ahead
 
 // LDS-qualified pointers: accesses through them compile to ds_* operations
 // however the pointers are passed around.
@@ -864,17 +865,18 @@ constexpr int kXcds = 8;         // XCDs that workgroups are dispatched to in tu
 //   rank   [S]          position of the slot in its window's list
 //   wslots [S]          slots grouped by window
 //   win_h  [S]          half length of each window
+//   pk     [6 S]        bin of each (grouped slot, harmonic), -1 past nh
 //   hkey   [HS]         hashed half lengths (-1: empty), later slot counts
 //   hval   [HS]         window of each hashed half length, later list starts
 //   bink   [kFrameBins] bins whose results are in res
-//   jidx   [6 S]        result index of each (slot, harmonic)
+//   jidx   [6 S]        result index of each (grouped slot, harmonic)
 //   misc   [4]          counters
 #define WAVE_LDS_FIELDS(X)                                                   \
   X(double, f0, S) X(double, q, S) X(double, res, 2 * kFrameBins)            \
   X(double, xwd, 2 * kStage) X(unsigned long long, bits, W) X(int, wpre, W)  \
   X(int, h, S) X(int, nh, S) X(int, win, S) X(int, rank, S)                  \
   X(int, wslots, S) X(int, win_h, S) X(int, hkey, HS) X(int, hval, HS)       \
-  X(int, bink, kFrameBins) X(unsigned char, jidx, 6 * S) X(int, misc, 4)
+  X(int, bink, kFrameBins) X(short, pk, 6 * S) X(unsigned char, jidx, 6 * S) X(int, misc, 4)
 
 struct WaveLds {
 #define X(T, name, count) LDS T* name;
@@ -1077,14 +1079,14 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
     if (w < 64) return __builtin_amdgcn_readlane(pw_basic, w);
     return mround((t + (-__builtin_amdgcn_readfirstlane(L.win_h[w])) / fs) * fs + 0.001);
   };
-  // The first 128 windowed samples of the next window are loaded ahead into
+  // The first 64 kYPre samples of the next window are loaded ahead into
   // registers, overlapping their latency with the current window's work.
-  double ypre0 = 0.0, ypre1 = 0.0;
+  double ypre[kYPre];
   auto prefetch = [&](int w) {
     if (w >= nwin) return;
     const int bs = window_basic(w), len = 2 * __builtin_amdgcn_readfirstlane(L.win_h[w]) + 1;
-    ypre0 = lane < len ? y_at(bs + lane - 1) : 0.0;
-    ypre1 = lane + 64 < len ? y_at(bs + lane + 63) : 0.0;
+#pragma unroll
+    for (int q = 0; q < kYPre; ++q) ypre[q] = lane + 64 * q < len ? y_at(bs + lane + 64 * q - 1) : 0.0;
   };
   prefetch(0);
   wave_sync();
@@ -1109,19 +1111,27 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
   }
   wave_sync();
   for (int s = lane; s < ncand; s += 64)
-    if (L.win[s] >= 0) L.wslots[wbeg[L.win[s]] + L.rank[s]] = s;
+    if (L.win[s] >= 0) {
+      const int pos = wbeg[L.win[s]] + L.rank[s];
+      const int H = L.nh[s];
+      const double qv = L.q[s];
+      L.wslots[pos] = s;
+#pragma unroll
+      for (int hh = 0; hh < 6; ++hh) L.pk[6 * pos + hh] = (short)(hh < H ? mround(qv * (hh + 1)) : -1);
+    }
   wave_sync();
 
-  // FixF0 (harvest.cpp:507-536) and GetRefinedF0's gate (:610-614) of slot s
-  // from its harmonics' results res[jidx].
-  auto finish = [&](int s) {
-    const double f0 = L.f0[s], qv = L.q[s];
+  // FixF0 (harvest.cpp:507-536) and GetRefinedF0's gate (:610-614) of the
+  // slot at grouped position p from its harmonics' results res[jidx].
+  auto finish = [&](int p) {
+    const int s = L.wslots[p];
+    const double f0 = L.f0[s];
     const int H = L.nh[s];
     const double inv_n = 1.0 / window_fft(L.h[s]);  // exact: a power of two
     double numer = 0.0, den = 0.0, dev = 0.0;
     for (int hh = 0; hh < H; ++hh) {
-      const int k = mround(qv * (hh + 1));
-      const int j = L.jidx[6 * s + hh];
+      const int k = L.pk[6 * p + hh];
+      const int j = L.jidx[6 * p + hh];
       const double pw = L.res[2 * j], nu = L.res[2 * j + 1];
       const double ifr = pw == 0.0 ? 0.0 : (double)k * fs * inv_n + nu / pw * fs * 0.5 / kPi;
       const double amp = sqrt(pw);
@@ -1142,7 +1152,7 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
   auto flush = [&](int w0, int w1) {
     const int b = __builtin_amdgcn_readfirstlane(wbeg[w0]);
     const int e = __builtin_amdgcn_readfirstlane(wbeg[w1]);
-    for (int p = b + lane; p < e; p += 64) finish(L.wslots[p]);
+    for (int p = b + lane; p < e; p += 64) finish(p);
     wave_sync();
   };
 
@@ -1198,7 +1208,9 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
         for (int c0 = 0; c0 < len; c0 += kStage) {
           const int cend = min(c0 + kStage, len);
           if (fill) {
-            for (int s0 = c0; s0 < cend; s0 += 64) {
+            // Samples s0 + lane of one 64-sample chunk; xv is y there when
+            // `have` (a prefetched chunk), else it is loaded.
+            auto chunk = [&](int s0, double xv, bool have) {
               const int m = s0 + lane;
               const double nc = fma(wc, gc, -(ws * gs)), ns = fma(ws, gc, wc * gs);
               wc = nc;
@@ -1212,13 +1224,16 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
               if (m == len - 1) wn = 0.0;
               if (m < len) {
                 const double d = -(wn - wp) / 2.0;
-                const double xv =
-                    s0 == 0 ? ypre0 : (s0 == 64 ? ypre1 : y_at(basic + m - 1));
+                if (!have) xv = y_at(basic + m - 1);
                 L.xwd[2 * (m - c0)] = xv * cur;
                 L.xwd[2 * (m - c0) + 1] = xv * d;
               }
               cur = nxt;
-            }
+            };
+#pragma unroll
+            for (int q = 0; q < kYPre; ++q)
+              if (64 * q >= c0 && 64 * q < cend) chunk(64 * q, ypre[q], true);
+            for (int s0 = max(c0, 64 * kYPre); s0 < cend; s0 += 64) chunk(s0, 0.0, false);
             if (lane == 0) wsamp += (unsigned long long)(cend - c0);
             wave_sync();
           }
@@ -1294,19 +1309,44 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
       wave_sync();
     };
 
-    // Distinct bins of the window (bins 0..n/2), one (slot, harmonic) pair
-    // per lane.
+    // Distinct bins of the window (bins 0..n/2) and the result index of each
+    // (slot, harmonic) pair.  Up to 64 pairs sit one per lane and are matched
+    // by ballots, one round per distinct bin; more pairs mark an LDS bitmap
+    // whose popcounts rank the bins.  A bin's value does not depend on its
+    // index.
+    if (pairs <= 64) {
+      const int k = lane < pairs ? (int)L.pk[6 * b + lane] : -1;
+      unsigned long long left = __ballot(k >= 0);
+      int nb = 0, myj = 0, mybin = 0;
+      while (left) {
+        const int kl = __builtin_amdgcn_readlane(k, __builtin_ctzll(left));
+        const unsigned long long same = __ballot(k == kl);
+        if (k == kl) myj = nb;
+        if (lane == nb) mybin = kl;
+        left &= ~same;
+        ++nb;
+      }
+      if (rbase + nb > kFrameBins) {  // results buffer full: finish what is pending
+        flush(wflush, w);
+        wflush = w;
+        rbase = 0;
+      }
+      if (lane < nb) L.bink[rbase + lane] = mybin;
+      if (k >= 0) L.jidx[6 * b + lane] = (unsigned char)(rbase + myj);
+      wave_sync();
+      dft_bins(rbase, nb);
+      prefetch(w + 1);
+      rbase += nb;
+      continue;
+    }
     const int words = (n / 2 + 64) >> 6;
     int nb = kFrameBins + 1;
     if (W > 0) {
       for (int k = lane; k < words; k += 64) L.bits[k] = 0ull;
       wave_sync();
       for (int p = lane; p < pairs; p += 64) {
-        const int s = L.wslots[b + p / 6], hh = p % 6;
-        if (hh < L.nh[s]) {
-          const int k = mround(L.q[s] * (hh + 1));
-          lds_or(L.bits + (k >> 6), 1ull << (k & 63));
-        }
+        const int k = L.pk[6 * b + p];
+        if (k >= 0) lds_or(L.bits + (k >> 6), 1ull << (k & 63));
       }
       wave_sync();
       if (words <= 64) {
@@ -1332,7 +1372,7 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
     }
     wave_sync();
     if (nb <= kFrameBins) {
-      if (rbase + nb > kFrameBins) {  // results buffer full: finish what is pending
+      if (rbase + nb > kFrameBins) {
         flush(wflush, w);
         wflush = w;
         rbase = 0;
@@ -1345,20 +1385,18 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
           bb &= bb - 1;
         }
       }
+      for (int p = lane; p < pairs; p += 64) {
+        const int k = L.pk[6 * b + p];
+        if (k >= 0) {
+          const int wd = k >> 6;
+          L.jidx[6 * b + p] = (unsigned char)(rbase + L.wpre[wd] +
+                                              __popcll(L.bits[wd] & ((1ull << (k & 63)) - 1ull)));
+        }
+      }
       wave_sync();
       dft_bins(rbase, nb);
       prefetch(w + 1);
-      for (int p = lane; p < pairs; p += 64) {
-        const int s = L.wslots[b + p / 6], hh = p % 6;
-        if (hh < L.nh[s]) {
-          const int k = mround(L.q[s] * (hh + 1));
-          const int wd = k >> 6;
-          L.jidx[6 * s + hh] = (unsigned char)(rbase + L.wpre[wd] +
-                                               __popcll(L.bits[wd] & ((1ull << (k & 63)) - 1ull)));
-        }
-      }
       rbase += nb;
-      wave_sync();
     } else {
       // More distinct bins than the buffer holds (or no bitmap): finish what
       // is pending, then slot by slot.
@@ -1367,12 +1405,12 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
         const int s = __builtin_amdgcn_readfirstlane(L.wslots[p]);
         const int H = __builtin_amdgcn_readfirstlane(L.nh[s]);
         if (lane < H) {
-          L.bink[lane] = mround(L.q[s] * (lane + 1));
-          L.jidx[6 * s + lane] = (unsigned char)lane;
+          L.bink[lane] = L.pk[6 * p + lane];
+          L.jidx[6 * p + lane] = (unsigned char)lane;
         }
         wave_sync();
         dft_bins(0, H);
-        if (lane == 0) finish(s);
+        if (lane == 0) finish(p);
         wave_sync();
       }
       prefetch(w + 1);
