@@ -181,7 +181,7 @@ struct hv_workspace {
   DevBuf utt, c_off, t_off, f_off, p_off, y_off, x_off;
   DevBuf cinit;                                        // initial f0 of each CSR entry
   DevBuf full, gy, gx, ent_frame, ent_f0, ent_w, ent_out;  // frozen-decision adjoint
-  DevBuf coef, coef_off, half, bounds, twiddle, work;
+  DevBuf coef, coef_off, half, bounds, twiddle, wintab, work;
   PinnedBuf h_x, h_meta, h_offsets, h_best, h_cand, h_work, h_ent, h_gx;
   // Filter-bank cache key (working rate, floor, ceil).
   double key_fs = -1.0, key_floor = -1.0, key_ceil = -1.0;
@@ -189,6 +189,7 @@ struct hv_workspace {
   std::vector<int> halves;
   int max_half = 0;
   int twiddle_n = -1;
+  int wintab_h = -1;  // largest half window the window table covers
   long long last_nnz = -1;
   long long last_frames = 0;
   int lds_limit = 64 * 1024;
@@ -315,7 +316,20 @@ void prepare_filters(hv_workspace* ws, double fs_work, double f0_floor,
   ws->max_half = max_half;
 }
 
+// Largest refinement half window: candidates are means of values in
+// [f0_floor, f0_ceil]; the 1e-9 slack covers their rounding, the + 1 the
+// truncation of harvest.cpp:598.
+int refine_hmax(double fs_work, double f0_floor) {
+  return (int)(1.5 * fs_work / (f0_floor * (1.0 - 1e-9)) + 1.0) + 1;
+}
+
 void prepare_twiddles(hv_workspace* ws, double fs_work, double f0_floor) {
+  const int wh = refine_hmax(fs_work, f0_floor);
+  if (wh > ws->wintab_h) {
+    double* tab = ws->wintab.ensure<double>(2 * 65 * ((size_t)wh + 1));
+    launch_window_table(ws->stream, tab, wh);
+    ws->wintab_h = wh;
+  }
   // Largest refinement window: candidates are >= f0_floor (harvest.cpp:250-252).
   const double hmax = 1.5 * fs_work / (f0_floor * (1.0 - 1e-9)) + 2.0;
   int need = 0;
@@ -349,9 +363,7 @@ void fill_config(Config& c, int count, int ratio, int fs, double fs_work, double
   c.slots = 7 * capacity;
   c.twiddle_n = twiddle_n;
   c.lds_limit = 64 * 1024;
-  // Candidates are means of values in [f0_floor, f0_ceil]; the 1e-9 slack
-  // covers their rounding, the + 1 the truncation of harvest.cpp:598.
-  c.hmax = (int)(1.5 * fs_work / (f0_floor * (1.0 - 1e-9)) + 1.0) + 1;
+  c.hmax = refine_hmax(fs_work, f0_floor);
   const double* co = kDecimation[ratio >= 2 && ratio <= 12 ? ratio : 0];
   c.a0 = co[0];
   c.a1 = co[1];
@@ -496,7 +508,7 @@ void analyze_device(hv_workspace* ws, const Plan& p, int32_t* offsets, double* b
   launch_fill_int(s, reinterpret_cast<int*>(work), 2 * kWorkWords, 0);
   timed(ws, 6, [&] {
     launch_refine(s, c, dutt, ws->f_off.get<long long>(), p.total_frames, y, official, count,
-                  ws->twiddle.get<double>(), rf0, rsc, work);
+                  ws->twiddle.get<double>(), ws->wintab.get<double>(), rf0, rsc, work);
   });
   double* pf0 = ws->pf0.ensure<double>(slots);
   double* psc = ws->psc.ensure<double>(slots);

@@ -31,6 +31,8 @@ void launch_quirk(hipStream_t s, const Config& c, const Utt* utt,
 void launch_fill_int(hipStream_t s, int* p, long long n, int value);
 // Refinement twiddles (cos, sin) of 2 pi q / n for q < n.
 void launch_twiddle(hipStream_t s, double* tw, int n);
+// (cos, sin) of 2 pi m / (2h + 1) for h <= hmax, m <= 64, 65 (h + 1) entries.
+void launch_window_table(hipStream_t s, double* tab, int hmax);
 
 // Filterbank + crossing detection for bands [band0, band0 + group).
 void launch_filter_events(hipStream_t s, const Config& c, const Utt* utt,
@@ -54,7 +56,7 @@ void launch_official(hipStream_t s, const Config& c, const long long* f_off,
 void launch_refine(hipStream_t s, const Config& c, const Utt* utt,
                    const long long* f_off, long long total_frames,
                    const double* y, const double* official, const int* count,
-                   const double* twiddle, double* rf0, double* rscore,
+                   const double* twiddle, const double* wintab, double* rf0, double* rscore,
                    unsigned long long* work);
 void launch_prune_best(hipStream_t s, const Config& c, const Utt* utt,
                        const long long* f_off, long long total_frames,

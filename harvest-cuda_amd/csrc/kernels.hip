@@ -444,6 +444,18 @@ __global__ __launch_bounds__(kThreads) void k_twiddle(double* tw, int n) {
   tw[2 * q + 1] = sn;
 }
 
+// Window rotations: (cos, sin) of 2 pi m / (2h + 1) at tab[2 (65 h + m)] for
+// h <= hmax and m <= 64 (sincospi of the exact rational argument 2m / len).
+__global__ __launch_bounds__(kThreads) void k_window_table(double* tab, int hmax) {
+  const long long id = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (id >= 65LL * (hmax + 1)) return;
+  const int h = (int)(id / 65), m = (int)(id % 65);
+  double sn, cs;
+  sincospi(2.0 * m / (2 * h + 1), &sn, &cs);
+  tab[2 * id] = cs;
+  tab[2 * id + 1] = sn;
+}
+
 __global__ __launch_bounds__(kThreads) void k_fill_int(int* p, long long n, int v) {
   long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
   if (i < n) p[i] = v;
@@ -995,7 +1007,7 @@ __device__ __forceinline__ void twiddle_at(const double* tw, int stride, int n, 
 __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
     Config c, const Utt* utt, const long long* f_off, long long total_frames, int HS, int W,
     const double* y, const double* official, const int* count, const double* twiddle,
-    double* rf0, double* rscore, unsigned long long* work) {
+    const double* wintab, double* rf0, double* rscore, unsigned long long* work) {
   extern __shared__ double lds_raw[];
   const int S = c.slots;
   // The wave index is uniform: readfirstlane lets the compiler keep the
@@ -1065,15 +1077,20 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
   const int nwin = __builtin_amdgcn_readfirstlane(L.misc[0]);
   for (int s = lane; s < ncand; s += 64) L.win[s] = L.win[s] >= 0 ? L.hval[L.win[s]] : -1;
   // Per-window constants, lane w holding those of window w < 64: the first
-  // sample index (GetBaseIndex), the Blackman phase offset a and the rotation
-  // by 64 samples; windows from 64 on compute them in the loop.
+  // sample index (GetBaseIndex), e^{2 pi i a / len} of the Blackman phase
+  // offset a, and the rotation by 64 samples; windows from 64 on compute them
+  // in the loop.
+  const double2* wrot = reinterpret_cast<const double2*>(wintab);
   int pw_basic = 0;
-  double pw_a = 0.0, pw_gc = 1.0, pw_gs = 0.0;
+  double pw_ac = 1.0, pw_as = 0.0, pw_gc = 1.0, pw_gs = 0.0;
   if (lane < nwin) {
     const int hw = L.win_h[lane];
     pw_basic = mround((t + (-hw) / fs) * fs + 0.001);
-    pw_a = (double)(pw_basic - 1) - t * fs;
-    sincospi(128.0 / (2 * hw + 1), &pw_gs, &pw_gc);
+    const double a = (double)(pw_basic - 1) - t * fs;
+    sincospi(2.0 * a / (2 * hw + 1), &pw_as, &pw_ac);
+    const double2 g = wrot[65 * hw + 64];
+    pw_gc = g.x;
+    pw_gs = g.y;
   }
   auto window_basic = [&](int w) -> int {
     if (w < 64) return __builtin_amdgcn_readlane(pw_basic, w);
@@ -1081,12 +1098,16 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
   };
   // The first 64 kYPre samples of the next window are loaded ahead into
   // registers, overlapping their latency with the current window's work.
+  // With them, e^{2 pi i lane / len} of the next window.
   double ypre[kYPre];
+  double2 tpre = make_double2(1.0, 0.0);
   auto prefetch = [&](int w) {
     if (w >= nwin) return;
-    const int bs = window_basic(w), len = 2 * __builtin_amdgcn_readfirstlane(L.win_h[w]) + 1;
+    const int hw = __builtin_amdgcn_readfirstlane(L.win_h[w]);
+    const int bs = window_basic(w), len = 2 * hw + 1;
 #pragma unroll
     for (int q = 0; q < kYPre; ++q) ypre[q] = lane + 64 * q < len ? y_at(bs + lane + 64 * q - 1) : 0.0;
+    tpre = wrot[65 * hw + lane];
   };
   prefetch(0);
   wave_sync();
@@ -1166,19 +1187,23 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
     // Blackman argument (harvest.cpp:446-456) in units of pi:
     // 2 tmp / wl with tmp = (basic + m - 1) / fs - t and wl = len / fs.
     int basic;
-    double a, gc, gs;
+    double ac, as, gc, gs;
     if (w < 64) {
       basic = __builtin_amdgcn_readlane(pw_basic, w);
-      a = lane_value(pw_a, w);
+      ac = lane_value(pw_ac, w);
+      as = lane_value(pw_as, w);
       gc = lane_value(pw_gc, w);
       gs = lane_value(pw_gs, w);
     } else {
       basic = window_basic(w);
-      a = (double)(basic - 1) - t * fs;
-      sincospi(128.0 / len, &gs, &gc);  // advance by 64 samples
+      const double a = (double)(basic - 1) - t * fs;
+      sincospi(2.0 * a / len, &as, &ac);
+      const double2 g = wrot[65 * hw + 64];  // advance by 64 samples
+      gc = g.x;
+      gs = g.y;
     }
-    double wc0, ws0;
-    sincospi(2.0 * (a + lane) / len, &ws0, &wc0);  // sample m = lane
+    // Sample m = lane: e^{2 pi i (a + m) / len} = e^{2 pi i a / len} e^{2 pi i m / len}.
+    const double wc0 = fma(ac, tpre.x, -(as * tpre.y)), ws0 = fma(as, tpre.x, ac * tpre.y);
     const int nstage = (len + kStage - 1) / kStage;
     bool staged = false;
 
@@ -1771,6 +1796,11 @@ void launch_twiddle(hipStream_t s, double* tw, int n) {
   check_launch("twiddle");
 }
 
+void launch_window_table(hipStream_t s, double* tab, int hmax) {
+  k_window_table<<<grid_for(65LL * (hmax + 1)), kThreads, 0, s>>>(tab, hmax);
+  check_launch("window_table");
+}
+
 void launch_fill_int(hipStream_t s, int* p, long long n, int value) {
   if (n <= 0) return;
   k_fill_int<<<grid_for(n), kThreads, 0, s>>>(p, n, value);
@@ -1832,7 +1862,7 @@ void launch_official(hipStream_t s, const Config& c, const long long* f_off,
 void launch_refine(hipStream_t s, const Config& c, const Utt* utt,
                    const long long* f_off, long long total_frames,
                    const double* y, const double* official, const int* count,
-                   const double* twiddle, double* rf0, double* rscore,
+                   const double* twiddle, const double* wintab, double* rf0, double* rscore,
                    unsigned long long* work) {
   int HS = 64;  // window hash: a power of two >= 2 slots keeps probing short
   while (HS < 2 * c.slots) HS <<= 1;
@@ -1845,7 +1875,7 @@ void launch_refine(hipStream_t s, const Config& c, const Utt* utt,
                              std::to_string(c.lds_limit));
   const unsigned grid = (unsigned)((total_frames + kWaves - 1) / kWaves);
   k_refine_waves<<<grid, kThreads, bytes, s>>>(c, utt, f_off, total_frames, HS, W, y, official,
-                                               count, twiddle, rf0, rscore, work);
+                                               count, twiddle, wintab, rf0, rscore, work);
   check_launch("refine");
 }
 
