@@ -178,7 +178,12 @@ struct hv_workspace {
   DevBuf x, pass, states, carries, y, partial, mean, partial3, sums3, quirk;
   DevBuf slab, counts, offs, totals, raw, official, count;
   DevBuf rf0, rsc, pf0, psc, best, row_nnz, offsets, scan_tmp;
-  DevBuf utt, c_off, t_off, f_off, p_off, y_off, x_off;
+  // The plan (utterances and six offset tables) in one device buffer, uploaded
+  // with a single copy, and typed views into it.
+  DevBuf plan;
+  const Utt* d_utt = nullptr;
+  const long long *d_c_off = nullptr, *d_t_off = nullptr, *d_f_off = nullptr;
+  const long long *d_p_off = nullptr, *d_y_off = nullptr, *d_x_off = nullptr;
   DevBuf cinit;                                        // initial f0 of each CSR entry
   DevBuf full, gy, gx, ent_frame, ent_f0, ent_w, ent_out;  // frozen-decision adjoint
   DevBuf coef, coef_off, half, bounds, twiddle, wintab, work;
@@ -440,16 +445,16 @@ void upload_plan(hv_workspace* ws, const Plan& p) {
   size_t at = U * sizeof(Utt);
   const std::vector<long long>* arrays[kArrays] = {&p.c_off, &p.t_off, &p.f_off,
                                                    &p.p_off, &p.y_off, &p.x_off};
-  DevBuf* dst[kArrays] = {&ws->c_off, &ws->t_off, &ws->f_off,
-                          &ws->p_off, &ws->y_off, &ws->x_off};
-  check(hipMemcpyAsync(ws->utt.ensure<Utt>(U), h, U * sizeof(Utt), hipMemcpyHostToDevice,
-                       ws->stream), "upload plan");
+  const long long** views[kArrays] = {&ws->d_c_off, &ws->d_t_off, &ws->d_f_off,
+                                      &ws->d_p_off, &ws->d_y_off, &ws->d_x_off};
+  char* d = ws->plan.ensure<char>(bytes);
+  ws->d_utt = reinterpret_cast<const Utt*>(d);
   for (int k = 0; k < kArrays; ++k) {
     std::memcpy(h + at, arrays[k]->data(), (U + 1) * sizeof(long long));
-    check(hipMemcpyAsync(dst[k]->ensure<long long>(U + 1), h + at, (U + 1) * sizeof(long long),
-                         hipMemcpyHostToDevice, ws->stream), "upload plan");
+    *views[k] = reinterpret_cast<const long long*>(d + at);
     at += (U + 1) * sizeof(long long);
   }
+  check(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ws->stream), "upload plan");
 }
 
 // Everything after the decimated, mean-removed y is on the device.
@@ -458,14 +463,14 @@ void analyze_device(hv_workspace* ws, const Plan& p, int32_t* offsets, double* b
   const Config& c = p.cfg;
   const int U = c.count, B = c.bands;
   hipStream_t s = ws->stream;
-  const Utt* dutt = ws->utt.get<Utt>();
+  const Utt* dutt = ws->d_utt;
   double* y = ws->y.get<double>();
   const double* coef = ws->coef.get<double>();
   const int* coef_off = ws->coef_off.get<int>();
   const int* half = ws->half.get<int>();
 
   Quirk* quirk = ws->quirk.ensure<Quirk>((size_t)U * B);
-  launch_quirk(s, c, dutt, ws->p_off.get<long long>(), p.total_pblocks, y, coef, coef_off, half,
+  launch_quirk(s, c, dutt, ws->d_p_off, p.total_pblocks, y, coef, coef_off, half,
                ws->partial3.ensure<double>(3 * p.total_pblocks), ws->sums3.ensure<double>(3 * U),
                quirk);
 
@@ -482,19 +487,19 @@ void analyze_device(hv_workspace* ws, const Plan& p, int32_t* offsets, double* b
     int mh = 0;
     for (int b = b0; b < b0 + g; ++b) mh = std::max(mh, ws->halves[b]);
     timed(ws, 4, [&] {
-      launch_filter_events(s, c, dutt, ws->t_off.get<long long>(), p.total_tiles, b0, g, mh, y,
+      launch_filter_events(s, c, dutt, ws->d_t_off, p.total_tiles, b0, g, mh, y,
                            coef, coef_off, half, quirk, slab, counts);
     });
     timed(ws, 5, [&] {
       launch_tile_scan(s, c, dutt, p.total_tiles, g, counts, offs, totals);
-      launch_interp(s, c, dutt, ws->t_off.get<long long>(), p.total_tiles, p.total_frames, b0, g,
+      launch_interp(s, c, dutt, ws->d_t_off, p.total_tiles, p.total_frames, b0, g,
                     ws->bounds.get<double>(), slab, counts, offs, totals, raw);
     });
   }
   int* count = ws->count.ensure<int>(U);
   launch_fill_int(s, count, U, 0);
   double* official = ws->official.ensure<double>((size_t)p.total_frames * c.capacity);
-  launch_official(s, c, ws->f_off.get<long long>(), p.total_frames, raw, official, count);
+  launch_official(s, c, ws->d_f_off, p.total_frames, raw, official, count);
   check(hipEventRecord(ws->ev[2], s), "hipEventRecord");
 
   const size_t slots = (size_t)p.total_frames * c.slots;
@@ -507,7 +512,7 @@ void analyze_device(hv_workspace* ws, const Plan& p, int32_t* offsets, double* b
   unsigned long long* work = ws->work.ensure<unsigned long long>(kWorkWords);
   launch_fill_int(s, reinterpret_cast<int*>(work), 2 * kWorkWords, 0);
   timed(ws, 6, [&] {
-    launch_refine(s, c, dutt, ws->f_off.get<long long>(), p.total_frames, y, official, count,
+    launch_refine(s, c, dutt, ws->d_f_off, p.total_frames, y, official, count,
                   ws->twiddle.get<double>(), ws->wintab.get<double>(), rf0, rsc, work);
   });
   double* pf0 = ws->pf0.ensure<double>(slots);
@@ -518,11 +523,11 @@ void analyze_device(hv_workspace* ws, const Plan& p, int32_t* offsets, double* b
   const long long nb = scan_block_count(p.total_frames);
   int* scan_tmp = ws->scan_tmp.ensure<int>(nb + 1);
   timed(ws, 7, [&] {
-    launch_prune_best(s, c, dutt, ws->f_off.get<long long>(), p.total_frames, count, rf0, rsc,
+    launch_prune_best(s, c, dutt, ws->d_f_off, p.total_frames, count, rf0, rsc,
                       pf0, psc, dbest, row_nnz);
     launch_exclusive_scan(s, row_nnz, doffsets, p.total_frames, scan_tmp, nb + 1);
     // Compacted candidates reuse the refinement buffers (no longer needed).
-    launch_compact(s, c, dutt, ws->f_off.get<long long>(), p.total_frames, count, official,
+    launch_compact(s, c, dutt, ws->d_f_off, p.total_frames, count, official,
                    pf0, psc, doffsets, rf0, rsc, ws->cinit.ensure<double>(slots));
   });
   int32_t* hoff = ws->h_offsets.ensure<int32_t>(p.total_frames + 1);
@@ -654,14 +659,14 @@ Plan preprocess_batch(hv_workspace* ws, int count, const double* const* xs,
   double* carries = ws->carries.ensure<double>(3 * p.total_chunks);
   timed(ws, 3, [&] {
     if (ratio > 1)
-      launch_decimate(s, p.cfg, ws->utt.get<Utt>(), ws->c_off.get<long long>(), p.total_chunks,
+      launch_decimate(s, p.cfg, ws->d_utt, ws->d_c_off, p.total_chunks,
                       dx, pass, states, carries, y);
     else
-      launch_copy_input(s, p.cfg, ws->utt.get<Utt>(), ws->y_off.get<long long>(), p.total_y, dx,
+      launch_copy_input(s, p.cfg, ws->d_utt, ws->d_y_off, p.total_y, dx,
                         y);
   });
-  launch_remove_mean(s, p.cfg, ws->utt.get<Utt>(), ws->p_off.get<long long>(), p.total_pblocks,
-                     ws->y_off.get<long long>(), p.total_y, y,
+  launch_remove_mean(s, p.cfg, ws->d_utt, ws->d_p_off, p.total_pblocks,
+                     ws->d_y_off, p.total_y, y,
                      ws->partial.ensure<double>(p.total_pblocks), ws->mean.ensure<double>(count));
   check(hipEventRecord(ws->ev[1], s), "hipEventRecord");
   return p;
@@ -691,7 +696,7 @@ void refine_entries(hv_workspace* ws, int count, const double* const* xs,
   const Plan p = preprocess_batch(ws, count, xs, lengths, fs, f0_floor, f0_ceil, nullptr, false);
   const Config& c = p.cfg;
   hipStream_t s = ws->stream;
-  const Utt* dutt = ws->utt.get<Utt>();
+  const Utt* dutt = ws->d_utt;
   for (int64_t e = 0; e < n_entries; ++e) {
     if (frame[e] < 0 || frame[e] >= p.total_frames)
       invalid("entry frame " + std::to_string(frame[e]) + " is outside the batch's frames");
@@ -723,20 +728,20 @@ void refine_entries(hv_workspace* ws, int count, const double* const* xs,
     gy = ws->gy.ensure<double>(p.total_y);
     launch_fill_int(s, reinterpret_cast<int*>(gy), 2 * p.total_y, 0);
   }
-  launch_refine_entries(s, c, dutt, ws->f_off.get<long long>(), n_entries, dframe, dinit,
+  launch_refine_entries(s, c, dutt, ws->d_f_off, n_entries, dframe, dinit,
                         grad_x ? dw : nullptr, ws->y.get<double>(), refined ? dout : nullptr, gy);
   if (refined && ne > 0) {
     check(hipMemcpyAsync(hw, dout, ne * sizeof(double), hipMemcpyDeviceToHost, s), "download refined");
   }
   if (grad_x) {
     // Mean removal is symmetric: y_bar minus its mean.
-    launch_remove_mean(s, c, dutt, ws->p_off.get<long long>(), p.total_pblocks,
-                       ws->y_off.get<long long>(), p.total_y, gy,
+    launch_remove_mean(s, c, dutt, ws->d_p_off, p.total_pblocks,
+                       ws->d_y_off, p.total_y, gy,
                        ws->partial.ensure<double>(p.total_pblocks), ws->mean.ensure<double>(count));
     double* gx = ws->gx.ensure<double>(p.total_x);
     if (c.ratio > 1) {
-      launch_decimate_transpose(s, c, dutt, ws->c_off.get<long long>(), p.total_chunks,
-                                ws->x_off.get<long long>(), p.total_x, gy,
+      launch_decimate_transpose(s, c, dutt, ws->d_c_off, p.total_chunks,
+                                ws->d_x_off, p.total_x, gy,
                                 ws->pass.ensure<double>(p.total_m),
                                 ws->states.ensure<double>(3 * p.total_chunks),
                                 ws->carries.ensure<double>(3 * p.total_chunks),
@@ -834,11 +839,11 @@ hv_status hv_decimate(hv_workspace* ws, const double* x, int64_t length, int rat
     check(hipMemcpyAsync(dx, hx, length * sizeof(double), hipMemcpyHostToDevice, s), "upload x");
     double* dy = ws->y.ensure<double>(y_length);
     if (ratio > 1)
-      launch_decimate(s, p.cfg, ws->utt.get<Utt>(), ws->c_off.get<long long>(), p.total_chunks, dx,
+      launch_decimate(s, p.cfg, ws->d_utt, ws->d_c_off, p.total_chunks, dx,
                       ws->pass.ensure<double>(p.total_m), ws->states.ensure<double>(3 * p.total_chunks),
                       ws->carries.ensure<double>(3 * p.total_chunks), dy);
     else
-      launch_copy_input(s, p.cfg, ws->utt.get<Utt>(), ws->y_off.get<long long>(), p.total_y, dx, dy);
+      launch_copy_input(s, p.cfg, ws->d_utt, ws->d_y_off, p.total_y, dx, dy);
     double* hy = ws->h_best.ensure<double>(y_length);
     check(hipMemcpyAsync(hy, dy, y_length * sizeof(double), hipMemcpyDeviceToHost, s), "download y");
     check(hipStreamSynchronize(s), "decimate");

@@ -165,14 +165,15 @@ __device__ __forceinline__ void mat3_apply(const double* m, const double* v,
 // One workgroup per utterance: each thread owns a run of K consecutive chunks,
 // runs are combined with a Kogge-Stone scan using (P^K)^(2^j), then every run
 // replays its chunks to emit the carry-in state of each chunk.
-__global__ __launch_bounds__(kThreads) void k_iir_carry(Config c, const Utt* utt,
-                                                       const double* states,
-                                                       double* carries) {
-  __shared__ double qpow[8][9];
-  __shared__ double scan[kThreads][3];
+constexpr int kCarryThreads = kThreads, kCarryLevels = 8;
+__global__ __launch_bounds__(kCarryThreads) void k_iir_carry(Config c, const Utt* utt,
+                                                            const double* states,
+                                                            double* carries) {
+  __shared__ double qpow[kCarryLevels][9];
+  __shared__ double scan[kCarryThreads][3];
   const Utt u = utt[blockIdx.x];
   const int C = u.chunks;
-  const int K = (C + kThreads - 1) / kThreads;
+  const int K = (C + kCarryThreads - 1) / kCarryThreads;
   const double* P = c.chunk_pow;
   if (threadIdx.x == 0) {
     double q[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, base[9], tmp[9];
@@ -186,7 +187,7 @@ __global__ __launch_bounds__(kThreads) void k_iir_carry(Config c, const Utt* utt
       for (int k = 0; k < 9; ++k) base[k] = tmp[k];
     }
     for (int k = 0; k < 9; ++k) qpow[0][k] = q[k];
-    for (int j = 1; j < 8; ++j) mat3_mul(qpow[j - 1], qpow[j - 1], qpow[j]);
+    for (int j = 1; j < kCarryLevels; ++j) mat3_mul(qpow[j - 1], qpow[j - 1], qpow[j]);
   }
   const long long first = (long long)threadIdx.x * K;
   const long long last = min(first + (long long)K, (long long)C);
@@ -207,7 +208,7 @@ __global__ __launch_bounds__(kThreads) void k_iir_carry(Config c, const Utt* utt
   scan[threadIdx.x][1] = agg[1];
   scan[threadIdx.x][2] = agg[2];
   __syncthreads();
-  for (int j = 0; j < 8; ++j) {
+  for (int j = 0; j < kCarryLevels; ++j) {
     int d = 1 << j;
     double nv[3] = {scan[threadIdx.x][0], scan[threadIdx.x][1], scan[threadIdx.x][2]};
     if ((int)threadIdx.x >= d) {
@@ -400,19 +401,21 @@ __global__ __launch_bounds__(kThreads) void k_quirk_final(Config c, const Utt* u
   }
 }
 
-// One thread per (utterance, band): filter spectrum at the two bins, direct
-// sums over the 2h+1 taps (coefficients are stored reversed: cr[mm] = c[2h-mm]).
+// One wavefront per (utterance, band): filter spectrum at the two bins, direct
+// sums over the 2h+1 taps (coefficients are stored reversed: cr[mm] = c[2h-mm]),
+// lane-strided and reduced by a fixed butterfly.
 __global__ __launch_bounds__(kThreads) void k_quirk_bands(
     Config c, const Utt* utt, const double* sums3, const double* coef,
     const int* coef_off, const int* half, Quirk* quirk) {
-  long long id = (long long)blockIdx.x * kThreads + threadIdx.x;
+  const long long id = (long long)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (id >= (long long)c.count * c.bands) return;
   int ui = (int)(id / c.bands), b = (int)(id % c.bands);
   const Utt u = utt[ui];
   const double* cr = coef + coef_off[b];
   int h = half[b];
   double hh = 0.0, hr = 0.0, hi = 0.0;
-  for (int m = 0; m <= 2 * h; ++m) {
+  for (int m = lane; m <= 2 * h; m += 64) {
     double v = cr[2 * h - m];
     if (m & 1) v = -v;
     double sn, cs;
@@ -421,6 +424,13 @@ __global__ __launch_bounds__(kThreads) void k_quirk_bands(
     hr = fma(v, cs, hr);
     hi = fma(v, sn, hi);
   }
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    hh += __shfl_xor(hh, d, 64);
+    hr += __shfl_xor(hr, d, 64);
+    hi += __shfl_xor(hi, d, 64);
+  }
+  if (lane != 0) return;
   double shalf = sums3[ui * 3 + 0], s1r = sums3[ui * 3 + 1], s1i = sums3[ui * 3 + 2];
   double p1r = s1r * hr - s1i * hi;
   double p1i = s1r * hi + s1i * hr;
@@ -1669,7 +1679,11 @@ __global__ __launch_bounds__(kThreads) void k_compact(Config c, const Utt* utt,
                                                      const double* pf0, const double* pscore,
                                                      const int* offsets, double* of0,
                                                      double* oscore, double* oinit) {
-  const long long gf = (long long)blockIdx.x * kThreads + threadIdx.x;
+  // One wavefront per frame: lanes take 64 slots at a time and the nonzero
+  // ones are packed in slot order by ballot ranks.
+  const long long gf = (long long)blockIdx.x * (kThreads / 64) +
+                       __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (gf >= total_frames) return;
   const int ui = find_segment(f_off, c.count, gf);
   const int cnt = count[ui];
@@ -1677,18 +1691,22 @@ __global__ __launch_bounds__(kThreads) void k_compact(Config c, const Utt* utt,
   const Utt u = utt[ui];
   const long long i = gf - u.f_off;
   int pos = offsets[gf];
-  for (int k = 0; k < ncand; ++k) {
-    const double f = pf0[gf * c.slots + k];
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int k0 = 0; k0 < ncand; k0 += 64) {
+    const int k = k0 + lane;
+    const double f = k < ncand ? pf0[gf * c.slots + k] : 0.0;
+    const unsigned long long nz = __ballot(f != 0.0);
     if (f != 0.0) {
-      of0[pos] = f;
-      oscore[pos] = pscore[gf * c.slots + k];
+      const int at = pos + __popcll(nz & below);
+      of0[at] = f;
+      oscore[at] = pscore[gf * c.slots + k];
       const int g = k / cnt, jj = k % cnt;
       long long src = i;
       if (g >= 1 && g <= 3) src = i - g;
       else if (g >= 4) src = i + (g - 3);
-      oinit[pos] = official[(u.f_off + src) * c.capacity + jj];
-      ++pos;
+      oinit[at] = official[(u.f_off + src) * c.capacity + jj];
     }
+    pos += __popcll(nz);
   }
 }
 
@@ -1752,12 +1770,12 @@ void launch_decimate(hipStream_t s, const Config& c, const Utt* utt,
   const unsigned grid = grid_for(total_chunks);
   k_iir_chunk_state<0><<<grid, kThreads, 0, s>>>(c, utt, c_off, total_chunks, x,
                                                   pass_buffer, states);
-  k_iir_carry<<<c.count, kThreads, 0, s>>>(c, utt, states, carries);
+  k_iir_carry<<<c.count, kCarryThreads, 0, s>>>(c, utt, states, carries);
   k_iir_replay<0><<<grid, kThreads, 0, s>>>(c, utt, c_off, total_chunks, x,
                                              pass_buffer, carries, y);
   k_iir_chunk_state<1><<<grid, kThreads, 0, s>>>(c, utt, c_off, total_chunks, x,
                                                   pass_buffer, states);
-  k_iir_carry<<<c.count, kThreads, 0, s>>>(c, utt, states, carries);
+  k_iir_carry<<<c.count, kCarryThreads, 0, s>>>(c, utt, states, carries);
   k_iir_replay<1><<<grid, kThreads, 0, s>>>(c, utt, c_off, total_chunks, x,
                                              pass_buffer, carries, y);
   check_launch("decimate");
@@ -1786,7 +1804,7 @@ void launch_quirk(hipStream_t s, const Config& c, const Utt* utt,
                   double* partial3, double* sums3, Quirk* quirk) {
   k_quirk_partial<<<(unsigned)total_blocks, kThreads, 0, s>>>(c, utt, p_off, y, partial3);
   k_quirk_final<<<c.count, kThreads, 0, s>>>(c, utt, partial3, sums3);
-  k_quirk_bands<<<grid_for((long long)c.count * c.bands), kThreads, 0, s>>>(
+  k_quirk_bands<<<grid_for((long long)c.count * c.bands * 64), kThreads, 0, s>>>(
       c, utt, sums3, coef, coef_off, half, quirk);
   check_launch("quirk");
 }
@@ -1914,9 +1932,9 @@ void launch_compact(hipStream_t s, const Config& c, const Utt* utt,
                     const int* count, const double* official, const double* pf0,
                     const double* pscore, const int* offsets, double* out_f0,
                     double* out_score, double* out_init) {
-  k_compact<<<grid_for(total_frames), kThreads, 0, s>>>(c, utt, f_off, total_frames, count,
-                                                       official, pf0, pscore, offsets, out_f0,
-                                                       out_score, out_init);
+  k_compact<<<grid_for(total_frames * 64), kThreads, 0, s>>>(c, utt, f_off, total_frames, count,
+                                                            official, pf0, pscore, offsets, out_f0,
+                                                            out_score, out_init);
   check_launch("compact");
 }
 
@@ -1928,12 +1946,12 @@ void launch_decimate_transpose(hipStream_t s, const Config& c, const Utt* utt,
   const unsigned grid = grid_for(total_chunks);
   k_iir_chunk_state<2><<<grid, kThreads, 0, s>>>(c, utt, c_off, total_chunks, gy, pass_buffer,
                                                   states);
-  k_iir_carry<<<c.count, kThreads, 0, s>>>(c, utt, states, carries);
+  k_iir_carry<<<c.count, kCarryThreads, 0, s>>>(c, utt, states, carries);
   k_iir_replay<2><<<grid, kThreads, 0, s>>>(c, utt, c_off, total_chunks, gy, pass_buffer,
                                              carries, full);
   k_iir_chunk_state<3><<<grid, kThreads, 0, s>>>(c, utt, c_off, total_chunks, gy, pass_buffer,
                                                   states);
-  k_iir_carry<<<c.count, kThreads, 0, s>>>(c, utt, states, carries);
+  k_iir_carry<<<c.count, kCarryThreads, 0, s>>>(c, utt, states, carries);
   k_iir_replay<3><<<grid, kThreads, 0, s>>>(c, utt, c_off, total_chunks, gy, pass_buffer,
                                              carries, full);
   k_iir_fold<<<grid_for(total_x), kThreads, 0, s>>>(c, utt, x_off, total_x, full, gx);
