@@ -198,6 +198,10 @@ struct hv_workspace {
   long long last_nnz = -1;
   long long last_frames = 0;
   int lds_limit = 64 * 1024;
+  // Candidate downloads of the last analysis into h_cand: f0 (cand_ev[0])
+  // then scores (cand_ev[1]).  The _async entry points return before they
+  // complete; hv_wait_candidates waits for them.
+  hipEvent_t cand_ev[2] = {nullptr, nullptr};
 
   explicit hv_workspace(int dev) : device(dev) {
     hipDeviceProp_t prop;
@@ -205,10 +209,14 @@ struct hv_workspace {
     lds_limit = (int)prop.sharedMemPerBlock;
     check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
     for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
+    for (auto& e : cand_ev)
+      check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
   }
   ~hv_workspace() {
     if (stream) (void)hipStreamSynchronize(stream);
     for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& e : cand_ev)
       if (e) (void)hipEventDestroy(e);
     for (auto& e : timer_events)
       if (e) (void)hipEventDestroy(e);
@@ -459,7 +467,7 @@ void upload_plan(hv_workspace* ws, const Plan& p) {
 
 // Everything after the decimated, mean-removed y is on the device.
 void analyze_device(hv_workspace* ws, const Plan& p, int32_t* offsets, double* best,
-                    int64_t* nnz, double* timings) {
+                    int64_t* nnz, double* timings, bool defer = false) {
   const Config& c = p.cfg;
   const int U = c.count, B = c.bands;
   hipStream_t s = ws->stream;
@@ -550,15 +558,18 @@ void analyze_device(hv_workspace* ws, const Plan& p, int32_t* offsets, double* b
     throw std::runtime_error("refinement window beyond its precomputed bound");
   // The candidates go straight to the pinned staging buffer, where the host
   // contour reads them in place (hv_result_view) or callers copy them out.
+  // The stream is idle here, so no earlier download still targets h_cand.
   const long long n = hoff[p.total_frames];
-  if (n > 0) {
-    double* hc = ws->h_cand.ensure<double>(2 * n);
+  double* hc = ws->h_cand.ensure<double>(2 * std::max(n, 1LL));
+  if (n > 0)
     check(hipMemcpyAsync(hc, rf0, n * sizeof(double), hipMemcpyDeviceToHost, s),
           "download candidates");
+  check(hipEventRecord(ws->cand_ev[0], s), "hipEventRecord");
+  if (n > 0)
     check(hipMemcpyAsync(hc + n, rsc, n * sizeof(double), hipMemcpyDeviceToHost, s),
           "download candidates");
-    check(hipStreamSynchronize(s), "download candidates");
-  }
+  check(hipEventRecord(ws->cand_ev[1], s), "hipEventRecord");
+  if (!defer) check(hipStreamSynchronize(s), "download candidates");
   if (offsets) std::memcpy(offsets, hoff, (p.total_frames + 1) * sizeof(int32_t));
   if (best) std::memcpy(best, hbest, p.total_frames * sizeof(double));
   ws->last_nnz = n;
@@ -674,12 +685,12 @@ Plan preprocess_batch(hv_workspace* ws, int count, const double* const* xs,
 
 void run_batch(hv_workspace* ws, int count, const double* const* xs, const int64_t* lengths,
                int fs, double f0_floor, double f0_ceil, int32_t* offsets, double* best,
-               int64_t* nnz, double* timings, bool device_x) {
+               int64_t* nnz, double* timings, bool device_x, bool defer = false) {
   if (!ws) invalid("null workspace");
   DeviceGuard guard(ws->device);
   const Plan p = preprocess_batch(ws, count, xs, lengths, fs, f0_floor, f0_ceil, timings,
                                   device_x);
-  analyze_device(ws, p, offsets, best, nnz, timings);
+  analyze_device(ws, p, offsets, best, nnz, timings, defer);
 }
 
 // Frozen-decision refinement of entries (value and/or adjoint), then the
@@ -910,6 +921,39 @@ hv_status hv_analyze_batch_device(hv_workspace* ws, int count, const double* x_d
   });
 }
 
+hv_status hv_analyze_batch_async(hv_workspace* ws, int count, const double* const* xs,
+                                 const int64_t* lengths, int fs, double f0_floor,
+                                 double f0_ceil, int32_t* offsets, double* best, int64_t* nnz,
+                                 double* timings_ms) {
+  return guarded([&] {
+    if (!xs || !lengths || !nnz) invalid("null pointer");
+    for (int u = 0; u < count; ++u)
+      if (!xs[u]) invalid("null waveform pointer");
+    run_batch(ws, count, xs, lengths, fs, f0_floor, f0_ceil, offsets, best, nnz, timings_ms, false,
+              true);
+  });
+}
+
+hv_status hv_analyze_batch_device_async(hv_workspace* ws, int count, const double* x_device,
+                                        const int64_t* lengths, int fs, double f0_floor,
+                                        double f0_ceil, int32_t* offsets, double* best,
+                                        int64_t* nnz, double* timings_ms) {
+  return guarded([&] {
+    if (!x_device || !lengths || !nnz) invalid("null pointer");
+    const double* xs[1] = {x_device};
+    run_batch(ws, count, xs, lengths, fs, f0_floor, f0_ceil, offsets, best, nnz, timings_ms, true,
+              true);
+  });
+}
+
+hv_status hv_wait_candidates(hv_workspace* ws, int part) {
+  return guarded([&] {
+    if (!ws) invalid("null workspace");
+    if (part != 0 && part != 1) invalid("part must be 0 (f0) or 1 (f0 and scores)");
+    check(hipEventSynchronize(ws->cand_ev[part]), "wait for candidates");
+  });
+}
+
 hv_status hv_fetch_initial(hv_workspace* ws, double* f0_initial) {
   return guarded([&] {
     if (!ws) invalid("null workspace");
@@ -951,6 +995,7 @@ hv_status hv_fetch_candidates(hv_workspace* ws, double* f0, double* scores) {
     if (ws->last_nnz < 0) invalid("no analysis result to fetch");
     if (ws->last_nnz == 0) return;
     if (!f0 || !scores) invalid("null pointer");
+    check(hipEventSynchronize(ws->cand_ev[1]), "wait for candidates");
     const long long n = ws->last_nnz;
     const double* h = ws->h_cand.get<double>();
     std::memcpy(f0, h, n * sizeof(double));

@@ -270,6 +270,7 @@ void fix_step3(const double* in, int frames, const CandidateRows& rows, double* 
   boundary_list(in, frames, list);
   const int count = (int)list.size() / 2;
   std::vector<Section> sections(count);
+  if (rows.pending && count > 0) throw_status(hv_wait_candidates(rows.pending, 0));
   // Each section's ExtendF0 reads only the input and the candidates.
   parallel_for(count, frames >= kParallelFrames ? threads : 1, [&](int i) {
     Section& s = sections[i];
@@ -300,6 +301,7 @@ void fix_step3(const double* in, int frames, const CandidateRows& rows, double* 
   }
   std::vector<int> order;
   sorted_order(bl, selected, order);
+  bool scores_ready = rows.pending == nullptr;
   for (int i = 0; i < frames; ++i) out[i] = sections[0].at(i);
   for (int i = 1; i < selected; ++i) {
     const int k = order[i];
@@ -317,6 +319,10 @@ void fix_step3(const double* in, int frames, const CandidateRows& rows, double* 
       continue;
     }
     double score1 = 0.0, score2 = 0.0;
+    if (!scores_ready) {
+      throw_status(hv_wait_candidates(rows.pending, 1));
+      scores_ready = true;
+    }
     for (int j = st2; j <= ed1; ++j) {
       score1 += search_score(out[j], rows, j);
       score2 += search_score(s.at(j), rows, j);
@@ -755,8 +761,10 @@ Result analyze(hv_workspace* ws, const double* x, int64_t length, int fs, double
   int64_t nnz = 0;
   double device_ms[HV_TIMING_SLOTS];
   const int64_t len = length;
-  throw_status(hv_analyze_batch(ws, 1, &x, &len, fs, f0_floor, f0_ceil, nullptr, nullptr, &nnz,
-                                timings ? device_ms : nullptr));
+  // The candidate download overlaps FixStep1-2 and the f0-only part of
+  // FixStep3 (the contour waits for each part where it first needs it).
+  throw_status(hv_analyze_batch_async(ws, 1, &x, &len, fs, f0_floor, f0_ceil, nullptr, nullptr,
+                                      &nnz, timings ? device_ms : nullptr));
   const View v = result_view(ws);
   if (timings) {
     ms_since(mark);
@@ -764,7 +772,7 @@ Result analyze(hv_workspace* ws, const double* x, int64_t length, int fs, double
     record_stats(ws, *timings);
   }
   Result r;
-  CandidateRows rows{v.offsets, v.f0, v.scores};
+  CandidateRows rows{v.offsets, v.f0, v.scores, ws};
   finish_utterance(rows, p.frames, v.best, frame_period, p.out_frames, r, lease.arena(), timings,
                    &mark, std::min(host_threads, kSectionThreads));
   return r;
@@ -805,7 +813,7 @@ std::vector<Result> batch_common(hv_workspace* ws, Call&& call,
   const int inner = std::max(1, std::min(kSectionThreads, host_threads / std::max(1, count)));
   parallel_for(count, host_threads, [&](int k) {
     const int u = order[k];
-    CandidateRows rows{v.offsets + frame_off[u], v.f0, v.scores};
+    CandidateRows rows{v.offsets + frame_off[u], v.f0, v.scores, ws};
     finish_utterance(rows, params[u].frames, v.best + frame_off[u], frame_period,
                      params[u].out_frames, results[u], nullptr, nullptr, nullptr, inner);
   });
@@ -819,8 +827,8 @@ std::vector<Result> analyze_batch(hv_workspace* ws, const std::vector<const doub
                                   double f0_floor, double f0_ceil, double frame_period,
                                   int host_threads, Timings* timings) {
   auto call = [&](int32_t* offsets, double* best, int64_t* nnz, double* ms) {
-    return hv_analyze_batch(ws, (int)xs.size(), xs.data(), lengths.data(), fs, f0_floor,
-                            f0_ceil, offsets, best, nnz, ms);
+    return hv_analyze_batch_async(ws, (int)xs.size(), xs.data(), lengths.data(), fs, f0_floor,
+                                  f0_ceil, offsets, best, nnz, ms);
   };
   return batch_common(ws, call, lengths, fs, f0_floor, f0_ceil, frame_period, host_threads,
                       timings);
@@ -831,8 +839,8 @@ std::vector<Result> analyze_batch_device(hv_workspace* ws, const double* x_devic
                                          double f0_floor, double f0_ceil, double frame_period,
                                          int host_threads, Timings* timings) {
   auto call = [&](int32_t* offsets, double* best, int64_t* nnz, double* ms) {
-    return hv_analyze_batch_device(ws, (int)lengths.size(), x_device, lengths.data(), fs,
-                                   f0_floor, f0_ceil, offsets, best, nnz, ms);
+    return hv_analyze_batch_device_async(ws, (int)lengths.size(), x_device, lengths.data(), fs,
+                                         f0_floor, f0_ceil, offsets, best, nnz, ms);
   };
   return batch_common(ws, call, lengths, fs, f0_floor, f0_ceil, frame_period, host_threads,
                       timings);

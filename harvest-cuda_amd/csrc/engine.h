@@ -18,10 +18,13 @@ struct Result {
 };
 
 // Nonzero refined candidates in slot order, rows offsets[i]..offsets[i+1].
+// With `pending` set, f0 and scores are still being downloaded into the
+// workspace's pinned buffers: readers call hv_wait_candidates first.
 struct CandidateRows {
   const int32_t* offsets;
   const double* f0;
   const double* scores;
+  hv_workspace* pending = nullptr;
 };
 
 using Timings = std::map<std::string, double>;

@@ -124,6 +124,29 @@ hv_status hv_analyze_batch_device(hv_workspace* ws, int count, const double* x_d
                                   double f0_ceil, int32_t* offsets, double* best,
                                   int64_t* nnz, double* timings_ms);
 
+/* hv_analyze_batch / hv_analyze_batch_device that return as soon as the
+ * offsets, best and nnz are on the host, with the candidate download (f0,
+ * then scores) still running on the workspace's stream.  Before reading the
+ * candidates through hv_result_view, call hv_wait_candidates(ws, 0) for f0
+ * and hv_wait_candidates(ws, 1) for the scores as well; hv_fetch_candidates
+ * waits by itself.  No reference counterpart: the host engine runs contour
+ * steps that need only `best` (FixStep1-2, harvest.cpp:710-762) and the
+ * f0-only part of FixStep3 while the download completes. */
+hv_status hv_analyze_batch_async(hv_workspace* ws, int count,
+                                 const double* const* xs, const int64_t* lengths,
+                                 int fs, double f0_floor, double f0_ceil,
+                                 int32_t* offsets, double* best, int64_t* nnz,
+                                 double* timings_ms);
+hv_status hv_analyze_batch_device_async(hv_workspace* ws, int count, const double* x_device,
+                                        const int64_t* lengths, int fs, double f0_floor,
+                                        double f0_ceil, int32_t* offsets, double* best,
+                                        int64_t* nnz, double* timings_ms);
+
+/* Wait until the last analysis' candidate f0 (part 0) or f0 and scores
+ * (part 1) are in the pinned buffers hv_result_view exposes.  Returns at once
+ * after the synchronous entry points.  No reference counterpart. */
+hv_status hv_wait_candidates(hv_workspace* ws, int part);
+
 /* Work counters (HV_STAT_SLOTS entries) of the last analysis on `ws`, for
  * roofline accounting.  No reference counterpart. */
 hv_status hv_last_stats(hv_workspace* ws, int64_t* stats);
