@@ -263,3 +263,64 @@ def test_clear_cache_while_a_call_runs(gpu):
     for r in got:
         for a, b in zip(r, expected):
             np.testing.assert_array_equal(a, b)
+
+
+def test_c_abi_async_entry_points_match_sync(gpu):
+    """hv_analyze_batch_async returns before the candidate download completes;
+    after hv_wait_candidates the zero-copy view holds exactly what the
+    synchronous entry point produces (called through the C ABI)."""
+    import ctypes
+    from pathlib import Path
+    lib = ctypes.CDLL(str(Path(gpu.__file__).parent / "libharvest_hip.so"))
+    P, D, I64 = ctypes.c_void_p, ctypes.c_double, ctypes.c_int64
+    sig = [P, ctypes.c_int, P, P, ctypes.c_int, D, D, P, P, P, P]
+    for name in ("hv_analyze_batch", "hv_analyze_batch_async"):
+        getattr(lib, name).argtypes = sig
+    lib.hv_wait_candidates.argtypes = [P, ctypes.c_int]
+    lib.hv_fetch_candidates.argtypes = [P, P, P]
+    lib.hv_result_view.argtypes = [P, P, P, P, P, P]
+    lib.hv_workspace_destroy.argtypes = [P]
+    fs = 16000
+    xs = [reference_signal("speechlike", seconds=s, fs=fs) for s in (0.6, 1.1)]
+    frames = sum(int(1000.0 * len(x) / fs) + 1 for x in xs)
+    ptrs = (ctypes.c_void_p * 2)(*[x.ctypes.data for x in xs])
+    lens = (I64 * 2)(*[len(x) for x in xs])
+    ws = ctypes.c_void_p()
+    assert lib.hv_workspace_create(0, ctypes.byref(ws)) == 0
+    try:
+        def analyze(fn):
+            off = np.zeros(frames + 1, np.int32)
+            best = np.zeros(frames)
+            nnz = I64()
+            assert fn(ws, 2, ptrs, lens, fs, 71.0, 800.0, off.ctypes.data, best.ctypes.data,
+                      ctypes.byref(nnz), None) == 0
+            return off, best, nnz.value
+
+        def fetch(n):
+            f0, sc = np.zeros(max(n, 1)), np.zeros(max(n, 1))
+            assert lib.hv_fetch_candidates(ws, f0.ctypes.data, sc.ctypes.data) == 0
+            return f0[:n], sc[:n]
+
+        off1, best1, n1 = analyze(lib.hv_analyze_batch)
+        f1, s1 = fetch(n1)
+        assert n1 > 0
+        # Deferred download, waited for explicitly, read in place.
+        off2, best2, n2 = analyze(lib.hv_analyze_batch_async)
+        assert lib.hv_wait_candidates(ws, 0) == 0
+        assert lib.hv_wait_candidates(ws, 1) == 0
+        vo, vb, vf, vs, vn = P(), P(), P(), P(), I64()
+        assert lib.hv_result_view(ws, ctypes.byref(vo), ctypes.byref(vb), ctypes.byref(vf),
+                                  ctypes.byref(vs), ctypes.byref(vn)) == 0
+        assert vn.value == n2 == n1
+        f2 = np.ctypeslib.as_array(ctypes.cast(vf, ctypes.POINTER(D)), (n2,)).copy()
+        s2 = np.ctypeslib.as_array(ctypes.cast(vs, ctypes.POINTER(D)), (n2,)).copy()
+        for a, b in ((off1, off2), (best1, best2), (f1, f2), (s1, s2)):
+            np.testing.assert_array_equal(a, b)
+        # hv_fetch_candidates waits by itself after an async call.
+        analyze(lib.hv_analyze_batch_async)
+        f3, s3 = fetch(n1)
+        np.testing.assert_array_equal(f3, f1)
+        np.testing.assert_array_equal(s3, s1)
+        assert lib.hv_wait_candidates(ws, 2) == 1  # HV_EINVAL
+    finally:
+        lib.hv_workspace_destroy(ws)
