@@ -323,9 +323,11 @@ void fix_step3(const double* in, int frames, const CandidateRows& rows, double* 
       throw_status(hv_wait_candidates(rows.pending, 1));
       scores_ready = true;
     }
-    for (int j = st2; j <= ed1; ++j) {
-      score1 += search_score(out[j], rows, j);
-      score2 += search_score(s.at(j), rows, j);
+    for (int j = st2; j <= ed1; ++j) {  // equal values share one search
+      const double v1 = out[j], v2 = s.at(j);
+      const double a1 = search_score(v1, rows, j);
+      score1 += a1;
+      score2 += v2 == v1 ? a1 : search_score(v2, rows, j);
     }
     const int from = score1 > score2 ? ed1 : st2;
     for (int j = from; j <= ed2; ++j) out[j] = s.at(j);
