@@ -175,7 +175,7 @@ struct hv_workspace {
   std::vector<int> timer_slots;
   size_t timers_used = 0;
   bool timing = false;
-  DevBuf x, pass, states, carries, y, partial, mean, partial3, sums3, quirk;
+  DevBuf x, pass, states, carries, carry_tmp, y, partial, mean, partial3, sums3, quirk;
   DevBuf slab, counts, offs, totals, raw, official, count;
   DevBuf rf0, rsc, pf0, psc, best, row_nnz, offsets, scan_tmp;
   // The plan (utterances and six offset tables) in one device buffer, uploaded
@@ -233,6 +233,7 @@ struct Plan {
   std::vector<long long> c_off, t_off, f_off, p_off, y_off, x_off;
   long long total_x = 0, total_m = 0, total_y = 0, total_frames = 0;
   long long total_tiles = 0, total_chunks = 0, total_pblocks = 0;
+  long long max_chunks = 0;  // largest utterance's IIR chunk count
 };
 
 // Records `launch` between two events attributed to timing slot `slot`.
@@ -427,6 +428,7 @@ void layout(Plan& p, const std::vector<long long>& lengths, const std::vector<in
     p.f_off[u + 1] = p.f_off[u] + t.frames;
     p.t_off[u + 1] = p.t_off[u] + t.tiles;
     p.c_off[u + 1] = p.c_off[u] + t.chunks;
+    p.max_chunks = std::max(p.max_chunks, (long long)t.chunks);
     p.p_off[u + 1] = p.p_off[u] + t.pblocks;
     x += t.length;
     m += t.m;
@@ -670,8 +672,10 @@ Plan preprocess_batch(hv_workspace* ws, int count, const double* const* xs,
   double* carries = ws->carries.ensure<double>(3 * p.total_chunks);
   timed(ws, 3, [&] {
     if (ratio > 1)
-      launch_decimate(s, p.cfg, ws->d_utt, ws->d_c_off, p.total_chunks,
-                      dx, pass, states, carries, y);
+      launch_decimate(s, p.cfg, ws->d_utt, ws->d_c_off, p.total_chunks, p.max_chunks, dx, pass,
+                      states, carries,
+                      ws->carry_tmp.ensure<double>(carry_scratch_doubles(count, p.max_chunks)),
+                      y);
     else
       launch_copy_input(s, p.cfg, ws->d_utt, ws->d_y_off, p.total_y, dx,
                         y);
@@ -751,12 +755,12 @@ void refine_entries(hv_workspace* ws, int count, const double* const* xs,
                        ws->partial.ensure<double>(p.total_pblocks), ws->mean.ensure<double>(count));
     double* gx = ws->gx.ensure<double>(p.total_x);
     if (c.ratio > 1) {
-      launch_decimate_transpose(s, c, dutt, ws->d_c_off, p.total_chunks,
-                                ws->d_x_off, p.total_x, gy,
-                                ws->pass.ensure<double>(p.total_m),
-                                ws->states.ensure<double>(3 * p.total_chunks),
-                                ws->carries.ensure<double>(3 * p.total_chunks),
-                                ws->full.ensure<double>(p.total_m), gx);
+      launch_decimate_transpose(
+          s, c, dutt, ws->d_c_off, p.total_chunks, p.max_chunks, ws->d_x_off, p.total_x, gy,
+          ws->pass.ensure<double>(p.total_m), ws->states.ensure<double>(3 * p.total_chunks),
+          ws->carries.ensure<double>(3 * p.total_chunks),
+          ws->carry_tmp.ensure<double>(carry_scratch_doubles(count, p.max_chunks)),
+          ws->full.ensure<double>(p.total_m), gx);
     } else {
       check(hipMemcpyAsync(gx, gy, p.total_x * sizeof(double), hipMemcpyDeviceToDevice, s),
             "copy gradient");
@@ -850,9 +854,10 @@ hv_status hv_decimate(hv_workspace* ws, const double* x, int64_t length, int rat
     check(hipMemcpyAsync(dx, hx, length * sizeof(double), hipMemcpyHostToDevice, s), "upload x");
     double* dy = ws->y.ensure<double>(y_length);
     if (ratio > 1)
-      launch_decimate(s, p.cfg, ws->d_utt, ws->d_c_off, p.total_chunks, dx,
+      launch_decimate(s, p.cfg, ws->d_utt, ws->d_c_off, p.total_chunks, p.max_chunks, dx,
                       ws->pass.ensure<double>(p.total_m), ws->states.ensure<double>(3 * p.total_chunks),
-                      ws->carries.ensure<double>(3 * p.total_chunks), dy);
+                      ws->carries.ensure<double>(3 * p.total_chunks),
+                      ws->carry_tmp.ensure<double>(carry_scratch_doubles(1, p.max_chunks)), dy);
     else
       launch_copy_input(s, p.cfg, ws->d_utt, ws->d_y_off, p.total_y, dx, dy);
     double* hy = ws->h_best.ensure<double>(y_length);

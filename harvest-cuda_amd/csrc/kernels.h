@@ -10,11 +10,14 @@ struct Quirk {  // WORLD's two overwritten bins, per (utterance, band)
   double d1re, d1im, d2, pad;
 };
 
-// Decimation (exact chunked scan of WORLD's zero-phase IIR).
+// Decimation (exact chunked scan of WORLD's zero-phase IIR).  max_chunks is
+// the largest utterance's chunk count; carry_scratch holds
+// carry_scratch_doubles(count, max_chunks) doubles.
+size_t carry_scratch_doubles(int count, long long max_chunks);
 void launch_decimate(hipStream_t s, const Config& c, const Utt* utt,
-                     const long long* c_off, long long total_chunks,
+                     const long long* c_off, long long total_chunks, long long max_chunks,
                      const double* x, double* pass_buffer, double* states,
-                     double* carries, double* y);
+                     double* carries, double* carry_scratch, double* y);
 void launch_copy_input(hipStream_t s, const Config& c, const Utt* utt,
                        const long long* y_off, long long total, const double* x,
                        double* y);
@@ -76,9 +79,10 @@ void launch_compact(hipStream_t s, const Config& c, const Utt* utt,
 // one pass-length buffer per utterance.
 void launch_decimate_transpose(hipStream_t s, const Config& c, const Utt* utt,
                                const long long* c_off, long long total_chunks,
-                               const long long* x_off, long long total_x, const double* gy,
-                               double* pass_buffer, double* states, double* carries,
-                               double* full, double* gx);
+                               long long max_chunks, const long long* x_off, long long total_x,
+                               const double* gy, double* pass_buffer, double* states,
+                               double* carries, double* carry_scratch, double* full,
+                               double* gx);
 // Frozen-decision refined F0 of (frame, initial f0) entries and/or the
 // weighted adjoint added into gy.
 void launch_refine_entries(hipStream_t s, const Config& c, const Utt* utt,

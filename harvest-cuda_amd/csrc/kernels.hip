@@ -162,54 +162,59 @@ __device__ __forceinline__ void mat3_apply(const double* m, const double* v,
   r[2] = t2;
 }
 
-// One workgroup per utterance: each thread owns a run of K consecutive chunks,
-// runs are combined with a Kogge-Stone scan using (P^K)^(2^j), then every run
-// replays its chunks to emit the carry-in state of each chunk.
-constexpr int kCarryThreads = kThreads, kCarryLevels = 8;
-__global__ __launch_bounds__(kCarryThreads) void k_iir_carry(Config c, const Utt* utt,
-                                                            const double* states,
-                                                            double* carries) {
-  __shared__ double qpow[kCarryLevels][9];
-  __shared__ double scan[kCarryThreads][3];
-  const Utt u = utt[blockIdx.x];
-  const int C = u.chunks;
-  const int K = (C + kCarryThreads - 1) / kCarryThreads;
+// Carry-in state of every chunk from the chunk end states (zero start), in
+// three parallel steps.  An utterance's chunks form blocks of kCarryBlock =
+// 256 runs of kCarryRun chunks.
+//   k_carry_blocks<false>: per block, the end state from a zero start (runs
+//     by Horner, then a Kogge-Stone scan over runs with (P^kCarryRun)^(2^j));
+//   k_carry_top: per utterance, the block start states, sequentially with
+//     Q = P^(kCarryRun * 256);
+//   k_carry_blocks<true>: per block, each run's start state
+//     P^(kCarryRun t) cin + (scan of the runs before it), then the run
+//     replays its chunks and writes their carry-in states.
+constexpr int kCarryRun = 4, kCarryBlock = kThreads * kCarryRun;
+static_assert(kCarryBlock == 1024, "k_carry_top forms P^1024 by ten squarings");
+
+__device__ void carry_powers(const double* P, double (*qpow)[9]) {
+  // qpow[0] = P^kCarryRun, qpow[j] = qpow[j - 1]^2 (qpow[8] = Q).
+  double t[9];
+  mat3_mul(P, P, t);
+  mat3_mul(t, t, qpow[0]);
+  for (int j = 1; j < 9; ++j) mat3_mul(qpow[j - 1], qpow[j - 1], qpow[j]);
+}
+
+template <bool kApply>
+__global__ __launch_bounds__(kThreads) void k_carry_blocks(Config c, const Utt* utt,
+                                                           const double* states, int max_blocks,
+                                                           double* block_agg,
+                                                           const double* block_cin,
+                                                           double* carries) {
+  __shared__ double qpow[9][9];
+  __shared__ double scan[kThreads][3];
+  const int ui = blockIdx.y, b = blockIdx.x;
+  const Utt u = utt[ui];
+  const long long C = u.chunks;
+  const long long b0 = (long long)b * kCarryBlock;
+  if (b0 >= C) return;  // uniform over the workgroup
   const double* P = c.chunk_pow;
-  if (threadIdx.x == 0) {
-    double q[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, base[9], tmp[9];
-    for (int k = 0; k < 9; ++k) base[k] = P[k];
-    for (int e = K; e > 0; e >>= 1) {
-      if (e & 1) {
-        mat3_mul(q, base, tmp);
-        for (int k = 0; k < 9; ++k) q[k] = tmp[k];
-      }
-      mat3_mul(base, base, tmp);
-      for (int k = 0; k < 9; ++k) base[k] = tmp[k];
-    }
-    for (int k = 0; k < 9; ++k) qpow[0][k] = q[k];
-    for (int j = 1; j < kCarryLevels; ++j) mat3_mul(qpow[j - 1], qpow[j - 1], qpow[j]);
-  }
-  const long long first = (long long)threadIdx.x * K;
-  const long long last = min(first + (long long)K, (long long)C);
+  if (threadIdx.x == 0) carry_powers(P, qpow);
+  const long long first = b0 + (long long)threadIdx.x * kCarryRun;
+  const long long last = min(first + kCarryRun, C);
   const double* z = states + u.c_off * 3;
+  double zz[kCarryRun][3];
+#pragma unroll
+  for (int q = 0; q < kCarryRun; ++q)
+    for (int r = 0; r < 3; ++r) zz[q][r] = first + q < last ? z[(first + q) * 3 + r] : 0.0;
   double agg[3] = {0.0, 0.0, 0.0};
-  for (long long k0 = first; k0 < last; k0 += kIirAhead) {
-    double zz[kIirAhead][3];
 #pragma unroll
-    for (int q = 0; q < kIirAhead; ++q)
-      for (int r = 0; r < 3; ++r) zz[q][r] = k0 + q < last ? z[(k0 + q) * 3 + r] : 0.0;
-#pragma unroll
-    for (int q = 0; q < kIirAhead; ++q) {
-      if (k0 + q >= last) break;
-      mat3_apply(P, agg, zz[q], agg);
-    }
-  }
+  for (int q = 0; q < kCarryRun; ++q)
+    if (first + q < last) mat3_apply(P, agg, zz[q], agg);
   scan[threadIdx.x][0] = agg[0];
   scan[threadIdx.x][1] = agg[1];
   scan[threadIdx.x][2] = agg[2];
   __syncthreads();
-  for (int j = 0; j < kCarryLevels; ++j) {
-    int d = 1 << j;
+  for (int j = 0; j < 8; ++j) {
+    const int d = 1 << j;
     double nv[3] = {scan[threadIdx.x][0], scan[threadIdx.x][1], scan[threadIdx.x][2]};
     if ((int)threadIdx.x >= d) {
       double prev[3] = {scan[threadIdx.x - d][0], scan[threadIdx.x - d][1],
@@ -222,27 +227,54 @@ __global__ __launch_bounds__(kCarryThreads) void k_iir_carry(Config c, const Utt
     scan[threadIdx.x][2] = nv[2];
     __syncthreads();
   }
-  double carry[3] = {0.0, 0.0, 0.0};
-  if (threadIdx.x > 0) {
-    carry[0] = scan[threadIdx.x - 1][0];
-    carry[1] = scan[threadIdx.x - 1][1];
-    carry[2] = scan[threadIdx.x - 1][2];
+  const long long slot = ((long long)ui * max_blocks + b) * 3;
+  if (!kApply) {
+    // The end state of a full block (a partial last block's is never used).
+    if (threadIdx.x == kThreads - 1)
+      for (int r = 0; r < 3; ++r) block_agg[slot + r] = scan[threadIdx.x][r];
+    return;
   }
+  if (first >= C) return;
+  // P^(kCarryRun t) cin from the binary digits of t, then the runs before t.
+  double carry[3] = {block_cin[slot], block_cin[slot + 1], block_cin[slot + 2]};
+  const double zero[3] = {0.0, 0.0, 0.0};
+  for (int j = 0; j < 8; ++j)
+    if ((threadIdx.x >> j) & 1) mat3_apply(qpow[j], carry, zero, carry);
+  if (threadIdx.x > 0)
+    for (int r = 0; r < 3; ++r) carry[r] += scan[threadIdx.x - 1][r];
   double* out = carries + u.c_off * 3;
-  for (long long k0 = first; k0 < last; k0 += kIirAhead) {
-    double zz[kIirAhead][3];
 #pragma unroll
-    for (int q = 0; q < kIirAhead; ++q)
-      for (int r = 0; r < 3; ++r) zz[q][r] = k0 + q < last ? z[(k0 + q) * 3 + r] : 0.0;
-#pragma unroll
-    for (int q = 0; q < kIirAhead; ++q) {
-      const long long k = k0 + q;
-      if (k >= last) break;
-      out[k * 3 + 0] = carry[0];
-      out[k * 3 + 1] = carry[1];
-      out[k * 3 + 2] = carry[2];
-      mat3_apply(P, carry, zz[q], carry);
-    }
+  for (int q = 0; q < kCarryRun; ++q) {
+    const long long k = first + q;
+    if (k >= last) break;
+    out[k * 3 + 0] = carry[0];
+    out[k * 3 + 1] = carry[1];
+    out[k * 3 + 2] = carry[2];
+    mat3_apply(P, carry, zz[q], carry);
+  }
+}
+
+// One thread per utterance: block start states in block order.
+__global__ __launch_bounds__(kThreads) void k_carry_top(Config c, const Utt* utt,
+                                                        int max_blocks,
+                                                        const double* block_agg,
+                                                        double* block_cin) {
+  const int ui = blockIdx.x * kThreads + threadIdx.x;
+  if (ui >= c.count) return;
+  const Utt u = utt[ui];
+  const int nb = (int)((u.chunks + kCarryBlock - 1) / kCarryBlock);
+  double q[9], t[9];  // Q = P^1024 by ten squarings
+  for (int k = 0; k < 9; ++k) q[k] = c.chunk_pow[k];
+  for (int j = 0; j < 10; ++j) {
+    mat3_mul(q, q, t);
+    for (int k = 0; k < 9; ++k) q[k] = t[k];
+  }
+  double cur[3] = {0.0, 0.0, 0.0};
+  for (int b = 0; b < nb; ++b) {
+    const long long slot = ((long long)ui * max_blocks + b) * 3;
+    for (int r = 0; r < 3; ++r) block_cin[slot + r] = cur[r];
+    const double a[3] = {block_agg[slot], block_agg[slot + 1], block_agg[slot + 2]};
+    mat3_apply(q, cur, a, cur);
   }
 }
 
@@ -1763,19 +1795,37 @@ __global__ __launch_bounds__(kThreads) void k_scan_local(const int* in, int* out
 }  // namespace
 
 // ------------------------------------------------------------------ launchers
+size_t carry_scratch_doubles(int count, long long max_chunks) {
+  const long long blocks = (max_chunks + kCarryBlock - 1) / kCarryBlock;
+  return (size_t)2 * 3 * (size_t)count * (size_t)std::max(blocks, 1LL);
+}
+
+// The same block kernels serve every utterance length, so an utterance's
+// carries do not depend on the batch it is analysed in.
+void launch_carries(hipStream_t s, const Config& c, const Utt* utt, long long max_chunks,
+                    const double* states, double* carries, double* scratch) {
+  const int max_blocks = (int)std::max(1LL, (max_chunks + kCarryBlock - 1) / kCarryBlock);
+  double* agg = scratch;
+  double* cin = scratch + (size_t)3 * c.count * max_blocks;
+  const dim3 grid((unsigned)max_blocks, (unsigned)c.count);
+  k_carry_blocks<false><<<grid, kThreads, 0, s>>>(c, utt, states, max_blocks, agg, cin, carries);
+  k_carry_top<<<grid_for(c.count), kThreads, 0, s>>>(c, utt, max_blocks, agg, cin);
+  k_carry_blocks<true><<<grid, kThreads, 0, s>>>(c, utt, states, max_blocks, agg, cin, carries);
+}
+
 void launch_decimate(hipStream_t s, const Config& c, const Utt* utt,
-                     const long long* c_off, long long total_chunks,
+                     const long long* c_off, long long total_chunks, long long max_chunks,
                      const double* x, double* pass_buffer, double* states,
-                     double* carries, double* y) {
+                     double* carries, double* carry_scratch, double* y) {
   const unsigned grid = grid_for(total_chunks);
   k_iir_chunk_state<0><<<grid, kThreads, 0, s>>>(c, utt, c_off, total_chunks, x,
                                                   pass_buffer, states);
-  k_iir_carry<<<c.count, kCarryThreads, 0, s>>>(c, utt, states, carries);
+  launch_carries(s, c, utt, max_chunks, states, carries, carry_scratch);
   k_iir_replay<0><<<grid, kThreads, 0, s>>>(c, utt, c_off, total_chunks, x,
                                              pass_buffer, carries, y);
   k_iir_chunk_state<1><<<grid, kThreads, 0, s>>>(c, utt, c_off, total_chunks, x,
                                                   pass_buffer, states);
-  k_iir_carry<<<c.count, kCarryThreads, 0, s>>>(c, utt, states, carries);
+  launch_carries(s, c, utt, max_chunks, states, carries, carry_scratch);
   k_iir_replay<1><<<grid, kThreads, 0, s>>>(c, utt, c_off, total_chunks, x,
                                              pass_buffer, carries, y);
   check_launch("decimate");
@@ -1940,18 +1990,19 @@ void launch_compact(hipStream_t s, const Config& c, const Utt* utt,
 
 void launch_decimate_transpose(hipStream_t s, const Config& c, const Utt* utt,
                                const long long* c_off, long long total_chunks,
-                               const long long* x_off, long long total_x, const double* gy,
-                               double* pass_buffer, double* states, double* carries,
-                               double* full, double* gx) {
+                               long long max_chunks, const long long* x_off, long long total_x,
+                               const double* gy, double* pass_buffer, double* states,
+                               double* carries, double* carry_scratch, double* full,
+                               double* gx) {
   const unsigned grid = grid_for(total_chunks);
   k_iir_chunk_state<2><<<grid, kThreads, 0, s>>>(c, utt, c_off, total_chunks, gy, pass_buffer,
                                                   states);
-  k_iir_carry<<<c.count, kCarryThreads, 0, s>>>(c, utt, states, carries);
+  launch_carries(s, c, utt, max_chunks, states, carries, carry_scratch);
   k_iir_replay<2><<<grid, kThreads, 0, s>>>(c, utt, c_off, total_chunks, gy, pass_buffer,
                                              carries, full);
   k_iir_chunk_state<3><<<grid, kThreads, 0, s>>>(c, utt, c_off, total_chunks, gy, pass_buffer,
                                                   states);
-  k_iir_carry<<<c.count, kCarryThreads, 0, s>>>(c, utt, states, carries);
+  launch_carries(s, c, utt, max_chunks, states, carries, carry_scratch);
   k_iir_replay<3><<<grid, kThreads, 0, s>>>(c, utt, c_off, total_chunks, gy, pass_buffer,
                                              carries, full);
   k_iir_fold<<<grid_for(total_x), kThreads, 0, s>>>(c, utt, x_off, total_x, full, gx);
