@@ -730,9 +730,22 @@ __global__ __launch_bounds__(kThreads) void k_interp(
   if (i_lo > i_hi) return;
   double* out = raw + (long long)b * total_frames + u.f_off;
 
+  // Every type's counts and offsets, loaded together: its total E, this
+  // tile's first rank and count, and the rank ranges of the adjacent tiles
+  // (where the window margins almost always fall).
+  int tE[4], tlo[4], tn[4], tprev[4], tnext[4];
+  const bool has_prev = g > u.t_off, has_next = g + 1 < u.t_off + u.tiles;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const long long q = bg * 4 + t;
+    tE[t] = totals[q * c.count + ui];
+    tlo[t] = offs[q * total_tiles + g];
+    tn[t] = counts[q * total_tiles + g];
+    tprev[t] = has_prev ? counts[q * total_tiles + g - 1] : 0;
+    tnext[t] = has_next ? counts[q * total_tiles + g + 1] : 0;
+  }
   bool valid = true;
-  for (int t = 0; t < 4; ++t)
-    valid = valid && totals[(long long)(bg * 4 + t) * c.count + ui] >= 4;
+  for (int t = 0; t < 4; ++t) valid = valid && tE[t] >= 4;
   if (!valid) {
     for (long long i = i_lo + threadIdx.x; i <= i_hi; i += kThreads) out[i] = 0.0;
     return;
@@ -741,19 +754,25 @@ __global__ __launch_bounds__(kThreads) void k_interp(
   double sum[kReps];
   for (int ty = 0; ty < 4; ++ty) {
     const long long q = bg * 4 + ty;
-    const int* qcnt = counts + q * total_tiles;
     const int* qoff = offs + q * total_tiles;
-    const int E = totals[q * c.count + ui];
-    const int r_lo = qoff[g], n_in = qcnt[g];
+    const int E = tE[ty];
+    const int r_lo = tlo[ty], n_in = tn[ty];
+    const int r_prev = r_lo - tprev[ty], r_end = r_lo + n_in, r_next = r_end + tnext[ty];
     const int wlo = max(0, r_lo - kWindowMargin);
     const int whi = min(E - 1, r_lo + n_in - 1 + kWindowMargin);
     if (ty > 0) __syncthreads();  // the previous type's readers are done
     for (int k = wlo + threadIdx.x; k <= whi; k += kThreads) {
       long long gt;
       int local;
-      if (k >= r_lo && k < r_lo + n_in) {
+      if (k >= r_lo && k < r_end) {
         gt = g;
         local = k - r_lo;
+      } else if (k < r_lo && k >= r_prev) {
+        gt = g - 1;
+        local = k - r_prev;
+      } else if (k >= r_end && k < r_next) {
+        gt = g + 1;
+        local = k - r_end;
       } else {
         long long lo = u.t_off, hi = u.t_off + u.tiles - 1;
         while (lo < hi) {
