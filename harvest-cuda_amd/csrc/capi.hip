@@ -416,7 +416,7 @@ void layout(Plan& p, const std::vector<long long>& lengths, const std::vector<in
     t.n = ns[u];
     t.fft = ffts[u];
     t.frames = frames[u];
-    t.tiles = (int)((t.n + kTile - 1) / kTile);
+    t.tiles = (int)((t.n + kTileStride - 1) / kTileStride);
     t.chunks = (int)((t.m + kChunk - 1) / kChunk);
     t.pblocks = (int)((t.n + kMeanBlock - 1) / kMeanBlock);
     t.y_off = p.y_off[u];

@@ -12,16 +12,20 @@ constexpr double kGuard = 0.000000000001;
 
 constexpr int kThreads = 256;        // workgroup size of every kernel (4 waves)
 constexpr int kTile = 2048;          // filterbank output samples per workgroup
+// Tiles start kTileStride samples apart and own the crossings of their first
+// kTileStride samples; the two extra outputs are the next tile's first two,
+// which the derivative crossings of the last owned samples read.
+constexpr int kTileStride = kTile - 2;
 constexpr int kSlabCap = kTile / 2;  // max crossings of one type per tile
 constexpr int kChunk = 128;          // IIR samples per scan chunk
 constexpr int kMeanBlock = 4096;     // samples per deterministic partial sum
-constexpr int kWindowMargin = 4;
-// Frames per thread in k_interp: a tile spans kTile r 1000 / fs frames, at
+constexpr int kWindowMargin = 4;     // crossings kept either side of a tile
+// Frames per thread in k_interp: a tile spans kTileStride r 1000 / fs frames, at
 // most 385 for the smallest working rate (fs / r >= 5333 Hz); launches check it.
 constexpr int kInterpFramesPerThread = 2;
 // Work counters are spread over this many addresses (by workgroup) so that
 // the per-wave atomics of a launch do not queue on one address; the host sums.
-constexpr int kWorkSpread = 64;     // crossings kept either side of a tile
+constexpr int kWorkSpread = 64;
 
 // Per-utterance layout of a batch.  All offsets index the concatenated
 // device arrays of the batch.

@@ -504,7 +504,9 @@ __global__ __launch_bounds__(kThreads) void k_fill_int(int* p, long long n, int 
 }
 
 // ---------------------------------------------------------------------------
-// Filterbank + crossings.  One workgroup = one (tile of kTile samples, band).
+// Filterbank + crossings.  One workgroup = one (tile, band): kTile outputs
+// from sample kTileStride * tile, of which the first kTileStride own their
+// crossings (the last two serve only the derivative crossings before them).
 // f[i] = sum_{m=0}^{2h} c[m] y[i + h + 1 - m] + Q(i + h + 1) / n reproduces
 // WORLD's filtered_signal[i] / n (harvest.cpp:115-142; c2r scaling is a
 // power of two and cancels in every crossing).  Each thread produces 8
@@ -533,7 +535,7 @@ __global__ __launch_bounds__(kThreads) void k_filter_events(
   const int b = band0 + bg;
   const int ui = find_segment(t_off, c.count, g);
   const Utt u = utt[ui];
-  const long long i0 = (g - u.t_off) * kTile;
+  const long long i0 = (g - u.t_off) * kTileStride;
   const int N = u.n;
   const int h = half[b];
   const int taps8 = (2 * h + 1 + 7) & ~7;
@@ -581,12 +583,6 @@ __global__ __launch_bounds__(kThreads) void k_filter_events(
       for (int r = 0; r < 8; ++r) acc[r] = fma(cc, r + k < 8 ? vb[r + k] : va[r + k - 8], acc[r]);
     }
   }
-  // Two halo outputs (kTile, kTile + 1) for the derivative crossings.
-  double halo = 0.0;
-  if (threadIdx.x < 2) {
-    const int j = kTile + threadIdx.x;
-    for (int mm = 0; mm < taps8; ++mm) halo = fma(cr[mm], Y(j + mm), halo);
-  }
   // Two-bin correction (-1)^k [2 Re(D1 e^{-2 pi i k / n}) + D2] / n at
   // k = i0 + j + h + 1: the exact twiddle of the thread's first output, then
   // rotation by 2 pi / n per output.
@@ -610,13 +606,6 @@ __global__ __launch_bounds__(kThreads) void k_filter_events(
     tcs = nc;
     tsn = ns;
   }
-  if (threadIdx.x < 2) {
-    const int j = kTile + threadIdx.x;
-    const long long k = i0 + j + h + 1;
-    double sn, cs;
-    sincospi(2.0 * (double)(k & nmask) / u.fft, &sn, &cs);
-    f[pad8(j)] = halo + correction_at(k, cs, sn) * inv_n;
-  }
   __syncthreads();
 
   // Crossing flags: bit (type * 8 + r).  Types: 0 negative-going, 1
@@ -627,6 +616,7 @@ __global__ __launch_bounds__(kThreads) void k_filter_events(
   for (int r = 0; r < 8; ++r) {
     const int j = j0 + r;
     const long long i = i0 + j;
+    if (j >= kTileStride) break;  // owned by the next tile
     if (i <= N - 2) {
       const double a = F(j), bb = F(j + 1);
       if (0.0 < a && bb <= 0.0) flags |= 1u << (0 * 8 + r);
@@ -722,7 +712,7 @@ __global__ __launch_bounds__(kThreads) void k_interp(
   const int ui = find_segment(t_off, c.count, g);
   const Utt u = utt[ui];
   const long long tau = g - u.t_off;
-  const long long scale = (long long)kTile * 1000LL * c.ratio;
+  const long long scale = (long long)kTileStride * 1000LL * c.ratio;
   long long i_lo = (tau * scale + c.fs - 1) / c.fs;
   long long i_hi = (tau == u.tiles - 1) ? (long long)u.frames - 1
                                         : ((tau + 1) * scale + c.fs - 1) / c.fs - 1;
@@ -1907,7 +1897,7 @@ void launch_filter_events(hipStream_t s, const Config& c, const Utt* utt,
     k_filter_events<true><<<grid, kThreads, lds_bytes, s>>>(
         c, utt, t_off, total_tiles, band0, y, coef, coef_off, half, quirk, slab, counts);
   } else {
-    k_filter_events<false><<<grid, kThreads, sizeof(double) * pad8_size(kTile + 2), s>>>(
+    k_filter_events<false><<<grid, kThreads, sizeof(double) * pad8_size(kTile), s>>>(
         c, utt, t_off, total_tiles, band0, y, coef, coef_off, half, quirk, slab, counts);
   }
   check_launch("filter_events");
@@ -1927,8 +1917,8 @@ void launch_interp(hipStream_t s, const Config& c, const Utt* utt,
                    const double* boundaries, const double* slab,
                    const int* counts, const int* offs, const int* totals,
                    double* raw) {
-  // Frames per tile: at most ceil(kTile r 1000 / fs) + 1.
-  const long long per_tile = ((long long)kTile * 1000LL * c.ratio + c.fs - 1) / c.fs + 1;
+  // Frames per tile: at most ceil(kTileStride r 1000 / fs) + 1.
+  const long long per_tile = ((long long)kTileStride * 1000LL * c.ratio + c.fs - 1) / c.fs + 1;
   if (per_tile > (long long)kInterpFramesPerThread * kThreads)
     throw std::runtime_error("interp: " + std::to_string(per_tile) +
                              " frames per tile exceed the kernel's threads");
