@@ -15,7 +15,7 @@ contour).  One step = one call.  Also reported:
   * batched: config 3 (512 utterances at 24 kHz) as one batched call versus
     one plain call per utterance on the same GPU.
 
-Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--batched 1]
+Usage: python bench.py [--gpus 1] [--steps 50] [--warmup 15] [--batched 1]
 """
 from __future__ import annotations
 
@@ -182,8 +182,10 @@ def batched_leg(device, warmup):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=50)
+    # A process's first calls include one-time costs (one stall of several ms
+    # around the tenth call); the warm-up absorbs them.
+    ap.add_argument("--warmup", type=int, default=15)
     ap.add_argument("--batched", type=int, default=1, help="also run the config-3 batched leg")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -225,8 +227,8 @@ def main():
     def step(timed):
         return _core.harvest_device(ptr, [len(x)], fs, timings=timed)
 
-    for _ in range(args.warmup):
-        step(False)
+    for _ in range(args.warmup):  # with timings, so that their events exist before timing
+        step(True)
     kernel_keys = ["kernel_decimate", "kernel_filterbank", "kernel_interp", "kernel_refine",
                    "kernel_prune_compact"]
     kernels = {k: 0.0 for k in kernel_keys}
@@ -234,8 +236,10 @@ def main():
     device_ms = 0.0  # hipEvent span of the device launch sequence
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    marks = []
     for _ in range(args.steps):
         out, tm = step(True)
+        marks.append(time.perf_counter())
         for k in kernel_keys:
             kernels[k] += tm[k]
         work["window_samples"] += tm["work_window_samples"]
@@ -244,6 +248,8 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed / args.steps * 1000.0
+    per = [(b - a) * 1000.0 for a, b in zip([t0] + marks[:-1], marks)]
+    log("step ms: " + " ".join(f"{v:.2f}" for v in per))
     f0 = out[0][0]
     name, roof = roofline(kernels, work, [len(x)], fs, args.steps)
     log(f"plain config 2: {ms_per_step:.3f} ms/step, dominant {name} {roof['avg_ms']:.3f} ms")
