@@ -167,14 +167,33 @@ void boundary_list(const double* f0, int frames, std::vector<int>& list) {
   }
 }
 
-// FixStep1 (harvest.cpp:710-722), allowed range 0.008.
+// fabs(n) / fabs(d) > limit for the rounded quotient, without the division
+// when |n| is more than 1e-14 (relative) away from limit * |d|: that margin
+// exceeds every rounding involved.  Zero, tiny or non-finite operands take
+// the division.
+bool quotient_above(double n, double d, double limit) {
+  const double an = std::fabs(n), ad = std::fabs(d);
+  if (std::isfinite(an) && std::isfinite(ad) && ad > 1e-290) {
+    const double m = limit * ad;
+    if (an > m * (1.0 + 1e-14)) return true;
+    if (an < m * (1.0 - 1e-14)) return false;
+  }
+  return std::fabs(n / d) > limit;
+}
+
+// FixStep1 (harvest.cpp:710-722), allowed range 0.008.  Both tests are
+// WORLD's quotients (quotient_above decides them exactly); base >= 0, so
+// fabs(a - b) / b is fabs((a - b) / b) whenever b > 0, and b = 0 (0 / 0 or
+// x / 0) goes to the division.
 void fix_step1(const double* base, int frames, double* out) {
   for (int i = 0; i < frames; ++i) out[i] = 0.0;
   for (int i = 2; i < frames; ++i) {
     if (base[i] == 0.0) continue;
     const double reference = base[i - 1] * 2 - base[i - 2];
-    const bool jump = std::fabs((base[i] - reference) / reference) > 0.008 &&
-                      std::fabs((base[i] - base[i - 1])) / base[i - 1] > 0.008;
+    const double prev = base[i - 1];
+    const bool jump = quotient_above(base[i] - reference, reference, 0.008) &&
+                      (prev > 0.0 ? quotient_above(base[i] - prev, prev, 0.008)
+                                  : std::fabs(base[i] - prev) / prev > 0.008);
     out[i] = jump ? 0.0 : base[i];
   }
 }
@@ -199,6 +218,19 @@ struct Section {
     return i >= 0 && i < (int)values.size() ? values[i] : 0.0;
   }
 };
+
+// out[j] = s.at(j) for j in [a, b]: the section's stored values, 0 outside them.
+void copy_section(const Section& s, int a, int b, double* out) {
+  if (a > b) return;
+  const int lo = std::max(a, s.offset), hi = std::min(b, s.offset + (int)s.values.size() - 1);
+  if (lo > hi) {
+    std::fill(out + a, out + b + 1, 0.0);
+    return;
+  }
+  std::fill(out + a, out + lo, 0.0);
+  std::memcpy(out + lo, s.values.data() + (lo - s.offset), (hi - lo + 1) * sizeof(double));
+  std::fill(out + hi + 1, out + b + 1, 0.0);
+}
 
 // SelectBestF0 (harvest.cpp:636-650) over the nonzero candidates of a frame;
 // zero candidates have error 1 and never beat the 0.18 allowance.  A
@@ -302,12 +334,12 @@ void fix_step3(const double* in, int frames, const CandidateRows& rows, double* 
   std::vector<int> order;
   sorted_order(bl, selected, order);
   bool scores_ready = rows.pending == nullptr;
-  for (int i = 0; i < frames; ++i) out[i] = sections[0].at(i);
+  copy_section(sections[0], 0, frames - 1, out);
   for (int i = 1; i < selected; ++i) {
     const int k = order[i];
     const Section& s = sections[k];
     if (bl[2 * k] - bl[1] > 0) {
-      for (int j = bl[2 * k]; j <= bl[2 * k + 1]; ++j) out[j] = s.at(j);
+      copy_section(s, bl[2 * k], bl[2 * k + 1], out);
       bl[0] = bl[2 * k];
       bl[1] = bl[2 * k + 1];
       continue;
@@ -330,7 +362,7 @@ void fix_step3(const double* in, int frames, const CandidateRows& rows, double* 
       score2 += v2 == v1 ? a1 : search_score(v2, rows, j);
     }
     const int from = score1 > score2 ? ed1 : st2;
-    for (int j = from; j <= ed2; ++j) out[j] = s.at(j);
+    copy_section(s, from, ed2, out);
     bl[1] = ed2;
   }
 }

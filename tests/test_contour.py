@@ -197,3 +197,30 @@ def test_section_pool_survives_fork(core):
     with ctx.Pool(1) as pool:
         got = pool.apply_async(_smooth_in_child, ((contour, 4),)).get(timeout=120)
     np.testing.assert_array_equal(got, want)
+
+
+def test_fix_step1_threshold_ties(core, oracle):
+    """FixStep1's 0.008 tests (harvest.cpp:710-722) at quotients within a few
+    ulps of the threshold, zero neighbours and negative references: the
+    engine's division-free decision equals WORLD's rounded quotients."""
+    rng = np.random.default_rng(1718)
+    for trial in range(300):
+        frames = 12
+        base = np.zeros(frames)
+        b0, b1 = 100.0 + rng.random() * 300.0, 100.0 + rng.random() * 300.0
+        if trial % 7 == 0:
+            b0 = 0.0  # reference 2 b1 - 0
+        if trial % 11 == 0:
+            b0 = 3.0 * b1  # negative reference
+        ref = b1 * 2 - b0
+        target = ref + ref * 0.008 * (1 if trial % 2 else -1)
+        for _ in range(int(rng.integers(0, 6))):
+            target = np.nextafter(target, np.inf if trial % 3 else -np.inf)
+        base[3:9] = [b0, b1, abs(target) if target != 0 else 1.0, 150.0, 151.0, 152.0]
+        if trial % 13 == 0:
+            base[4] = 0.0  # previous frame unvoiced
+        f0 = base[:, None].copy()
+        sc = np.where(f0 > 0, 1.0, 0.0)
+        expect = oracle.fix_contour(f0, sc, 1)
+        got = engine_contour(core, oracle, f0, sc)
+        np.testing.assert_array_equal(got, expect, err_msg=f"trial {trial}")
