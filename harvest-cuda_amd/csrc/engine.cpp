@@ -167,33 +167,14 @@ void boundary_list(const double* f0, int frames, std::vector<int>& list) {
   }
 }
 
-// fabs(n) / fabs(d) > limit for the rounded quotient, without the division
-// when |n| is more than 1e-14 (relative) away from limit * |d|: that margin
-// exceeds every rounding involved.  Zero, tiny or non-finite operands take
-// the division.
-bool quotient_above(double n, double d, double limit) {
-  const double an = std::fabs(n), ad = std::fabs(d);
-  if (std::isfinite(an) && std::isfinite(ad) && ad > 1e-290) {
-    const double m = limit * ad;
-    if (an > m * (1.0 + 1e-14)) return true;
-    if (an < m * (1.0 - 1e-14)) return false;
-  }
-  return std::fabs(n / d) > limit;
-}
-
-// FixStep1 (harvest.cpp:710-722), allowed range 0.008.  Both tests are
-// WORLD's quotients (quotient_above decides them exactly); base >= 0, so
-// fabs(a - b) / b is fabs((a - b) / b) whenever b > 0, and b = 0 (0 / 0 or
-// x / 0) goes to the division.
+// FixStep1 (harvest.cpp:710-722), allowed range 0.008.
 void fix_step1(const double* base, int frames, double* out) {
   for (int i = 0; i < frames; ++i) out[i] = 0.0;
   for (int i = 2; i < frames; ++i) {
     if (base[i] == 0.0) continue;
     const double reference = base[i - 1] * 2 - base[i - 2];
-    const double prev = base[i - 1];
-    const bool jump = quotient_above(base[i] - reference, reference, 0.008) &&
-                      (prev > 0.0 ? quotient_above(base[i] - prev, prev, 0.008)
-                                  : std::fabs(base[i] - prev) / prev > 0.008);
+    const bool jump = std::fabs((base[i] - reference) / reference) > 0.008 &&
+                      std::fabs((base[i] - base[i - 1])) / base[i - 1] > 0.008;
     out[i] = jump ? 0.0 : base[i];
   }
 }
