@@ -25,12 +25,16 @@ namespace {
 using Input = py::array_t<double, py::array::c_style>;
 using Cache = WorkspaceCache<hv_workspace>;
 
+// Cache keys: the device index, and the device index plus kSecondSlot for the
+// second workspace a pipelined batch uses on that device.
+constexpr int kSecondSlot = 1 << 16;
+
 Cache& cache() {
   // Owned by the module for the life of the process; cleared explicitly.
   static Cache* instance = new Cache(
-      [](int device) {
+      [](int key) {
         hv_workspace* ws = nullptr;
-        throw_status(hv_workspace_create(device, &ws));
+        throw_status(hv_workspace_create(key % kSecondSlot, &ws));
         return ws;
       },
       [](hv_workspace* ws) { hv_workspace_destroy(ws); });
@@ -108,8 +112,17 @@ PYBIND11_MODULE(_core, m) {
     {
       py::gil_scoped_release release;
       Cache::Lease ws(cache(), device);
-      rs = analyze_batch(ws.get(), ptrs, lengths, fs, floor, ceil, period, host_threads(),
-                         timings ? &t : nullptr);
+      if (device >= kSecondSlot) throw std::invalid_argument("device index out of range");
+      std::vector<int64_t> frames;
+      for (int64_t n : lengths) frames.push_back(derive_params(n, fs, floor, ceil, period).frames);
+      if (!timings && pipeline_groups(frames) > 1) {
+        Cache::Lease second(cache(), device + kSecondSlot);
+        rs = analyze_batch(ws.get(), ptrs, lengths, fs, floor, ceil, period, host_threads(),
+                           nullptr, second.get());
+      } else {
+        rs = analyze_batch(ws.get(), ptrs, lengths, fs, floor, ceil, period, host_threads(),
+                           timings ? &t : nullptr);
+      }
     }
     py::list out;
     for (auto& r : rs) out.append(py::make_tuple(to_numpy(std::move(r.f0)), to_numpy(std::move(r.times))));
@@ -132,8 +145,18 @@ PYBIND11_MODULE(_core, m) {
     {
       py::gil_scoped_release release;
       Cache::Lease ws(cache(), device);
-      rs = analyze_batch_device(ws.get(), reinterpret_cast<const double*>(x_device), lengths, fs,
-                                floor, ceil, period, host_threads(), timings ? &t : nullptr);
+      if (device >= kSecondSlot) throw std::invalid_argument("device index out of range");
+      std::vector<int64_t> frames;
+      for (int64_t n : lengths) frames.push_back(derive_params(n, fs, floor, ceil, period).frames);
+      const double* xd = reinterpret_cast<const double*>(x_device);
+      if (!timings && pipeline_groups(frames) > 1) {
+        Cache::Lease second(cache(), device + kSecondSlot);
+        rs = analyze_batch_device(ws.get(), xd, lengths, fs, floor, ceil, period, host_threads(),
+                                  nullptr, second.get());
+      } else {
+        rs = analyze_batch_device(ws.get(), xd, lengths, fs, floor, ceil, period, host_threads(),
+                                  timings ? &t : nullptr);
+      }
     }
     py::list out;
     for (auto& r : rs) out.append(py::make_tuple(to_numpy(std::move(r.f0)), to_numpy(std::move(r.times))));

@@ -197,6 +197,8 @@ struct hv_workspace {
   int wintab_h = -1;  // largest half window the window table covers
   long long last_nnz = -1;
   long long last_frames = 0;
+  bool submitted = false;       // an analysis is enqueued and not yet collected
+  long long pending_frames = 0;  // its total frames
   int lds_limit = 64 * 1024;
   // Candidate downloads of the last analysis into h_cand: f0 (cand_ev[0])
   // then scores (cand_ev[1]).  The _async entry points return before they
@@ -468,8 +470,9 @@ void upload_plan(hv_workspace* ws, const Plan& p) {
 }
 
 // Everything after the decimated, mean-removed y is on the device.
-void analyze_device(hv_workspace* ws, const Plan& p, int32_t* offsets, double* best,
-                    int64_t* nnz, double* timings, bool defer = false) {
+// Enqueues the analysis after preprocessing, through the download of offsets,
+// best and the work counters; the host does not wait (analyze_collect does).
+void analyze_submit(hv_workspace* ws, const Plan& p) {
   const Config& c = p.cfg;
   const int U = c.count, B = c.bands;
   hipStream_t s = ws->stream;
@@ -550,6 +553,24 @@ void analyze_device(hv_workspace* ws, const Plan& p, int32_t* offsets, double* b
   check(hipMemcpyAsync(hwork, work, kWorkWords * sizeof(unsigned long long),
                        hipMemcpyDeviceToHost, s), "download counters");
   check(hipEventRecord(ws->ev[3], s), "hipEventRecord");
+  ws->pending_frames = p.total_frames;
+  ws->submitted = true;
+}
+
+// Waits for the submitted analysis, checks its fault flag, starts the
+// candidate download (waited for here unless `defer`) and returns offsets,
+// best and nnz.
+void analyze_collect(hv_workspace* ws, int32_t* offsets, double* best, int64_t* nnz,
+                     double* timings, bool defer) {
+  if (!ws->submitted) invalid("no submitted analysis to collect");
+  ws->submitted = false;
+  const long long frames = ws->pending_frames;
+  hipStream_t s = ws->stream;
+  const int32_t* hoff = ws->h_offsets.get<int32_t>();
+  const double* hbest = ws->h_best.get<double>();
+  unsigned long long* hwork = ws->h_work.get<unsigned long long>();
+  const double* rf0 = ws->rf0.get<double>();
+  const double* rsc = ws->rsc.get<double>();
   check(hipStreamSynchronize(s), "analysis");
   for (int k = 0; k <= HV_STAT_SLOTS; ++k) {  // fold to one entry per counter
     unsigned long long total = 0;
@@ -561,7 +582,7 @@ void analyze_device(hv_workspace* ws, const Plan& p, int32_t* offsets, double* b
   // The candidates go straight to the pinned staging buffer, where the host
   // contour reads them in place (hv_result_view) or callers copy them out.
   // The stream is idle here, so no earlier download still targets h_cand.
-  const long long n = hoff[p.total_frames];
+  const long long n = hoff[frames];
   double* hc = ws->h_cand.ensure<double>(2 * std::max(n, 1LL));
   if (n > 0)
     check(hipMemcpyAsync(hc, rf0, n * sizeof(double), hipMemcpyDeviceToHost, s),
@@ -572,10 +593,10 @@ void analyze_device(hv_workspace* ws, const Plan& p, int32_t* offsets, double* b
           "download candidates");
   check(hipEventRecord(ws->cand_ev[1], s), "hipEventRecord");
   if (!defer) check(hipStreamSynchronize(s), "download candidates");
-  if (offsets) std::memcpy(offsets, hoff, (p.total_frames + 1) * sizeof(int32_t));
-  if (best) std::memcpy(best, hbest, p.total_frames * sizeof(double));
+  if (offsets) std::memcpy(offsets, hoff, (frames + 1) * sizeof(int32_t));
+  if (best) std::memcpy(best, hbest, frames * sizeof(double));
   ws->last_nnz = n;
-  ws->last_frames = p.total_frames;
+  ws->last_frames = frames;
   *nnz = ws->last_nnz;
   if (timings) {
     float ms[3];
@@ -586,6 +607,12 @@ void analyze_device(hv_workspace* ws, const Plan& p, int32_t* offsets, double* b
     collect_timers(ws, timings);
   }
   ws->timing = false;
+}
+
+void analyze_device(hv_workspace* ws, const Plan& p, int32_t* offsets, double* best,
+                    int64_t* nnz, double* timings, bool defer = false) {
+  analyze_submit(ws, p);
+  analyze_collect(ws, offsets, best, nnz, timings, defer);
 }
 
 template <class F> hv_status guarded(F&& body) {
@@ -691,10 +718,22 @@ void run_batch(hv_workspace* ws, int count, const double* const* xs, const int64
                int fs, double f0_floor, double f0_ceil, int32_t* offsets, double* best,
                int64_t* nnz, double* timings, bool device_x, bool defer = false) {
   if (!ws) invalid("null workspace");
+  if (ws->submitted) invalid("the workspace has a submitted analysis to collect first");
   DeviceGuard guard(ws->device);
   const Plan p = preprocess_batch(ws, count, xs, lengths, fs, f0_floor, f0_ceil, timings,
                                   device_x);
   analyze_device(ws, p, offsets, best, nnz, timings, defer);
+}
+
+void submit_batch(hv_workspace* ws, int count, const double* const* xs, const int64_t* lengths,
+                  int fs, double f0_floor, double f0_ceil, bool timings, bool device_x) {
+  if (!ws) invalid("null workspace");
+  if (ws->submitted) invalid("the workspace has a submitted analysis to collect first");
+  DeviceGuard guard(ws->device);
+  double on = 0.0;
+  const Plan p = preprocess_batch(ws, count, xs, lengths, fs, f0_floor, f0_ceil,
+                                  timings ? &on : nullptr, device_x);
+  analyze_submit(ws, p);
 }
 
 // Frozen-decision refinement of entries (value and/or adjoint), then the
@@ -704,6 +743,7 @@ void refine_entries(hv_workspace* ws, int count, const double* const* xs,
                     int64_t n_entries, const int64_t* frame, const double* f0_initial,
                     const double* weight, double* refined, double* grad_x) {
   if (!ws) invalid("null workspace");
+  if (ws->submitted) invalid("the workspace has a submitted analysis to collect first");
   DeviceGuard guard(ws->device);
   if (n_entries < 0) invalid("entry count must be nonnegative");
   if (n_entries > 0 && (!frame || !f0_initial)) invalid("null entry arrays");
@@ -839,6 +879,7 @@ hv_status hv_decimate(hv_workspace* ws, const double* x, int64_t length, int rat
                       int64_t y_length) {
   return guarded([&] {
     if (!ws || !x || !y) invalid("null pointer");
+    if (ws->submitted) invalid("the workspace has a submitted analysis to collect first");
     if (ratio < 1 || ratio > 12) invalid("ratio must be between 1 and 12");
     if (length < 2LL * ratio || length > 0x7fffffffLL) invalid("invalid length");
     if (y_length != (length + ratio - 1) / ratio) invalid("y_length must be ceil(length / ratio)");
@@ -873,6 +914,7 @@ hv_status hv_analyze(hv_workspace* ws, const double* y, int64_t y_length, int ff
                      double* best, int64_t* nnz, double* timings_ms) {
   return guarded([&] {
     if (!ws || !y || !boundaries || !nnz) invalid("null pointer");
+    if (ws->submitted) invalid("the workspace has a submitted analysis to collect first");
     if (y_length < 2 || y_length > 0x7fffffffLL || bands < 3 || frames < 1 || capacity < 1)
       invalid("invalid analysis sizes");
     if (fft_size < y_length || (fft_size & (fft_size - 1)) != 0)
@@ -948,6 +990,37 @@ hv_status hv_analyze_batch_device_async(hv_workspace* ws, int count, const doubl
     const double* xs[1] = {x_device};
     run_batch(ws, count, xs, lengths, fs, f0_floor, f0_ceil, offsets, best, nnz, timings_ms, true,
               true);
+  });
+}
+
+hv_status hv_analyze_batch_submit(hv_workspace* ws, int count, const double* const* xs,
+                                  const int64_t* lengths, int fs, double f0_floor,
+                                  double f0_ceil, int timings) {
+  return guarded([&] {
+    if (!xs || !lengths) invalid("null pointer");
+    for (int u = 0; u < count; ++u)
+      if (!xs[u]) invalid("null waveform pointer");
+    submit_batch(ws, count, xs, lengths, fs, f0_floor, f0_ceil, timings != 0, false);
+  });
+}
+
+hv_status hv_analyze_batch_device_submit(hv_workspace* ws, int count, const double* x_device,
+                                         const int64_t* lengths, int fs, double f0_floor,
+                                         double f0_ceil, int timings) {
+  return guarded([&] {
+    if (!x_device || !lengths) invalid("null pointer");
+    const double* xs[1] = {x_device};
+    submit_batch(ws, count, xs, lengths, fs, f0_floor, f0_ceil, timings != 0, true);
+  });
+}
+
+hv_status hv_analyze_batch_collect(hv_workspace* ws, int32_t* offsets, double* best,
+                                   int64_t* nnz, double* timings_ms) {
+  return guarded([&] {
+    if (!ws || !nnz) invalid("null pointer");
+    if (timings_ms && !ws->timing) invalid("the analysis was submitted without timings");
+    DeviceGuard guard(ws->device);
+    analyze_collect(ws, offsets, best, nnz, timings_ms, true);
   });
 }
 

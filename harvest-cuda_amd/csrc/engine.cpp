@@ -837,10 +837,105 @@ std::vector<Result> batch_common(hv_workspace* ws, Call&& call,
 }
 }  // namespace
 
+int pipeline_groups(const std::vector<int64_t>& frames) {
+  int64_t total = 0;
+  for (int64_t f : frames) total += f;
+  if (frames.size() < 4 || total < 200000) return 1;
+  return (int)std::min<int64_t>({6, (int64_t)frames.size(), total / 50000});
+}
+
+namespace {
+// Groups of consecutive utterances with about equal total frames.
+std::vector<int> group_bounds(const std::vector<int64_t>& frames, int groups) {
+  int64_t total = 0;
+  for (int64_t f : frames) total += f;
+  std::vector<int> bounds{0};
+  int64_t acc = 0;
+  for (int u = 0; u < (int)frames.size(); ++u) {
+    acc += frames[u];
+    const int g = (int)bounds.size();
+    if (g < groups && acc * groups >= total * g && u + 1 < (int)frames.size()) bounds.push_back(u + 1);
+  }
+  bounds.push_back((int)frames.size());
+  return bounds;
+}
+
+// `submit(w, first, n)` enqueues utterances [first, first + n) on workspace w.
+template <class Submit>
+std::vector<Result> batch_pipelined(hv_workspace* ws0, hv_workspace* ws1, Submit&& submit,
+                                    const std::vector<int64_t>& lengths, int fs,
+                                    double f0_floor, double f0_ceil, double frame_period,
+                                    int host_threads) {
+  const int count = (int)lengths.size();
+  std::vector<Params> params(count);
+  std::vector<int64_t> frames(count);
+  for (int u = 0; u < count; ++u) {
+    params[u] = derive_params(lengths[u], fs, f0_floor, f0_ceil, frame_period);
+    frames[u] = params[u].frames;
+  }
+  const std::vector<int> bounds = group_bounds(frames, pipeline_groups(frames));
+  const int groups = (int)bounds.size() - 1;
+  hv_workspace* ws[2] = {ws0, ws1};
+  bool pending[2] = {false, false};
+  std::vector<Result> results(count);
+  auto enqueue = [&](int g) {
+    throw_status(submit(ws[g % 2], bounds[g], bounds[g + 1] - bounds[g]));
+    pending[g % 2] = true;
+  };
+  try {
+    enqueue(0);
+    if (groups > 1) enqueue(1);
+    for (int g = 0; g < groups; ++g) {
+      hv_workspace* w = ws[g % 2];
+      int64_t nnz = 0;
+      pending[g % 2] = false;
+      throw_status(hv_analyze_batch_collect(w, nullptr, nullptr, &nnz, nullptr));
+      const View v = result_view(w);
+      const int first = bounds[g], n = bounds[g + 1] - first;
+      std::vector<int64_t> frame_off(n + 1, 0);
+      for (int k = 0; k < n; ++k) frame_off[k + 1] = frame_off[k] + frames[first + k];
+      std::vector<int> order(n);
+      std::iota(order.begin(), order.end(), 0);
+      std::sort(order.begin(), order.end(),
+                [&](int a, int b) { return frames[first + a] > frames[first + b]; });
+      const int inner = std::max(1, std::min(kSectionThreads, host_threads / std::max(1, n)));
+      parallel_for(n, host_threads, [&](int k) {
+        const int u = order[k];
+        CandidateRows rows{v.offsets + frame_off[u], v.f0, v.scores, w};
+        finish_utterance(rows, params[first + u].frames, v.best + frame_off[u], frame_period,
+                         params[first + u].out_frames, results[first + u], nullptr, nullptr,
+                         nullptr, inner);
+      });
+      // The workspace's pinned results are read: it takes the group after next.
+      if (g + 2 < groups) enqueue(g + 2);
+    }
+  } catch (...) {
+    // Leave both workspaces usable: collect (and drop) what is still enqueued.
+    for (int k = 0; k < 2; ++k) {
+      int64_t nnz = 0;
+      if (pending[k]) (void)hv_analyze_batch_collect(ws[k], nullptr, nullptr, &nnz, nullptr);
+    }
+    throw;
+  }
+  return results;
+}
+}  // namespace
+
 std::vector<Result> analyze_batch(hv_workspace* ws, const std::vector<const double*>& xs,
                                   const std::vector<int64_t>& lengths, int fs,
                                   double f0_floor, double f0_ceil, double frame_period,
-                                  int host_threads, Timings* timings) {
+                                  int host_threads, Timings* timings, hv_workspace* second) {
+  std::vector<int64_t> frames(lengths.size());
+  for (size_t u = 0; u < lengths.size(); ++u)
+    frames[u] = derive_params(lengths[u], fs, f0_floor, f0_ceil, frame_period).frames;
+  if (second && !timings && pipeline_groups(frames) > 1) {
+    auto submit = [&](hv_workspace* w, int first, int n) {
+      return hv_analyze_batch_submit(w, n, xs.data() + first, lengths.data() + first, fs,
+                                     f0_floor, f0_ceil, 0);
+    };
+    return batch_pipelined(ws, second, submit, lengths, fs, f0_floor, f0_ceil, frame_period,
+                           host_threads);
+  }
   auto call = [&](int32_t* offsets, double* best, int64_t* nnz, double* ms) {
     return hv_analyze_batch_async(ws, (int)xs.size(), xs.data(), lengths.data(), fs, f0_floor,
                                   f0_ceil, offsets, best, nnz, ms);
@@ -852,7 +947,21 @@ std::vector<Result> analyze_batch(hv_workspace* ws, const std::vector<const doub
 std::vector<Result> analyze_batch_device(hv_workspace* ws, const double* x_device,
                                          const std::vector<int64_t>& lengths, int fs,
                                          double f0_floor, double f0_ceil, double frame_period,
-                                         int host_threads, Timings* timings) {
+                                         int host_threads, Timings* timings,
+                                         hv_workspace* second) {
+  std::vector<int64_t> frames(lengths.size()), x_off(lengths.size() + 1, 0);
+  for (size_t u = 0; u < lengths.size(); ++u) {
+    frames[u] = derive_params(lengths[u], fs, f0_floor, f0_ceil, frame_period).frames;
+    x_off[u + 1] = x_off[u] + lengths[u];
+  }
+  if (second && !timings && pipeline_groups(frames) > 1) {
+    auto submit = [&](hv_workspace* w, int first, int n) {
+      return hv_analyze_batch_device_submit(w, n, x_device + x_off[first],
+                                            lengths.data() + first, fs, f0_floor, f0_ceil, 0);
+    };
+    return batch_pipelined(ws, second, submit, lengths, fs, f0_floor, f0_ceil, frame_period,
+                           host_threads);
+  }
   auto call = [&](int32_t* offsets, double* best, int64_t* nnz, double* ms) {
     return hv_analyze_batch_device_async(ws, (int)lengths.size(), x_device, lengths.data(), fs,
                                          f0_floor, f0_ceil, offsets, best, nnz, ms);

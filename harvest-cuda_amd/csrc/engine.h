@@ -49,17 +49,27 @@ Result analyze(hv_workspace* ws, const double* x, int64_t length, int fs, double
                double f0_ceil, double frame_period, int host_threads = 1,
                Timings* timings = nullptr);
 // Batched analysis: one device launch sequence for all utterances, then the
-// host contour of every utterance on a pool of host threads.
+// host contour of every utterance on a pool of host threads.  Given a second
+// workspace on the same device, a large batch (see pipeline_groups) is split
+// into groups of consecutive utterances analysed alternately on the two
+// workspaces, so that the host contours and candidate download of one group
+// overlap the device work of the next.  Results do not depend on the split.
 std::vector<Result> analyze_batch(hv_workspace* ws, const std::vector<const double*>& xs,
                                   const std::vector<int64_t>& lengths, int fs,
                                   double f0_floor, double f0_ceil, double frame_period,
-                                  int host_threads, Timings* timings = nullptr);
+                                  int host_threads, Timings* timings = nullptr,
+                                  hv_workspace* second = nullptr);
 
 // Batched analysis of waveforms already in device memory (back to back).
 std::vector<Result> analyze_batch_device(hv_workspace* ws, const double* x_device,
                                          const std::vector<int64_t>& lengths, int fs,
                                          double f0_floor, double f0_ceil, double frame_period,
-                                         int host_threads, Timings* timings = nullptr);
+                                         int host_threads, Timings* timings = nullptr,
+                                         hv_workspace* second = nullptr);
+
+// Number of pipeline groups for a batch: 1 (no split) below 4 utterances or
+// 200 000 frames in total, else up to 6 groups of at least 50 000 frames.
+int pipeline_groups(const std::vector<int64_t>& frames);
 
 // Contour stages, exposed for the bit-exactness tests against WORLD.
 // FixF0Contour (harvest.cpp:1027-1044) from SearchF0Base's output `base`.

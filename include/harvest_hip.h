@@ -142,6 +142,23 @@ hv_status hv_analyze_batch_device_async(hv_workspace* ws, int count, const doubl
                                         double f0_ceil, int32_t* offsets, double* best,
                                         int64_t* nnz, double* timings_ms);
 
+/* Two-step form of hv_analyze_batch_async for pipelines that keep several
+ * workspaces busy: _submit validates the batch and enqueues the whole
+ * analysis on the workspace's stream (through the download of offsets, best
+ * and the work counters) and returns at once; _collect waits for it, starts
+ * the candidate download (as the _async entry points do) and returns
+ * offsets, best and nnz.  Pass timings = 1 to _submit to collect timings_ms.
+ * Until _collect, every other analysis on the workspace fails with
+ * HV_EINVAL.  No reference counterpart. */
+hv_status hv_analyze_batch_submit(hv_workspace* ws, int count, const double* const* xs,
+                                  const int64_t* lengths, int fs, double f0_floor,
+                                  double f0_ceil, int timings);
+hv_status hv_analyze_batch_device_submit(hv_workspace* ws, int count, const double* x_device,
+                                         const int64_t* lengths, int fs, double f0_floor,
+                                         double f0_ceil, int timings);
+hv_status hv_analyze_batch_collect(hv_workspace* ws, int32_t* offsets, double* best,
+                                   int64_t* nnz, double* timings_ms);
+
 /* Wait until the last analysis' candidate f0 (part 0) or f0 and scores
  * (part 1) are in the pinned buffers hv_result_view exposes.  Returns at once
  * after the synchronous entry points.  No reference counterpart. */

@@ -322,5 +322,46 @@ def test_c_abi_async_entry_points_match_sync(gpu):
         np.testing.assert_array_equal(f3, f1)
         np.testing.assert_array_equal(s3, s1)
         assert lib.hv_wait_candidates(ws, 2) == 1  # HV_EINVAL
+        # Two-step form: submit returns at once, analyses wait for collect.
+        lib.hv_analyze_batch_submit.argtypes = [P, ctypes.c_int, P, P, ctypes.c_int, D, D,
+                                                ctypes.c_int]
+        lib.hv_analyze_batch_collect.argtypes = [P, P, P, P, P]
+        assert lib.hv_analyze_batch_submit(ws, 2, ptrs, lens, fs, 71.0, 800.0, 0) == 0
+        nnz = I64()
+        assert lib.hv_analyze_batch(ws, 2, ptrs, lens, fs, 71.0, 800.0, None, None,
+                                    ctypes.byref(nnz), None) == 1  # collect first
+        off4, best4 = np.zeros(frames + 1, np.int32), np.zeros(frames)
+        assert lib.hv_analyze_batch_collect(ws, off4.ctypes.data, best4.ctypes.data,
+                                            ctypes.byref(nnz), None) == 0
+        f4, s4 = fetch(nnz.value)
+        for a, b in ((off1, off4), (best1, best4), (f1, f4), (s1, s4)):
+            np.testing.assert_array_equal(a, b)
+        assert lib.hv_analyze_batch_collect(ws, None, None, ctypes.byref(nnz), None) == 1
     finally:
         lib.hv_workspace_destroy(ws)
+
+
+def test_pipelined_batches_equal_plain_bitwise(gpu):
+    """A batch of >= 4 utterances and >= 200 000 frames is analysed in groups
+    alternating between two workspaces (the host contours of one group
+    overlap the device work of the next); every result still equals the
+    plain call's, bit for bit, for host and device-resident input."""
+    import torch
+    fs = 16000
+    kinds = ["speechlike", "low", "noise", "speechlike", "high", "speechlike", "low"]
+    secs = [41.0, 33.5, 27.0, 45.25, 30.0, 38.0, 12.0]
+    xs = [reference_signal(k, seconds=s, fs=fs) for k, s in zip(kinds, secs)]
+    frames = [int(1000.0 * len(x) / fs) + 1 for x in xs]
+    assert sum(frames) >= 200000
+    plain = [gpu.harvest(x, fs) for x in xs]
+    for _ in range(2):  # the second call reuses both cached workspaces
+        batched = gpu.harvest_batch(xs, fs)
+        for (bf0, bt), (f0, t) in zip(batched, plain):
+            np.testing.assert_array_equal(bf0, f0)
+            np.testing.assert_array_equal(bt, t)
+    flat = torch.from_numpy(np.concatenate(xs)).to("cuda:0")
+    torch.cuda.synchronize()
+    out = gpu.harvest_device(flat.data_ptr(), [len(x) for x in xs], fs)
+    for (df0, dt), (f0, t) in zip(out, plain):
+        np.testing.assert_array_equal(df0, f0)
+        np.testing.assert_array_equal(dt, t)
