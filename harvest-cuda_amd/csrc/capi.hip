@@ -187,6 +187,15 @@ struct hv_workspace {
   DevBuf cinit;                                        // initial f0 of each CSR entry
   DevBuf full, gy, gx, ent_frame, ent_f0, ent_w, ent_out;  // frozen-decision adjoint
   DevBuf coef, coef_off, half, bounds, twiddle, wintab, work;
+  // Filterbank by FFT: (cos, sin)(2 pi t / 4096), the spectra of the FFT
+  // bands [0, fft_bands) (cached with the filters) and of the tiles, and the
+  // blocks' lead D (the longest FFT band's half length).
+  DevBuf tw4096, band_spec, tile_spec;
+  bool tw4096_ready = false;
+  // Two-level phase table of the filterbank's correction (size phase_n).
+  DevBuf phase;
+  long long phase_n = 0;
+  int fft_bands = 0, fft_D = 0;
   PinnedBuf h_x, h_meta, h_offsets, h_best, h_cand, h_work, h_ent, h_gx;
   // Filter-bank cache key (working rate, floor, ceil).
   double key_fs = -1.0, key_floor = -1.0, key_ceil = -1.0;
@@ -323,7 +332,22 @@ void prepare_filters(hv_workspace* ws, double fs_work, double f0_floor,
         "upload filters");
   check(hipMemcpyAsync(dbound, bounds.data(), B * sizeof(double), hipMemcpyHostToDevice,
                        ws->stream), "upload filters");
+  // FFT bands: the leading (longest) bands with h >= kFftMinHalf, when the
+  // longest fits a 4096-point block.
+  int nf = 0;
+  if (halves[0] <= kFftMaxHalf)
+    while (nf < B && halves[nf] >= kFftMinHalf && halves[nf] <= halves[0]) ++nf;
+  if (nf > 0) {
+    if (!ws->tw4096_ready) {
+      launch_twiddle(ws->stream, ws->tw4096.ensure<double>(2 * 4096), 4096);
+      ws->tw4096_ready = true;
+    }
+    launch_band_spectra(ws->stream, Config{}, nf, dcoef, doff, dhalf, ws->tw4096.get<double>(),
+                        ws->band_spec.ensure<double>((size_t)nf * kSpecDoubles));
+  }
   check(hipStreamSynchronize(ws->stream), "upload filters");
+  ws->fft_bands = nf;
+  ws->fft_D = nf > 0 ? halves[0] : 0;
   ws->key_fs = fs_work;
   ws->key_floor = f0_floor;
   ws->key_ceil = f0_ceil;
@@ -337,6 +361,24 @@ void prepare_filters(hv_workspace* ws, double fs_work, double f0_floor,
 // truncation of harvest.cpp:598.
 int refine_hmax(double fs_work, double f0_floor) {
   return (int)(1.5 * fs_work / (f0_floor * (1.0 - 1e-9)) + 1.0) + 1;
+}
+
+// The filterbank's phase table for every fft of the plan (grown, cached):
+// size P = the next power of two >= max(4096, largest fft), 2^12 fine entries.
+void prepare_phase(hv_workspace* ws, Plan& p) {
+  constexpr int kFineBits = 12;
+  long long need = 1LL << kFineBits;
+  for (const Utt& u : p.utts) need = std::max(need, (long long)u.fft);
+  long long P = 1LL << kFineBits;
+  while (P < need) P <<= 1;
+  if (P > ws->phase_n) {
+    launch_phase_table(ws->stream,
+                       ws->phase.ensure<double>(2 * ((1LL << kFineBits) + (P >> kFineBits))), P,
+                       kFineBits);
+    ws->phase_n = P;
+  }
+  p.cfg.phase_n = ws->phase_n;
+  p.cfg.phase_bits = kFineBits;
 }
 
 void prepare_twiddles(hv_workspace* ws, double fs_work, double f0_floor) {
@@ -495,13 +537,28 @@ void analyze_submit(hv_workspace* ws, const Plan& p) {
   int* offs = ws->offs.ensure<int>((size_t)group * 4 * p.total_tiles);
   int* totals = ws->totals.ensure<int>((size_t)group * 4 * U);
   double* raw = ws->raw.ensure<double>((size_t)B * p.total_frames);
+  // The long bands [0, nfft) by FFT (one spectrum per tile, shared by the
+  // bands), the others by the direct FIR.
+  const int nfft = ws->fft_bands, lead = ws->fft_D;
+  double* tspec = nullptr;
+  if (nfft > 0) {
+    tspec = ws->tile_spec.ensure<double>((size_t)p.total_tiles * kSpecDoubles);
+    timed(ws, 4, [&] {
+      launch_tile_spectra(s, c, dutt, ws->d_t_off, p.total_tiles, lead, y,
+                          ws->tw4096.get<double>(), tspec);
+    });
+  }
   for (int b0 = 0; b0 < B; b0 += group) {
     const int g = std::min(group, B - b0);
+    const int nf = std::max(0, std::min(g, nfft - b0));
     int mh = 0;
-    for (int b = b0; b < b0 + g; ++b) mh = std::max(mh, ws->halves[b]);
+    for (int b = b0 + nf; b < b0 + g; ++b) mh = std::max(mh, ws->halves[b]);
     timed(ws, 4, [&] {
-      launch_filter_events(s, c, dutt, ws->d_t_off, p.total_tiles, b0, g, mh, y,
-                           coef, coef_off, half, quirk, slab, counts);
+      launch_filter_fft(s, c, dutt, ws->d_t_off, p.total_tiles, b0, nf, 0, lead, tspec,
+                        ws->band_spec.get<double>(), half, quirk, ws->tw4096.get<double>(),
+                        ws->phase.get<double>(), slab, counts);
+      launch_filter_events(s, c, dutt, ws->d_t_off, p.total_tiles, b0 + nf, g - nf, nf, mh, y,
+                           coef, coef_off, half, quirk, ws->phase.get<double>(), slab, counts);
     });
     timed(ws, 5, [&] {
       launch_tile_scan(s, c, dutt, p.total_tiles, g, counts, offs, totals);
@@ -678,6 +735,7 @@ Plan preprocess_batch(hv_workspace* ws, int count, const double* const* xs,
               ws->twiddle_n);
   p.cfg.lds_limit = ws->lds_limit;
   layout(p, lens, ns, ffts, frames);
+  prepare_phase(ws, p);
   hipStream_t s = ws->stream;
   start_timers(ws, timings != nullptr);
   check(hipEventRecord(ws->ev[0], s), "hipEventRecord");
@@ -933,6 +991,7 @@ hv_status hv_analyze(hv_workspace* ws, const double* y, int64_t y_length, int ff
     p.cfg.lds_limit = ws->lds_limit;
     p.cfg.lag = 0;
     layout(p, {y_length}, {(int)y_length}, {fft_size}, {frames});
+    prepare_phase(ws, p);
     hipStream_t s = ws->stream;
     start_timers(ws, timings_ms != nullptr);
     check(hipEventRecord(ws->ev[0], s), "hipEventRecord");

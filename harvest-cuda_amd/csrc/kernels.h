@@ -40,10 +40,31 @@ void launch_window_table(hipStream_t s, double* tab, int hmax);
 // Filterbank + crossing detection for bands [band0, band0 + group).
 void launch_filter_events(hipStream_t s, const Config& c, const Utt* utt,
                           const long long* t_off, long long total_tiles,
-                          int band0, int group, int max_half, const double* y,
+                          int band0, int count, int slot0, int max_half, const double* y,
                           const double* coef, const int* coef_off,
-                          const int* half, const Quirk* quirk, double* slab,
-                          int* counts);
+                          const int* half, const Quirk* quirk, const double* phase,
+                          double* slab, int* counts);
+// Filterbank by FFT for bands [band0, band0 + count) (slab slots from slot0):
+// tile spectra of 4096-sample blocks starting D samples before each tile,
+// band filter spectra, and the per-(tile, band) inverse transforms.  Spectra
+// hold kSpecDoubles doubles each; tw4096 holds (cos, sin)(2 pi t / 4096).
+constexpr int kSpecDoubles = 2 * 2050;
+constexpr int kFftMaxHalf = 1000;  // FFT bands need D + h <= 2047
+constexpr int kFftMinHalf = 64;    // shorter bands run the direct FIR (cheaper there)
+void launch_tile_spectra(hipStream_t s, const Config& c, const Utt* utt, const long long* t_off,
+                         long long total_tiles, int D, const double* y, const double* tw4096,
+                         double* spec);
+void launch_band_spectra(hipStream_t s, const Config& c, int bands, const double* coef,
+                         const int* coef_off, const int* half, const double* tw4096,
+                         double* spec);
+void launch_filter_fft(hipStream_t s, const Config& c, const Utt* utt, const long long* t_off,
+                       long long total_tiles, int band0, int count, int slot0, int D,
+                       const double* tile_spec, const double* band_spec, const int* half,
+                       const Quirk* quirk, const double* tw4096, const double* phase,
+                       double* slab, int* counts);
+// Two-level table of e^{2 pi i k / P} (P a power of two): 2^fine_bits fine
+// entries then P / 2^fine_bits coarse ones, (cos, sin) each.
+void launch_phase_table(hipStream_t s, double* tab, long long P, int fine_bits);
 void launch_tile_scan(hipStream_t s, const Config& c, const Utt* utt,
                       long long total_tiles, int group, const int* counts,
                       int* offs, int* totals);

@@ -16,6 +16,10 @@
 #include "kernels.h"
 
 namespace hv {
+
+// LDS-qualified pointers: accesses through them compile to ds_* operations
+// however the pointers are passed around.
+#define LDS __attribute__((address_space(3)))
 namespace {
 
 void check_launch(const char* what) {
@@ -498,6 +502,21 @@ __global__ __launch_bounds__(kThreads) void k_window_table(double* tab, int hmax
   tab[2 * id + 1] = sn;
 }
 
+// Two-level phase table of size P = 2^bits_total: fine entries
+// (cos, sin)(2 pi a / P), a < 2^fine_bits, then coarse entries
+// (cos, sin)(2 pi b 2^fine_bits / P), b < P / 2^fine_bits.
+__global__ __launch_bounds__(kThreads) void k_phase_table(double* tab, long long P,
+                                                          int fine_bits) {
+  const long long id = (long long)blockIdx.x * kThreads + threadIdx.x;
+  const long long F = 1LL << fine_bits;
+  if (id >= F + P / F) return;
+  const long long q = id < F ? id : (id - F) * F;
+  double sn, cs;
+  sincospi(2.0 * (double)q / (double)P, &sn, &cs);
+  tab[2 * id] = cs;
+  tab[2 * id + 1] = sn;
+}
+
 __global__ __launch_bounds__(kThreads) void k_fill_int(int* p, long long n, int v) {
   long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
   if (i < n) p[i] = v;
@@ -523,82 +542,38 @@ __device__ __forceinline__ int pad8(int s) { return s + (s >> 3); }
 
 __host__ __device__ inline size_t pad8_size(int n) { return (size_t)n + (size_t)(n >> 3) + 1; }
 
-template <bool kLds>
-__global__ __launch_bounds__(kThreads) void k_filter_events(
-    Config c, const Utt* utt, const long long* t_off, long long total_tiles,
-    int band0, const double* y, const double* coef, const int* coef_off,
-    const int* half, const Quirk* quirk, double* slab, int* counts) {
-  extern __shared__ double smem[];
-  __shared__ unsigned long long wave_sums[kThreads / 64];
-  const long long g = blockIdx.x;
-  const int bg = blockIdx.y;
-  const int b = band0 + bg;
-  const int ui = find_segment(t_off, c.count, g);
-  const Utt u = utt[ui];
-  const long long i0 = (g - u.t_off) * kTileStride;
-  const int N = u.n;
-  const int h = half[b];
-  const int taps8 = (2 * h + 1 + 7) & ~7;
-  const double* cr = coef + coef_off[b];
-  const double* yu = y + u.y_off;
-  const long long base = i0 - h + 1;
-  const int span = kTile + taps8 + 10;
-  if (kLds) {
-    for (int s = threadIdx.x; s < span; s += kThreads) {
-      long long idx = base + s;
-      smem[pad8(s)] = (idx >= 0 && idx < N) ? yu[idx] : 0.0;
-    }
-    __syncthreads();
-  }
-  auto Y = [&](int s) -> double {
-    if (kLds) return smem[pad8(s)];
-    long long idx = base + s;
-    return (idx >= 0 && idx < N) ? yu[idx] : 0.0;
-  };
-
-  // acc[r] = sum over taps mm in increasing order of cr[mm] y[j0 + r + mm].
+// The two-bin correction, crossing detection and slab write of one tile of
+// one band.  acc[r] is this thread's FIR output j0 + r (j0 = 8 threadIdx.x);
+// f is the workgroup's LDS array of pad8_size(kTile) doubles, free when
+// called.
+__device__ void tile_crossings(const Config& c, const Utt& u, const Quirk& qk, long long g,
+                               int bg, long long total_tiles, long long i0, int h,
+                               const double (&acc)[8], double* f, const double2* phase,
+                               unsigned long long* wave_sums, double* slab, int* counts) {
   const int j0 = threadIdx.x * 8;
-  double acc[8], va[8], vb[8];
-#pragma unroll
-  for (int r = 0; r < 8; ++r) {
-    acc[r] = 0.0;
-    va[r] = Y(j0 + r);
-  }
-  for (int mm = 0; mm < taps8; mm += 16) {
-#pragma unroll
-    for (int q = 0; q < 8; ++q) vb[q] = Y(j0 + mm + 8 + q);
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const double cc = cr[mm + k];
-#pragma unroll
-      for (int r = 0; r < 8; ++r) acc[r] = fma(cc, r + k < 8 ? va[r + k] : vb[r + k - 8], acc[r]);
-    }
-    if (mm + 8 >= taps8) break;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) va[q] = Y(j0 + mm + 16 + q);
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const double cc = cr[mm + 8 + k];
-#pragma unroll
-      for (int r = 0; r < 8; ++r) acc[r] = fma(cc, r + k < 8 ? vb[r + k] : va[r + k - 8], acc[r]);
-    }
-  }
+  const int N = u.n;
   // Two-bin correction (-1)^k [2 Re(D1 e^{-2 pi i k / n}) + D2] / n at
-  // k = i0 + j + h + 1: the exact twiddle of the thread's first output, then
-  // rotation by 2 pi / n per output.
-  const Quirk qk = quirk[(long long)ui * c.bands + b];
+  // k = i0 + j + h + 1: the twiddle of the thread's first output from the
+  // two-level phase table (fine entries e^{2 pi i a / P}, a < 2^phase_bits,
+  // then coarse entries e^{2 pi i b 2^phase_bits / P}; P = phase_n is a power
+  // of two no smaller than n), then rotation by 2 pi / n per output.
   const double inv_n = 1.0 / u.fft;
   const long long nmask = u.fft - 1;
   const long long k0 = i0 + j0 + h + 1;
+  const long long fine = (1LL << c.phase_bits) - 1;
+  auto phase_at = [&](long long k, double& cs, double& sn) {  // (cos, sin)(2 pi k / n)
+    const long long kk = (k & nmask) * (c.phase_n / u.fft);
+    const double2 a = phase[kk & fine], b = phase[fine + 1 + (kk >> c.phase_bits)];
+    cs = b.x * a.x - b.y * a.y;
+    sn = b.x * a.y + b.y * a.x;
+  };
   double tsn, tcs, rsn, rcs;
-  sincospi(2.0 * (double)(k0 & nmask) / u.fft, &tsn, &tcs);
-  sincospi(2.0 / u.fft, &rsn, &rcs);
+  phase_at(k0, tcs, tsn);
+  phase_at(1, rcs, rsn);
   auto correction_at = [&](long long k, double cs, double sn) -> double {
     const double q = 2.0 * (qk.d1re * cs + qk.d1im * sn) + qk.d2;
     return (k & 1) ? -q : q;
   };
-  if (kLds) __syncthreads();
-  double* f = smem;
 #pragma unroll
   for (int r = 0; r < 8; ++r) {
     f[pad8(j0 + r)] = acc[r] + correction_at(k0 + r, tcs, tsn) * inv_n;
@@ -657,6 +632,358 @@ __global__ __launch_bounds__(kThreads) void k_filter_events(
   if (threadIdx.x < 4)
     counts[(long long)(bg * 4 + threadIdx.x) * total_tiles + g] =
         (int)((total >> (16 * threadIdx.x)) & 0xffffu);
+}
+
+template <bool kLds>
+__global__ __launch_bounds__(kThreads) void k_filter_events(
+    Config c, const Utt* utt, const long long* t_off, long long total_tiles,
+    int band0, int slot0, const double* y, const double* coef, const int* coef_off,
+    const int* half, const Quirk* quirk, const double2* phase, double* slab, int* counts) {
+  extern __shared__ double smem[];
+  __shared__ unsigned long long wave_sums[kThreads / 64];
+  const long long g = blockIdx.x;
+  const int bg = slot0 + blockIdx.y;  // the band's slab slot within its group
+  const int b = band0 + blockIdx.y;
+  const int ui = find_segment(t_off, c.count, g);
+  const Utt u = utt[ui];
+  const long long i0 = (g - u.t_off) * kTileStride;
+  const int N = u.n;
+  const int h = half[b];
+  const int taps8 = (2 * h + 1 + 7) & ~7;
+  const double* cr = coef + coef_off[b];
+  const double* yu = y + u.y_off;
+  const long long base = i0 - h + 1;
+  const int span = kTile + taps8 + 10;
+  if (kLds) {
+    for (int s = threadIdx.x; s < span; s += kThreads) {
+      long long idx = base + s;
+      smem[pad8(s)] = (idx >= 0 && idx < N) ? yu[idx] : 0.0;
+    }
+    __syncthreads();
+  }
+  auto Y = [&](int s) -> double {
+    if (kLds) return smem[pad8(s)];
+    long long idx = base + s;
+    return (idx >= 0 && idx < N) ? yu[idx] : 0.0;
+  };
+
+  // acc[r] = sum over taps mm in increasing order of cr[mm] y[j0 + r + mm].
+  const int j0 = threadIdx.x * 8;
+  double acc[8], va[8], vb[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    acc[r] = 0.0;
+    va[r] = Y(j0 + r);
+  }
+  for (int mm = 0; mm < taps8; mm += 16) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) vb[q] = Y(j0 + mm + 8 + q);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const double cc = cr[mm + k];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) acc[r] = fma(cc, r + k < 8 ? va[r + k] : vb[r + k - 8], acc[r]);
+    }
+    if (mm + 8 >= taps8) break;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) va[q] = Y(j0 + mm + 16 + q);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const double cc = cr[mm + 8 + k];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) acc[r] = fma(cc, r + k < 8 ? vb[r + k] : va[r + k - 8], acc[r]);
+    }
+  }
+  const Quirk qk = quirk[(long long)ui * c.bands + b];
+  if (kLds) __syncthreads();  // the staged samples are read: smem takes the outputs
+  tile_crossings(c, u, qk, g, bg, total_tiles, i0, h, acc, smem, phase, wave_sums, slab, counts);
+}
+
+// ---------------------------------------------------------------------------
+// Filterbank by FFT for the long bands (overlap-save).  Tile g's input block
+// is x[j] = y[i0 - D + j], j < 4096 (zero outside the utterance), with D the
+// longest FFT band's half length.  For band b,
+//   f[i0 + j] = (x * c_b)[j + D + h_b + 1]          (linear convolution)
+// with no circular wrap for j < kTile since D >= h_b - 1 and D + h_b <= 2047.
+// It is the real inverse FFT of X_g C_b at size 4096, the block spectrum
+// shared by every band of the tile.  Real transforms of size 4096 run as
+// complex transforms of size 2048 of the even/odd samples packed as
+// re + i im, with the usual split/merge twiddles e^{-+2 pi i k / 4096}.
+// Results equal the direct FIR up to rounding; every tile is computed the
+// same way whatever the batch.
+
+// v[r] = sum_m v[m] e^{SIGN 2 pi i m r / 8}, in place.
+template <int SIGN>
+__device__ __forceinline__ void dft8(double* re, double* im) {
+  constexpr double h = 0.70710678118654752440;
+  double ar[8], ai[8];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    ar[m] = re[m] + re[m + 4];
+    ai[m] = im[m] + im[m + 4];
+    ar[m + 4] = re[m] - re[m + 4];
+    ai[m + 4] = im[m] - im[m + 4];
+  }
+  {  // the difference half times e^{SIGN 2 pi i m / 8}, m = 1, 2, 3
+    const double r1 = ar[5], i1 = ai[5];
+    ar[5] = h * (r1 - SIGN * i1);
+    ai[5] = h * (i1 + SIGN * r1);
+    const double r2 = ar[6], i2 = ai[6];
+    ar[6] = -SIGN * i2;
+    ai[6] = SIGN * r2;
+    const double r3 = ar[7], i3 = ai[7];
+    ar[7] = h * (-r3 - SIGN * i3);
+    ai[7] = h * (-i3 + SIGN * r3);
+  }
+#pragma unroll
+  for (int base = 0; base < 8; base += 4) {  // two 4-point DFTs: even and odd r
+    const double b0r = ar[base] + ar[base + 2], b0i = ai[base] + ai[base + 2];
+    const double b1r = ar[base] - ar[base + 2], b1i = ai[base] - ai[base + 2];
+    const double b2r = ar[base + 1] + ar[base + 3], b2i = ai[base + 1] + ai[base + 3];
+    const double b3r = ar[base + 1] - ar[base + 3], b3i = ai[base + 1] - ai[base + 3];
+    const double t3r = -SIGN * b3i, t3i = SIGN * b3r;
+    ar[base] = b0r + b2r;
+    ai[base] = b0i + b2i;
+    ar[base + 1] = b1r + t3r;
+    ai[base + 1] = b1i + t3i;
+    ar[base + 2] = b0r - b2r;
+    ai[base + 2] = b0i - b2i;
+    ar[base + 3] = b1r - t3r;
+    ai[base + 3] = b1i - t3i;
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    re[2 * q] = ar[q];
+    im[2 * q] = ai[q];
+    re[2 * q + 1] = ar[4 + q];
+    im[2 * q + 1] = ai[4 + q];
+  }
+}
+
+// v[r] = sum_m v[m] e^{SIGN 2 pi i m r / 4}, in place.
+template <int SIGN>
+__device__ __forceinline__ void dft4(double* re, double* im) {
+  const double b0r = re[0] + re[2], b0i = im[0] + im[2];
+  const double b1r = re[0] - re[2], b1i = im[0] - im[2];
+  const double b2r = re[1] + re[3], b2i = im[1] + im[3];
+  const double b3r = re[1] - re[3], b3i = im[1] - im[3];
+  const double t3r = -SIGN * b3i, t3i = SIGN * b3r;
+  re[0] = b0r + b2r;
+  im[0] = b0i + b2i;
+  re[1] = b1r + t3r;
+  im[1] = b1i + t3i;
+  re[2] = b0r - b2r;
+  im[2] = b0i - b2i;
+  re[3] = b1r - t3r;
+  im[3] = b1i - t3i;
+}
+
+constexpr int kFftM = 2048;        // complex transform size (real size 4096)
+constexpr int kSpecStride = 2050;  // complex entries per stored spectrum (k = 0..2048)
+
+// Element i of a transform is stored at LDS index sw(i): bits 5-9 XOR-ed into
+// bits 0-4, which spreads the butterflies' strided accesses over the banks.
+__device__ __forceinline__ int sw(int i) { return i ^ ((i >> 5) & 31); }
+
+// Position of frequency k after fft2048: k = r1 + 8 r2 + 64 r3 + 512 r4
+// (r4 < 4) lands at element r1 256 + r2 32 + r3 4 + r4.
+__device__ __forceinline__ int dig_rev(int k) {
+  return ((k & 7) << 8) | (((k >> 3) & 7) << 5) | (((k >> 6) & 7) << 2) | (k >> 9);
+}
+
+// This thread's twiddles W_L^{j r}, r = 1, 2, 4, of the three radix-8 passes
+// of fft2048 (as e^{+i}: the inverse sign), loaded ahead of the transform so
+// that their latency overlaps the caller's input stage; r = 3, 5, 6, 7 are
+// their products.
+struct FftTwiddles {
+  double2 w[3][3];
+};
+__device__ __forceinline__ FftTwiddles load_fft_twiddles(const double2* tw) {
+  FftTwiddles f;
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int pass = 0; pass < 3; ++pass) {
+    const int L = kFftM >> (3 * pass), S = L >> 3;
+    const int step = (t % S) * (2 * kFftM / L);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) f.w[pass][q] = tw[((1 << q) * step) & (2 * kFftM - 1)];
+  }
+  return f;
+}
+
+__device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// Unscaled in-place DIF FFT of 2048 complex values (re, im in LDS, element
+// i at sw(i)) by radix 8, 8, 8, 4, on 256 threads; output in dig_rev order.
+// SIGN -1 forward, +1 inverse.  Twiddles from load_fft_twiddles (tables of
+// (cos, sin)(2 pi t / 4096)).  Ends with a barrier.
+template <int SIGN>
+__device__ void fft2048(LDS double* re, LDS double* im, const FftTwiddles& tws) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int pass = 0; pass < 3; ++pass) {
+    const int L = kFftM >> (3 * pass), S = L >> 3;  // sub-transform length, butterfly stride
+    const int j = t % S;
+    const int base = (t / S) * L + j;
+    double vr[8], vi[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      vr[m] = re[sw(base + S * m)];
+      vi[m] = im[sw(base + S * m)];
+    }
+    dft8<SIGN>(vr, vi);
+    double2 wr[8];
+    wr[1] = tws.w[pass][0];
+    wr[2] = tws.w[pass][1];
+    wr[4] = tws.w[pass][2];
+    wr[3] = cmul2(wr[1], wr[2]);
+    wr[5] = cmul2(wr[1], wr[4]);
+    wr[6] = cmul2(wr[2], wr[4]);
+    wr[7] = cmul2(wr[3], wr[4]);
+#pragma unroll
+    for (int r = 1; r < 8; ++r) {
+      const double2 w = wr[r];
+      const double wi = SIGN * w.y;
+      const double xr = vr[r] * w.x - vi[r] * wi, xi = vr[r] * wi + vi[r] * w.x;
+      vr[r] = xr;
+      vi[r] = xi;
+    }
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      re[sw(base + S * m)] = vr[m];
+      im[sw(base + S * m)] = vi[m];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {  // radix 4, stride 1 (no twiddles)
+    const int base = 4 * (t + kThreads * q);
+    double vr[4], vi[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      vr[m] = re[sw(base + m)];
+      vi[m] = im[sw(base + m)];
+    }
+    dft4<SIGN>(vr, vi);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      re[sw(base + m)] = vr[m];
+      im[sw(base + m)] = vi[m];
+    }
+  }
+  __syncthreads();
+}
+
+// Spectrum X[k], k = 0..2048, of a real block of 4096 samples: a tile of y
+// (kTileInput) or a band filter c_b[t] = cr_b[2 h_b - t] zero-padded.  One
+// workgroup per tile or band (FFT bands [0, gridDim.x)).
+template <bool kTileInput>
+__global__ __launch_bounds__(kThreads) void k_spectrum(Config c, const Utt* utt,
+                                                       const long long* t_off, int D,
+                                                       const double* y, const double* coef,
+                                                       const int* coef_off, const int* half,
+                                                       const double2* tw, double2* spec) {
+  __shared__ double buf[2 * kFftM];
+  LDS double* re = (LDS double*)buf;
+  LDS double* im = re + kFftM;
+  const long long id = blockIdx.x;
+  const int t = threadIdx.x;
+  const FftTwiddles tws = load_fft_twiddles(tw);
+  if (kTileInput) {
+    const Utt u = utt[find_segment(t_off, c.count, id)];
+    const long long first = (id - u.t_off) * kTileStride - D;
+    const double* yu = y + u.y_off;
+    for (int n = t; n < kFftM; n += kThreads) {
+      const long long a = first + 2 * n, b = a + 1;
+      re[sw(n)] = a >= 0 && a < u.n ? yu[a] : 0.0;
+      im[sw(n)] = b >= 0 && b < u.n ? yu[b] : 0.0;
+    }
+  } else {
+    const double* cr = coef + coef_off[id];
+    const int h2 = 2 * half[id];
+    for (int n = t; n < kFftM; n += kThreads) {
+      re[sw(n)] = 2 * n <= h2 ? cr[h2 - 2 * n] : 0.0;
+      im[sw(n)] = 2 * n + 1 <= h2 ? cr[h2 - 2 * n - 1] : 0.0;
+    }
+  }
+  __syncthreads();
+  fft2048<-1>(re, im, tws);
+  // X[k] = E[k] + e^{-2 pi i k / 4096} O[k] with E = (C[k] + conj C[M - k]) / 2
+  // and O = -i (C[k] - conj C[M - k]) / 2.
+  double2* out = spec + id * kSpecStride;
+  for (int k = t; k <= kFftM; k += kThreads) {
+    const int p1 = sw(dig_rev(k & (kFftM - 1))), p2 = sw(dig_rev((kFftM - k) & (kFftM - 1)));
+    const double ckr = re[p1], cki = im[p1], cmr = re[p2], cmi = -im[p2];
+    const double er = 0.5 * (ckr + cmr), ei = 0.5 * (cki + cmi);
+    const double dr = ckr - cmr, di = cki - cmi;
+    const double orr = 0.5 * di, oi = -0.5 * dr;
+    const double2 w = tw[k];  // e^{-2 pi i k / 4096} = (w.x, -w.y)
+    out[k] = make_double2(er + (w.x * orr + w.y * oi), ei + (w.x * oi - w.y * orr));
+  }
+}
+
+// One workgroup per (tile, FFT band), the band varying fastest so that the
+// workgroups of a tile share its spectrum in L2: the band's 2048 outputs of
+// the tile by the inverse transform of X_g C_b, then the shared crossing stage.
+__global__ __launch_bounds__(kThreads) void k_filter_fft(
+    Config c, const Utt* utt, const long long* t_off, long long total_tiles, int band0,
+    int nbands, int slot0, int D, const double2* tile_spec, const double2* band_spec,
+    const int* half, const Quirk* quirk, const double2* tw, const double2* phase, double* slab,
+    int* counts) {
+  __shared__ double buf[2 * kFftM];
+  __shared__ unsigned long long wave_sums[kThreads / 64];
+  LDS double* re = (LDS double*)buf;
+  LDS double* im = re + kFftM;
+  const long long g = blockIdx.x / nbands;
+  const int bi = (int)(blockIdx.x % nbands);
+  const int bg = slot0 + bi;
+  const int b = band0 + bi;
+  const int ui = find_segment(t_off, c.count, g);
+  const Utt u = utt[ui];
+  const long long i0 = (g - u.t_off) * kTileStride;
+  const int h = half[b];
+  const int t = threadIdx.x;
+  const double2* X = tile_spec + g * kSpecStride;
+  const double2* C = band_spec + (long long)b * kSpecStride;
+  const FftTwiddles tws = load_fft_twiddles(tw);
+  // W[k] = A + i B, A = (Z[k] + conj Z[M - k]) / 2,
+  // B = (Z[k] - conj Z[M - k]) / 2 e^{2 pi i k / 4096}, Z = X C; k and M - k
+  // share their loads (k = 0 pairs with the Nyquist entry M, k = M / 2 with
+  // itself).
+  auto emit = [&](int k, double2 zk, double2 zm) {  // W[k] from Z[k] and Z[M - k]
+    const double ar = 0.5 * (zk.x + zm.x), ai = 0.5 * (zk.y - zm.y);
+    const double dr = 0.5 * (zk.x - zm.x), di = 0.5 * (zk.y + zm.y);
+    const double2 w = tw[k];
+    const double br = dr * w.x - di * w.y, bi = dr * w.y + di * w.x;
+    re[sw(k)] = ar - bi;
+    im[sw(k)] = ai + br;
+  };
+#pragma unroll
+  for (int q = 0; q < kFftM / 2 / kThreads; ++q) {
+    const int k = t + kThreads * q, m = kFftM - k;  // k < M / 2
+    const double2 zk = cmul2(X[k], C[k]), zm = cmul2(X[m], C[m]);
+    emit(k, zk, zm);
+    if (k > 0) emit(m, zm, zk);
+  }
+  if (t == 0) {
+    const double2 zh = cmul2(X[kFftM / 2], C[kFftM / 2]);
+    emit(kFftM / 2, zh, zh);
+  }
+  __syncthreads();
+  fft2048<1>(re, im, tws);
+  // z[s] = (s even ? Re : Im) w[s / 2] / 2048; output j is z[j + D + h + 1].
+  double acc[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int s = 8 * t + r + D + h + 1;
+    const int p = sw(dig_rev(s >> 1));
+    acc[r] = ((s & 1) ? im[p] : re[p]) * (1.0 / kFftM);
+  }
+  __syncthreads();  // every value is read: buf takes the outputs
+  const Quirk qk = quirk[(long long)ui * c.bands + b];
+  tile_crossings(c, u, qk, g, bg, total_tiles, i0, h, acc, buf, phase, wave_sums, slab, counts);
 }
 
 // Exclusive scan of per-tile crossing counts within each utterance, for every
@@ -911,9 +1238,6 @@ constexpr int kMaxWords = 128;   // bitmap words (bins 0..8191)
 constexpr int kXcds = 8;         // XCDs that workgroups are dispatched to in turn
 constexpr int kYPre = 6;         // 64-sample chunks of the next window loaded ahead
 
-// LDS-qualified pointers: accesses through them compile to ds_* operations
-// however the pointers are passed around.
-#define LDS __attribute__((address_space(3)))
 
 // One wave's part of k_refine_waves' dynamic LDS, in carve order:
 //   f0     [S]          candidate per slot
@@ -1873,6 +2197,12 @@ void launch_twiddle(hipStream_t s, double* tw, int n) {
   check_launch("twiddle");
 }
 
+void launch_phase_table(hipStream_t s, double* tab, long long P, int fine_bits) {
+  const long long n = (1LL << fine_bits) + (P >> fine_bits);
+  k_phase_table<<<grid_for(n), kThreads, 0, s>>>(tab, P, fine_bits);
+  check_launch("phase_table");
+}
+
 void launch_window_table(hipStream_t s, double* tab, int hmax) {
   k_window_table<<<grid_for(65LL * (hmax + 1)), kThreads, 0, s>>>(tab, hmax);
   check_launch("window_table");
@@ -1886,21 +2216,58 @@ void launch_fill_int(hipStream_t s, int* p, long long n, int value) {
 
 void launch_filter_events(hipStream_t s, const Config& c, const Utt* utt,
                           const long long* t_off, long long total_tiles,
-                          int band0, int group, int max_half, const double* y,
+                          int band0, int count, int slot0, int max_half, const double* y,
                           const double* coef, const int* coef_off,
-                          const int* half, const Quirk* quirk, double* slab,
-                          int* counts) {
+                          const int* half, const Quirk* quirk, const double* phase,
+                          double* slab, int* counts) {
+  if (count <= 0) return;
+  const double2* ph = reinterpret_cast<const double2*>(phase);
   const int taps8 = (2 * max_half + 1 + 7) & ~7;
   const size_t lds_bytes = sizeof(double) * pad8_size(kTile + taps8 + 10);
-  dim3 grid((unsigned)total_tiles, (unsigned)group);
+  dim3 grid((unsigned)total_tiles, (unsigned)count);
   if (lds_bytes <= 64 * 1024) {
     k_filter_events<true><<<grid, kThreads, lds_bytes, s>>>(
-        c, utt, t_off, total_tiles, band0, y, coef, coef_off, half, quirk, slab, counts);
+        c, utt, t_off, total_tiles, band0, slot0, y, coef, coef_off, half, quirk, ph, slab,
+        counts);
   } else {
     k_filter_events<false><<<grid, kThreads, sizeof(double) * pad8_size(kTile), s>>>(
-        c, utt, t_off, total_tiles, band0, y, coef, coef_off, half, quirk, slab, counts);
+        c, utt, t_off, total_tiles, band0, slot0, y, coef, coef_off, half, quirk, ph, slab,
+        counts);
   }
   check_launch("filter_events");
+}
+
+void launch_tile_spectra(hipStream_t s, const Config& c, const Utt* utt, const long long* t_off,
+                         long long total_tiles, int D, const double* y, const double* tw4096,
+                         double* spec) {
+  k_spectrum<true><<<(unsigned)total_tiles, kThreads, 0, s>>>(
+      c, utt, t_off, D, y, nullptr, nullptr, nullptr, reinterpret_cast<const double2*>(tw4096),
+      reinterpret_cast<double2*>(spec));
+  check_launch("tile_spectra");
+}
+
+void launch_band_spectra(hipStream_t s, const Config& c, int bands, const double* coef,
+                         const int* coef_off, const int* half, const double* tw4096,
+                         double* spec) {
+  if (bands <= 0) return;
+  k_spectrum<false><<<(unsigned)bands, kThreads, 0, s>>>(
+      c, nullptr, nullptr, 0, nullptr, coef, coef_off, half,
+      reinterpret_cast<const double2*>(tw4096), reinterpret_cast<double2*>(spec));
+  check_launch("band_spectra");
+}
+
+void launch_filter_fft(hipStream_t s, const Config& c, const Utt* utt, const long long* t_off,
+                       long long total_tiles, int band0, int count, int slot0, int D,
+                       const double* tile_spec, const double* band_spec, const int* half,
+                       const Quirk* quirk, const double* tw4096, const double* phase,
+                       double* slab, int* counts) {
+  if (count <= 0) return;
+  k_filter_fft<<<(unsigned)(total_tiles * count), kThreads, 0, s>>>(
+      c, utt, t_off, total_tiles, band0, count, slot0, D,
+      reinterpret_cast<const double2*>(tile_spec), reinterpret_cast<const double2*>(band_spec),
+      half, quirk, reinterpret_cast<const double2*>(tw4096),
+      reinterpret_cast<const double2*>(phase), slab, counts);
+  check_launch("filter_fft");
 }
 
 void launch_tile_scan(hipStream_t s, const Config& c, const Utt* utt,
