@@ -365,3 +365,21 @@ def test_pipelined_batches_equal_plain_bitwise(gpu):
     for (df0, dt), (f0, t) in zip(out, plain):
         np.testing.assert_array_equal(df0, f0)
         np.testing.assert_array_equal(dt, t)
+
+
+@pytest.mark.parametrize("floor,fs", [(30.0, 16000), (14.0, 16000), (71.0, 8000), (71.0, 96000)])
+def test_filterbank_fft_and_direct_bands_match_oracle(gpu, oracle, floor, fs):
+    """The long bands run by overlap-save FFT (h >= 64, longest h <= 1000),
+    the rest by the direct FIR; a 14 Hz floor makes the longest band exceed
+    the FFT block, so every band takes the direct FIR.  Candidates and best
+    F0 against the oracle's stages either way."""
+    x = reference_signal("speechlike", seconds=0.9, fs=fs)
+    off, f0, sc, best = gpu._candidates([x], fs, floor, 800.0)
+    st = oracle.stages(x, fs, floor, 800.0)
+    ref = st["f0"][:, :st["ncand"]]
+    refs = st["score"][:, :st["ncand"]]
+    nz = ref != 0
+    np.testing.assert_array_equal(np.diff(off), nz.sum(1))
+    np.testing.assert_allclose(f0, ref[nz], rtol=1e-9)
+    np.testing.assert_allclose(sc, refs[nz], rtol=1e-6)
+    assert_parity(st["best"], best)
