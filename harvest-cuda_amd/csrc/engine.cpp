@@ -264,6 +264,18 @@ double search_score(double f0, const CandidateRows& rows, int frame) {
   return score;
 }
 
+// search_score of two values in one pass over the frame's candidates.
+void search_scores(double f1, double f2, const CandidateRows& rows, int frame, double& s1,
+                   double& s2) {
+  s1 = 0.0;
+  s2 = 0.0;
+  for (int k = rows.offsets[frame]; k < rows.offsets[frame + 1]; ++k) {
+    const double f = rows.f0[k];
+    if (f == f1 && s1 < rows.scores[k]) s1 = rows.scores[k];
+    if (f == f2 && s2 < rows.scores[k]) s2 = rows.scores[k];
+  }
+}
+
 // MakeSortedOrder (harvest.cpp:883-896): WORLD's exchange pass, kept as is.
 void sorted_order(const std::vector<int>& list, int n, std::vector<int>& order) {
   order.resize(n);
@@ -336,11 +348,11 @@ void fix_step3(const double* in, int frames, const CandidateRows& rows, double* 
       throw_status(hv_wait_candidates(rows.pending, 1));
       scores_ready = true;
     }
-    for (int j = st2; j <= ed1; ++j) {  // equal values share one search
-      const double v1 = out[j], v2 = s.at(j);
-      const double a1 = search_score(v1, rows, j);
+    for (int j = st2; j <= ed1; ++j) {  // one pass over the row serves both searches
+      double a1, a2;
+      search_scores(out[j], s.at(j), rows, j, a1, a2);
       score1 += a1;
-      score2 += v2 == v1 ? a1 : search_score(v2, rows, j);
+      score2 += a2;
     }
     const int from = score1 > score2 ? ed1 : st2;
     copy_section(s, from, ed2, out);
