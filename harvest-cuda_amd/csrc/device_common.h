@@ -17,7 +17,7 @@ constexpr int kTile = 2048;          // filterbank output samples per workgroup
 // which the derivative crossings of the last owned samples read.
 constexpr int kTileStride = kTile - 2;
 constexpr int kSlabCap = kTile / 2;  // max crossings of one type per tile
-constexpr int kChunk = 128;          // IIR samples per scan chunk
+constexpr int kChunk = 64;          // IIR samples per scan chunk
 constexpr int kMeanBlock = 4096;     // samples per deterministic partial sum
 constexpr int kWindowMargin = 4;     // crossings kept either side of a tile
 // Frames per thread in k_interp: a tile spans kTileStride r 1000 / fs frames, at

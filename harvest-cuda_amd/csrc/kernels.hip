@@ -292,7 +292,16 @@ __global__ __launch_bounds__(kThreads) void k_iir_replay(
   long long start = (cg - u.c_off) * kChunk;
   long long end = min(start + (long long)kChunk, u.m);
   double w0 = carries[cg * 3 + 0], w1 = carries[cg * 3 + 1], w2 = carries[cg * 3 + 2];
-  const long long out_base = select_base(c, u);
+  // Pass 1 keeps output sample i when d = out_base - i is a nonnegative
+  // multiple of r: d falls by one per sample, so its residue and quotient are
+  // stepped instead of divided.
+  long long d = select_base(c, u) - start;
+  int rem = 0;
+  long long j = -1;
+  if (PASS == 1 && d >= 0) {
+    rem = (int)(d % c.ratio);
+    j = d / c.ratio;
+  }
   for (long long i0 = start; i0 < end; i0 += kIirAhead) {
     double in[kIirAhead];
 #pragma unroll
@@ -312,10 +321,12 @@ __global__ __launch_bounds__(kThreads) void k_iir_replay(
       } else if (PASS == 3) {  // y is the full-length output buffer here
         y[u.m_off + i] = out;
       } else {
-        long long d = out_base - i;
-        if (d >= 0 && d % c.ratio == 0) {
-          long long j = d / c.ratio;
-          if (j < u.n) y[u.y_off + j] = out;
+        if (rem == 0 && j >= 0 && j < u.n) y[u.y_off + j] = out;
+        if (rem == 0) {
+          rem = c.ratio - 1;
+          --j;
+        } else {
+          --rem;
         }
       }
     }
