@@ -202,7 +202,7 @@ def test_section_pool_survives_fork(core):
 def test_fix_step1_threshold_ties(core, oracle):
     """FixStep1's 0.008 tests (harvest.cpp:710-722) at quotients within a few
     ulps of the threshold, zero neighbours and negative references: the
-    engine's division-free decision equals WORLD's rounded quotients."""
+    engine's decisions equal the oracle's, bit for bit."""
     rng = np.random.default_rng(1718)
     for trial in range(300):
         frames = 12
