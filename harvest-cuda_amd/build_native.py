@@ -1,6 +1,7 @@
 """Build the native parts of harvest_cuda in place.
 
-  * libharvest_hip.so - hand-written gfx950 kernels + C ABI (hipcc)
+  * libharvest_hip.so - hand-written gfx950 kernels + C ABI (hipcc), and the
+                        host staging copy (g++)
   * _core*.so         - C++ host engine + pybind11 module (g++), linked to
                         libharvest_hip.so with an $ORIGIN rpath
 
@@ -48,8 +49,15 @@ def build_backend() -> Path:
         _run([HIPCC, *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
         return obj
 
-    with ThreadPoolExecutor(len(srcs)) as pool:
-        objs = list(pool.map(compile_one, srcs))
+    def compile_host(src):  # host-only code: the host compiler, no device pass
+        obj = tmp / (src + ".o")
+        _run(["g++", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-c", str(CSRC / src),
+              "-o", str(obj)])
+        return obj
+
+    with ThreadPoolExecutor(len(srcs) + 1) as pool:
+        host = pool.submit(compile_host, "host_copy.cpp")
+        objs = list(pool.map(compile_one, srcs)) + [host.result()]
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(out), *map(str, objs)])
     return out
 

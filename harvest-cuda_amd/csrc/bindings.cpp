@@ -58,6 +58,22 @@ int host_threads() {
   return (int)std::max(1u, std::min(n ? n : 1u, 16u));
 }
 
+// Runs a call on host waveforms whose parameters are already validated.  The
+// backend checks the sample values while it stages them for upload, so they
+// are read once; if the call fails before that (no usable GPU, say), the
+// samples are checked here so that a bad sample is still reported as such
+// (ValueError), as when samples are checked before the device is touched.
+template <class F>
+auto samples_first(const std::vector<const double*>& ptrs, const std::vector<int64_t>& lengths,
+                   F&& run) -> decltype(run()) {
+  try {
+    return run();
+  } catch (const std::runtime_error&) {
+    for (size_t u = 0; u < ptrs.size(); ++u) validate_samples(ptrs[u], lengths[u]);
+    throw;
+  }
+}
+
 }  // namespace
 
 PYBIND11_MODULE(_core, m) {
@@ -79,13 +95,12 @@ PYBIND11_MODULE(_core, m) {
   m.def("harvest", [](Input x, int fs, double floor, double ceil, double period, int device) {
     const double* p = data_1d(x);
     const int64_t n = (int64_t)x.size();
-    validate(p, n, fs, floor, ceil, period);
-    Result r;
-    {
+    validate_params(n, fs, floor, ceil, period);
+    Result r = samples_first({p}, {n}, [&] {
       py::gil_scoped_release release;
       Cache::Lease ws(cache(), device);
-      r = analyze(ws.get(), p, n, fs, floor, ceil, period, host_threads());
-    }
+      return analyze(ws.get(), p, n, fs, floor, ceil, period, host_threads());
+    });
     return py::make_tuple(to_numpy(std::move(r.f0)), to_numpy(std::move(r.times)));
   }, py::arg("x").noconvert(), py::arg("fs"), py::arg("f0_floor") = 71.0,
      py::arg("f0_ceil") = 800.0, py::arg("frame_period") = 5.0, py::kw_only(),
@@ -102,14 +117,14 @@ PYBIND11_MODULE(_core, m) {
       if (!Input::check_(h)) throw py::type_error("every utterance must be a 1-D C-contiguous float64 array");
       Input a = py::reinterpret_borrow<Input>(h);
       const double* p = data_1d(a);
-      validate(p, (int64_t)a.size(), fs, floor, ceil, period);
+      validate_params((int64_t)a.size(), fs, floor, ceil, period);
       keep.push_back(a);
       ptrs.push_back(p);
       lengths.push_back((int64_t)a.size());
     }
     if (ptrs.empty()) return py::object(py::list());
-    std::vector<Result> rs;
-    {
+    std::vector<Result> rs = samples_first(ptrs, lengths, [&] {
+      std::vector<Result> out;
       py::gil_scoped_release release;
       Cache::Lease ws(cache(), device);
       if (device >= kSecondSlot) throw std::invalid_argument("device index out of range");
@@ -117,13 +132,14 @@ PYBIND11_MODULE(_core, m) {
       for (int64_t n : lengths) frames.push_back(derive_params(n, fs, floor, ceil, period).frames);
       if (!timings && pipeline_groups(frames) > 1) {
         Cache::Lease second(cache(), device + kSecondSlot);
-        rs = analyze_batch(ws.get(), ptrs, lengths, fs, floor, ceil, period, host_threads(),
-                           nullptr, second.get());
+        out = analyze_batch(ws.get(), ptrs, lengths, fs, floor, ceil, period, host_threads(),
+                            nullptr, second.get());
       } else {
-        rs = analyze_batch(ws.get(), ptrs, lengths, fs, floor, ceil, period, host_threads(),
-                           timings ? &t : nullptr);
+        out = analyze_batch(ws.get(), ptrs, lengths, fs, floor, ceil, period, host_threads(),
+                            timings ? &t : nullptr);
       }
-    }
+      return out;
+    });
     py::list out;
     for (auto& r : rs) out.append(py::make_tuple(to_numpy(std::move(r.f0)), to_numpy(std::move(r.times))));
     if (timings) return py::object(py::make_tuple(out, t));
@@ -172,14 +188,13 @@ PYBIND11_MODULE(_core, m) {
     if (!gpu) throw std::invalid_argument("this build profiles the GPU path only (gpu=True)");
     const double* p = data_1d(x);
     const int64_t n = (int64_t)x.size();
-    validate(p, n, fs, floor, ceil, period);
-    Result r;
+    validate_params(n, fs, floor, ceil, period);
     Timings t;
-    {
+    Result r = samples_first({p}, {n}, [&] {
       py::gil_scoped_release release;
       Cache::Lease ws(cache(), device);
-      r = analyze(ws.get(), p, n, fs, floor, ceil, period, host_threads(), &t);
-    }
+      return analyze(ws.get(), p, n, fs, floor, ceil, period, host_threads(), &t);
+    });
     return py::make_tuple(to_numpy(std::move(r.f0)), to_numpy(std::move(r.times)), t);
   }, py::arg("x").noconvert(), py::arg("fs"), py::arg("f0_floor") = 71.0,
      py::arg("f0_ceil") = 800.0, py::arg("frame_period") = 5.0, py::arg("gpu") = true,

@@ -7,12 +7,14 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "../../include/harvest_hip.h"
+#include "host_copy.h"
 #include "kernels.h"
 
 namespace hv {
@@ -701,6 +703,9 @@ void world_sizes(long long length, int fs, double f0_floor, int ratio, double fs
   *frames = (int)(1000.0 * (double)length / fs / 1.0) + 1;
 }
 
+// Host input is staged and uploaded in slices of this many samples (4 MiB).
+constexpr long long kUploadSlice = 1LL << 19;
+
 void check_common(int fs, double f0_floor, double f0_ceil) {
   if (fs < 8000 || fs > 192000) invalid("fs must be between 8000 and 192000 Hz");
   if (!std::isfinite(f0_floor) || !std::isfinite(f0_ceil) || f0_floor <= 0 || f0_ceil <= f0_floor)
@@ -744,11 +749,30 @@ Plan preprocess_batch(hv_workspace* ws, int count, const double* const* xs,
   if (device_x) {
     dx = xs[0];
   } else {
+    // The batch goes through pinned staging in slices: the copy of slice k+1
+    // (which also checks that every sample is finite) overlaps the upload of
+    // slice k.  Slices split utterances freely; each utterance is contiguous
+    // at its x_off in both buffers.
     double* hx = ws->h_x.ensure<double>(p.total_x);
-    for (int u = 0; u < count; ++u)
-      std::memcpy(hx + p.utts[u].x_off, xs[u], lens[u] * sizeof(double));
     double* d = ws->x.ensure<double>(p.total_x);
-    check(hipMemcpyAsync(d, hx, p.total_x * sizeof(double), hipMemcpyHostToDevice, s), "upload x");
+    bool finite = true;
+    int u = 0;
+    for (long long at = 0; at < p.total_x; at += kUploadSlice) {
+      const long long end = std::min(at + kUploadSlice, p.total_x);
+      for (; u < count && p.utts[u].x_off < end; ++u) {
+        const long long a = std::max(at, p.utts[u].x_off);
+        const long long b = std::min(end, p.utts[u].x_off + p.utts[u].length);
+        finite &= copy_finite(hx + a, xs[u] + (a - p.utts[u].x_off), b - a);
+        if (b < p.utts[u].x_off + p.utts[u].length) break;  // continues in the next slice
+      }
+      check(hipMemcpyAsync(d + at, hx + at, (end - at) * sizeof(double), hipMemcpyHostToDevice, s),
+            "upload x");
+    }
+    if (!finite) {
+      // No kernel has been queued; the uploads finish before hx is reused.
+      check(hipStreamSynchronize(s), "upload x");
+      invalid("x must contain only finite samples");
+    }
     dx = d;
   }
   double* y = ws->y.ensure<double>(p.total_y);

@@ -519,12 +519,16 @@ void validate_params(int64_t length, int fs, double f0_floor, double f0_ceil,
     throw std::invalid_argument("input or analysis settings exceed supported allocation sizes");
 }
 
-void validate(const double* x, int64_t length, int fs, double f0_floor, double f0_ceil,
-              double frame_period) {
-  validate_params(length, fs, f0_floor, f0_ceil, frame_period);
+void validate_samples(const double* x, int64_t length) {
   bool finite = true;
   for (int64_t i = 0; i < length; ++i) finite &= std::isfinite(x[i]);
   if (!finite) throw std::invalid_argument("x must contain only finite samples");
+}
+
+void validate(const double* x, int64_t length, int fs, double f0_floor, double f0_ceil,
+              double frame_period) {
+  validate_params(length, fs, f0_floor, f0_ceil, frame_period);
+  validate_samples(x, length);
 }
 
 void fix_contour(const CandidateRows& rows, int frames, const double* base, double* out,

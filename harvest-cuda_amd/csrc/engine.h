@@ -38,9 +38,11 @@ struct Params {
 Params derive_params(int64_t length, int fs, double f0_floor, double f0_ceil,
                      double frame_period);
 // Validation of the public entry points (reference src/bindings.cpp:18-39);
-// validate_params checks everything but the sample values.
+// validate_params checks everything but the sample values, validate_samples
+// only those (the backend checks them too, while staging host input).
 void validate_params(int64_t length, int fs, double f0_floor, double f0_ceil,
                      double frame_period);
+void validate_samples(const double* x, int64_t length);
 void validate(const double* x, int64_t length, int fs, double f0_floor, double f0_ceil,
               double frame_period);
 

@@ -105,6 +105,11 @@ hv_status hv_analyze(hv_workspace* ws, const double* y, int64_t y_length,
  *   best[total_frames],
  *   *nnz = offsets[total_frames].
  * Every utterance's results are bitwise identical to a call with count = 1.
+ * The waveforms are staged through pinned memory in slices (the copy of one
+ * slice overlaps the upload of the one before); a sample that is infinite or
+ * NaN returns HV_EINVAL ("x must contain only finite samples") before any
+ * kernel is queued, as the reference's host validation does
+ * (src/bindings.cpp:36-38).
  * This is the batched form (no reference counterpart) and, at count = 1,
  * the plain form; it fuses gpu_decimate, the host mean removal
  * (src/engine.cpp:278-281) and gpu_analyze. */

@@ -367,6 +367,26 @@ def test_pipelined_batches_equal_plain_bitwise(gpu):
         np.testing.assert_array_equal(dt, t)
 
 
+def test_nonfinite_samples_rejected_by_staging(gpu):
+    """With a GPU the sample check runs in the backend's staging pass (host
+    input goes to pinned memory in 4 MiB slices): a NaN or inf in a later
+    slice, at either end, or in a later utterance of a batch raises
+    ValueError, and the next call on the same workspace is unaffected."""
+    fs = 16000
+    x = reference_signal("speechlike", seconds=45.0, fs=fs)  # 720 000 samples: two slices
+    want_f0, want_t = gpu.harvest(x, fs)
+    for bad, at in ((np.nan, 600_000), (np.inf, 0), (-np.inf, len(x) - 1)):
+        y = x.copy()
+        y[at] = bad
+        with pytest.raises(ValueError, match="finite"):
+            gpu.harvest(y, fs)
+        with pytest.raises(ValueError, match="finite"):
+            gpu.harvest_batch([np.ascontiguousarray(x[:50_000]), y], fs)
+        f0, t = gpu.harvest(x, fs)
+        np.testing.assert_array_equal(f0, want_f0)
+        np.testing.assert_array_equal(t, want_t)
+
+
 @pytest.mark.parametrize("floor,fs", [(30.0, 16000), (14.0, 16000), (71.0, 8000), (71.0, 96000)])
 def test_filterbank_fft_and_direct_bands_match_oracle(gpu, oracle, floor, fs):
     """The long bands run by overlap-save FFT (h >= 64, longest h <= 1000),
