@@ -14,6 +14,7 @@
 #include <condition_variable>
 #include <exception>
 #include <functional>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <stdexcept>
@@ -167,25 +168,29 @@ void boundary_list(const double* f0, int frames, std::vector<int>& list) {
   }
 }
 
-// FixStep1 (harvest.cpp:710-722), allowed range 0.008.
-void fix_step1(const double* base, int frames, double* out) {
-  for (int i = 0; i < frames; ++i) out[i] = 0.0;
-  for (int i = 2; i < frames; ++i) {
-    if (base[i] == 0.0) continue;
-    const double reference = base[i - 1] * 2 - base[i - 2];
-    const bool jump = std::fabs((base[i] - reference) / reference) > 0.008 &&
-                      std::fabs((base[i] - base[i - 1])) / base[i - 1] > 0.008;
-    out[i] = jump ? 0.0 : base[i];
-  }
-}
-
-// FixStep2 (harvest.cpp:748-762): voiced sections shorter than 6 frames.
-void fix_step2(const double* in, int frames, double* out, std::vector<int>& list) {
-  std::copy(in, in + frames, out);
-  boundary_list(in, frames, list);
-  for (size_t s = 0; s + 1 < list.size(); s += 2) {
-    if (list[s + 1] - list[s] >= 6) continue;
-    for (int j = list[s]; j <= list[s + 1]; ++j) out[j] = 0.0;
+// FixStep1 (harvest.cpp:710-722, allowed range 0.008) and FixStep2
+// (harvest.cpp:748-762) in one pass: FixStep1's value of each frame is
+// written as it is found, and each voiced section of those values, as
+// boundary_list delimits it (frames 0 and frames - 1 count as unvoiced), is
+// zeroed once it closes if it spans fewer than 6 frame steps.
+void fix_step12(const double* base, int frames, double* out) {
+  int start = -1;  // first frame of the open section
+  for (int i = 0; i < frames; ++i) {
+    double v = 0.0;
+    if (i >= 2 && base[i] != 0.0) {
+      const double reference = base[i - 1] * 2 - base[i - 2];
+      const bool jump = std::fabs((base[i] - reference) / reference) > 0.008 &&
+                        std::fabs((base[i] - base[i - 1])) / base[i - 1] > 0.008;
+      v = jump ? 0.0 : base[i];
+    }
+    out[i] = v;
+    const bool voiced = i >= 1 && i < frames - 1 && v > 0;
+    if (voiced) {
+      if (start < 0) start = i;
+    } else if (start >= 0) {
+      if (i - 1 - start < 6) std::fill(out + start, out + i, 0.0);
+      start = -1;
+    }
   }
 }
 
@@ -291,7 +296,6 @@ void sorted_order(const std::vector<int>& list, int n, std::vector<int>& order) 
 // neighbourhood instead of a full-length channel.
 void fix_step3(const double* in, int frames, const CandidateRows& rows, double* out,
                std::vector<int>& list, int threads) {
-  std::copy(in, in + frames, out);
   boundary_list(in, frames, list);
   const int count = (int)list.size() / 2;
   std::vector<Section> sections(count);
@@ -316,7 +320,10 @@ void fix_step3(const double* in, int frames, const CandidateRows& rows, double* 
     mean /= s.end - s.start;
     if (2200.0 / mean < s.end - s.start) std::swap(sections[selected++], sections[i]);
   }
-  if (selected == 0) return;
+  if (selected == 0) {  // nothing merged: the input stands
+    std::copy(in, in + frames, out);
+    return;
+  }
   // MergeF0 (harvest.cpp:937-963).  Boundary pair 0 doubles as the merge
   // cursor and is mutated in place, also when section 0 is revisited.
   std::vector<int> bl(2 * selected);
@@ -533,13 +540,13 @@ void validate(const double* x, int64_t length, int fs, double f0_floor, double f
 
 void fix_contour(const CandidateRows& rows, int frames, const double* base, double* out,
                  int threads) {
-  std::vector<double> a(frames), b(frames);
+  // Every element of both is written before it is read.
+  std::unique_ptr<double[]> step2(new double[frames]), step3(new double[frames]);
   std::vector<int> list;
   list.reserve(64);
-  fix_step1(base, frames, a.data());
-  fix_step2(a.data(), frames, b.data(), list);
-  fix_step3(b.data(), frames, rows, a.data(), list, threads);
-  fix_step4(a.data(), frames, out, list);
+  fix_step12(base, frames, step2.get());
+  fix_step3(step2.get(), frames, rows, step3.get(), list, threads);
+  fix_step4(step3.get(), frames, out, list);
 }
 
 // SmoothF0Contour (harvest.cpp:1079-1113).  WORLD pads the contour with 300
@@ -619,11 +626,10 @@ void smooth_contour(const double* f0, int frames, double* out, int threads) {
 
 void fix_contour_split(const CandidateRows& rows, int frames, const double* base, double* step3,
                        double* out, int threads) {
-  std::vector<double> a(frames), b(frames);
+  std::unique_ptr<double[]> step2(new double[frames]);
   std::vector<int> list;
-  fix_step1(base, frames, a.data());
-  fix_step2(a.data(), frames, b.data(), list);
-  fix_step3(b.data(), frames, rows, step3, list, threads);
+  fix_step12(base, frames, step2.get());
+  fix_step3(step2.get(), frames, rows, step3, list, threads);
   fix_step4(step3, frames, out, list);
 }
 

@@ -224,3 +224,29 @@ def test_fix_step1_threshold_ties(core, oracle):
         expect = oracle.fix_contour(f0, sc, 1)
         got = engine_contour(core, oracle, f0, sc)
         np.testing.assert_array_equal(got, expect, err_msg=f"trial {trial}")
+
+
+def test_short_sections_at_the_edges(core, oracle):
+    """FixStep1 and FixStep2 (harvest.cpp:710-762), fused into one pass in the
+    engine: dense voiced runs of 1-9 frames (FixStep2 drops those spanning
+    fewer than 6 frame steps), runs touching frames 1 and frames - 2, a voiced
+    last frame (boundary_list counts it unvoiced) and single-frame contours,
+    against the oracle's separate steps."""
+    rng = np.random.default_rng(4242)
+    for trial in range(400):
+        frames = 1 + int(rng.integers(60)) if trial % 4 else 1 + int(rng.integers(4))
+        base = np.zeros(frames)
+        i = int(rng.integers(0, 3))
+        while i < frames:
+            run = int(rng.integers(1, 10))
+            f = 120.0 + rng.random() * 5.0
+            for j in range(i, min(frames, i + run)):
+                base[j] = f * (1.0 + 0.001 * (j - i))  # smooth: FixStep1 keeps it
+            i += run + int(rng.integers(1, 4))
+        if trial % 5 == 0:
+            base[-1] = 130.0
+        f0 = base[:, None].copy()
+        sc = np.where(f0 > 0, 1.0, 0.0)
+        expect = oracle.fix_contour(f0, sc, 1)
+        got = engine_contour(core, oracle, f0, sc)
+        np.testing.assert_array_equal(got, expect, err_msg=f"trial {trial}")
