@@ -1240,7 +1240,8 @@ __global__ __launch_bounds__(kThreads) void k_official(Config c, const long long
 //     (FixF0 and the gate) in one lane-parallel pass at the end.
 // The order of operations per bin is fixed, so results do not depend on the
 // batch, on the staging passes or on which path served the bin.
-constexpr int kWaves = 4;        // frames per workgroup
+constexpr int kWaves = 4;        // frames per workgroup (prune)
+constexpr int kRefineWaves = 2;  // frames per refinement workgroup: finer blocks balance better
 constexpr int kSplit = 8;        // lanes sharing one DFT bin
 constexpr int kPer = 1;          // bins per lane group per pass
 constexpr int kFrameBins = 128;  // bin results of one frame kept in LDS (< 256)
@@ -1390,7 +1391,7 @@ __device__ __forceinline__ void twiddle_at(const double* tw, int stride, int n, 
   }
 }
 
-__global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
+__global__ __launch_bounds__(64 * kRefineWaves, 3) void k_refine_waves(
     Config c, const Utt* utt, const long long* f_off, long long total_frames, int HS, int W,
     const double* y, const double* official, const int* count, const double* twiddle,
     const double* wintab, double* rf0, double* rscore, unsigned long long* work) {
@@ -1403,7 +1404,7 @@ __global__ __launch_bounds__(kThreads, 3) void k_refine_waves(
   const WaveLds L =
       carve_wave_lds((LDS char*)(char*)lds_raw + wave * wave_lds_bytes(S, HS, W), S, HS, W);
 
-  const long long gf = (long long)xcd_block(blockIdx.x, gridDim.x) * kWaves + wave;
+  const long long gf = (long long)xcd_block(blockIdx.x, gridDim.x) * kRefineWaves + wave;
   if (gf >= total_frames) return;
   const int ui = find_segment(f_off, c.count, gf);
   const int cnt = count[ui];
@@ -2323,13 +2324,13 @@ void launch_refine(hipStream_t s, const Config& c, const Utt* utt,
   while (HS < 2 * c.slots) HS <<= 1;
   int W = (window_fft(c.hmax) / 2 + 64) >> 6;
   if (W > kMaxWords) W = 0;  // very low floors: every window takes the slot-by-slot path
-  const size_t bytes = kWaves * wave_lds_bytes(c.slots, HS, W);
+  const size_t bytes = kRefineWaves * wave_lds_bytes(c.slots, HS, W);
   if (bytes > (size_t)c.lds_limit)
     throw std::runtime_error("refinement needs " + std::to_string(bytes) +
                              " bytes of LDS per workgroup; the device allows " +
                              std::to_string(c.lds_limit));
-  const unsigned grid = (unsigned)((total_frames + kWaves - 1) / kWaves);
-  k_refine_waves<<<grid, kThreads, bytes, s>>>(c, utt, f_off, total_frames, HS, W, y, official,
+  const unsigned grid = (unsigned)((total_frames + kRefineWaves - 1) / kRefineWaves);
+  k_refine_waves<<<grid, 64 * kRefineWaves, bytes, s>>>(c, utt, f_off, total_frames, HS, W, y, official,
                                                count, twiddle, wintab, rf0, rscore, work);
   check_launch("refine");
 }
