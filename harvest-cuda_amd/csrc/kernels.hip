@@ -1248,21 +1248,21 @@ constexpr int kFrameBins = 128;  // bin results of one frame kept in LDS (< 256)
 constexpr int kStage = 128;      // windowed samples staged per pass (multiple of 64)
 constexpr int kMaxWords = 128;   // bitmap words (bins 0..8191)
 constexpr int kXcds = 8;         // XCDs that workgroups are dispatched to in turn
-constexpr int kYPre = 6;         // 64-sample chunks of the next window loaded ahead
+constexpr int kYPre = 1;         // 64-sample chunks of the next window loaded ahead (more
+                                 // cost registers: 6 chunks ran 8 % slower)
 
 
 // One wave's part of k_refine_waves' dynamic LDS, in carve order:
 //   f0     [S]          candidate per slot
-//   q      [S]          f0 n / fs of the slot's window
 //   res    [2 kFrameBins] power and IF numerator per bin result
 //   xwd    [2 kStage]   x w and x d of the staged samples
 //   bits   [W]          distinct bins of the current window
 //   wpre   [W]          bins below each bitmap word
 //   h      [S]          half window length (0: inactive slot)
-//   nh     [S]          harmonics FixF0 uses
-//   win    [S]          window of each slot
-//   rank   [S]          position of the slot in its window's list
-//   wslots [S]          slots grouped by window
+//   nh     [S]          harmonics FixF0 uses (8 bits)
+//   win    [S]          window of each slot (16 bits)
+//   rank   [S]          position of the slot in its window's list (16 bits)
+//   wslots [S]          slots grouped by window (16 bits)
 //   win_h  [S]          half length of each window
 //   pk     [6 S]        bin of each (grouped slot, harmonic), -1 past nh
 //   hkey   [HS]         hashed half lengths (-1: empty), later slot counts
@@ -1271,10 +1271,10 @@ constexpr int kYPre = 6;         // 64-sample chunks of the next window loaded a
 //   jidx   [6 S]        result index of each (grouped slot, harmonic)
 //   misc   [4]          counters
 #define WAVE_LDS_FIELDS(X)                                                   \
-  X(double, f0, S) X(double, q, S) X(double, res, 2 * kFrameBins)            \
+  X(double, f0, S) X(double, res, 2 * kFrameBins)                           \
   X(double, xwd, 2 * kStage) X(unsigned long long, bits, W) X(int, wpre, W)  \
-  X(int, h, S) X(int, nh, S) X(int, win, S) X(int, rank, S)                  \
-  X(int, wslots, S) X(int, win_h, S) X(int, hkey, HS) X(int, hval, HS)       \
+  X(int, h, S) X(unsigned char, nh, S) X(short, win, S) X(short, rank, S)    \
+  X(short, wslots, S) X(int, win_h, S) X(int, hkey, HS) X(int, hval, HS)     \
   X(int, bink, kFrameBins) X(short, pk, 6 * S) X(unsigned char, jidx, 6 * S) X(int, misc, 4)
 
 struct WaveLds {
@@ -1391,7 +1391,9 @@ __device__ __forceinline__ void twiddle_at(const double* tw, int stride, int n, 
   }
 }
 
-__global__ __launch_bounds__(64 * kRefineWaves, 3) void k_refine_waves(
+// Four waves per SIMD: at most 128 VGPRs (the compiler spills a few dwords)
+// and about 10 KB of LDS per wave at 71-800 Hz; 17 % faster than three.
+__global__ __launch_bounds__(64 * kRefineWaves, 4) void k_refine_waves(
     Config c, const Utt* utt, const long long* f_off, long long total_frames, int HS, int W,
     const double* y, const double* official, const int* count, const double* twiddle,
     const double* wintab, double* rf0, double* rscore, unsigned long long* work) {
@@ -1444,7 +1446,6 @@ __global__ __launch_bounds__(64 * kRefineWaves, 3) void k_refine_waves(
     int pos = -1;
     if (hw > 0) {
       L.nh[s] = min((int)(fs / 2.0 / f0), 6);
-      L.q[s] = f0 * window_fft(hw) / fs;
       pos = (int)(((unsigned)hw * 2654435761u) & (unsigned)(HS - 1));
       while (true) {
         const int prev = lds_cas(L.hkey + pos, -1, hw);
@@ -1522,7 +1523,7 @@ __global__ __launch_bounds__(64 * kRefineWaves, 3) void k_refine_waves(
     if (L.win[s] >= 0) {
       const int pos = wbeg[L.win[s]] + L.rank[s];
       const int H = L.nh[s];
-      const double qv = L.q[s];
+      const double qv = L.f0[s] * window_fft(L.h[s]) / fs;
       L.wslots[pos] = s;
 #pragma unroll
       for (int hh = 0; hh < 6; ++hh) L.pk[6 * pos + hh] = (short)(hh < H ? mround(qv * (hh + 1)) : -1);
@@ -2320,8 +2321,9 @@ void launch_refine(hipStream_t s, const Config& c, const Utt* utt,
                    const double* y, const double* official, const int* count,
                    const double* twiddle, const double* wintab, double* rf0, double* rscore,
                    unsigned long long* work) {
-  int HS = 64;  // window hash: a power of two >= 2 slots keeps probing short
-  while (HS < 2 * c.slots) HS <<= 1;
+  int HS = 64;  // window hash: a power of two above the slot count (never full;
+                // reused for the nwin + 1 list starts)
+  while (HS < c.slots + 1) HS <<= 1;
   int W = (window_fft(c.hmax) / 2 + 64) >> 6;
   if (W > kMaxWords) W = 0;  // very low floors: every window takes the slot-by-slot path
   const size_t bytes = kRefineWaves * wave_lds_bytes(c.slots, HS, W);
