@@ -20,6 +20,7 @@
 #include <stdexcept>
 #include <thread>
 
+#include <emmintrin.h>
 #include <unistd.h>
 
 namespace harvest {
@@ -270,11 +271,29 @@ double search_score(double f0, const CandidateRows& rows, int frame) {
 }
 
 // search_score of two values in one pass over the frame's candidates.
+// The running maximum (a score replaces it only if larger, so NaN scores
+// never do) is kept over the row's even and odd positions in two SSE2 lanes
+// and the lanes are then combined: the maximum of the matches' non-NaN
+// scores, or 0, is the same in any order.
 void search_scores(double f1, double f2, const CandidateRows& rows, int frame, double& s1,
                    double& s2) {
-  s1 = 0.0;
-  s2 = 0.0;
-  for (int k = rows.offsets[frame]; k < rows.offsets[frame + 1]; ++k) {
+  int k = rows.offsets[frame];
+  const int end = rows.offsets[frame + 1];
+  const __m128d v1 = _mm_set1_pd(f1), v2 = _mm_set1_pd(f2);
+  __m128d m1 = _mm_setzero_pd(), m2 = _mm_setzero_pd();
+  for (; k + 2 <= end; k += 2) {
+    const __m128d f = _mm_loadu_pd(rows.f0 + k), sc = _mm_loadu_pd(rows.scores + k);
+    const __m128d u1 = _mm_and_pd(_mm_cmpeq_pd(f, v1), _mm_cmpgt_pd(sc, m1));
+    const __m128d u2 = _mm_and_pd(_mm_cmpeq_pd(f, v2), _mm_cmpgt_pd(sc, m2));
+    m1 = _mm_or_pd(_mm_and_pd(u1, sc), _mm_andnot_pd(u1, m1));
+    m2 = _mm_or_pd(_mm_and_pd(u2, sc), _mm_andnot_pd(u2, m2));
+  }
+  double a[2], b[2];
+  _mm_storeu_pd(a, m1);
+  _mm_storeu_pd(b, m2);
+  s1 = a[1] > a[0] ? a[1] : a[0];
+  s2 = b[1] > b[0] ? b[1] : b[0];
+  for (; k < end; ++k) {
     const double f = rows.f0[k];
     if (f == f1 && s1 < rows.scores[k]) s1 = rows.scores[k];
     if (f == f2 && s2 < rows.scores[k]) s2 = rows.scores[k];
