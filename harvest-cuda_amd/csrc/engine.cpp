@@ -440,7 +440,11 @@ template <class F> Cycle find_cycle(const Smoother& start, long limit, F&& f) {
   ring[0] = s;
   for (long k = 1; k <= limit; ++k) {
     f(s);
-    for (int d = 1; d <= kRing && d <= k; ++d)
+    // The ring is searched every kRing steps (and at the limit): a cycle of
+    // length d <= kRing is then seen at most kRing - 1 steps later, with the
+    // same d (the smallest match is the cycle length) and the same entry.
+    const bool scan = (k % kRing) == 0 || k == limit;
+    for (int d = 1; scan && d <= kRing && d <= k; ++d)
       if (ring[(k - d) % kRing].same(s)) {
         // State k equals state k - d: cycle of length d entered at or
         // before k - d; locate its first entry.
