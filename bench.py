@@ -1,29 +1,41 @@
 #!/usr/bin/env python
 """Throughput benchmark of the MI355X Harvest path.
 
-Primary line (BASELINE metric): audio-seconds per second of the plain drop-in
-call on the config-2 workload (one 60 s float64 signal at 48 kHz), input
-already resident in HBM, output f0 on the host (device analysis + host
-contour).  One step = one call.  Also reported:
+Primary line: the batched workload (512 float64 utterances at 24 kHz,
+lengths log-uniform over 0.5-30 s, seed 1003) analysed by one batched call,
+NumPy in and NumPy out (the reference benchmark's scope,
+benchmarks/benchmark.py:73: host input, upload, device analysis, host
+contour, output arrays).  One step = one harvest_batch call over the whole
+workload.  `value` = audio-seconds per second over the K timed steps.
+Reported beside it:
+  * plain: the same utterances through one harvest() call each (the plain
+    per-call GPU path, same scope), median of --plain-repeats loops, and the
+    batched/plain ratio;
+  * first_call: the cold first batched call (allocations included);
   * roofline of the dominant kernel from hipEvents recorded around every
-    kernel launch on the backend's stream during the timed steps;
-  * parity of the step's output against WORLD (same V/UV, max relative F0
-    error on voiced frames);
-  * cpu_baseline: WORLD Harvest compiled from the reference (oracle/_ref) or,
-    where that library is absent, the oracle's C restatement, single thread,
-    on the same 60 s signal;
-  * batched: config 3 (512 utterances at 24 kHz) as one batched call versus
-    one plain call per utterance on the same GPU.
+    kernel launch on the backends' streams during the timed steps;
+  * cpu_baseline: WORLD Harvest (built from the reference into oracle/_ref,
+    or the oracle's C restatement where that is absent) threaded over the
+    512 utterances, longest first, on the host's cores; its outputs are also
+    the accuracy gate of every utterance of the batched output (identical
+    V/UV, relative F0 error <= 1e-6) with the reference benchmark's
+    accuracy figures;
+  * config2: one 60 s signal at 48 kHz through the plain call, host input
+    and HBM-resident input, with its kernel breakdown;
+  * config5: --long recordings of 30 min at 44.1 kHz through harvest_stream,
+    recording 0 checked against WORLD's stored F0.
 
-Usage: python bench.py [--gpus 1] [--steps 50] [--warmup 40] [--batched 1]
+Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 5]
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import statistics
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 ROOT = Path(__file__).resolve().parent
@@ -33,36 +45,23 @@ sys.path.insert(0, str(ROOT))
 METRIC = "audio-sec/sec on 1 MI355X, batched vs plain, same V/UV, ≤1e-6 rel F0 err"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+KERNEL_KEYS = ["kernel_decimate", "kernel_filterbank", "kernel_interp", "kernel_refine",
+               "kernel_prune_compact"]
+KERNEL_NAMES = {"kernel_refine": "k_refine_waves", "kernel_filterbank": "k_filter_fft+k_filter_events",
+                "kernel_interp": "k_tile_scan+k_interp", "kernel_decimate": "k_iir_*+k_carry_*",
+                "kernel_prune_compact": "k_prune_best+scan+k_compact"}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def fir_flops(lengths, fs, floor=71.0, ceil=800.0):
-    """Useful FP64 FLOPs of the filterbank kernel: one FMA (2 FLOPs) per tap
-    per output sample per band, 2 sum(2h_b + 1) N_u over utterances."""
-    from harvest_cuda import _core
-    import math
-    p = _core._params(lengths[0], fs, floor, ceil, 5.0)
-    fs_w = p["fs_work"]
-    af = floor * 0.9
-    taps = 0
-    for b in range(p["bands"]):
-        bound = af * 2.0 ** ((b + 1) / 40.0)
-        x = fs_w / bound * 2.0
-        h = int(x + 0.5) if x > 0 else int(x - 0.5)
-        taps += 2 * h + 1
-    n_total = sum(_core._params(n, fs, floor, ceil, 5.0)["y_length"] for n in lengths)
-    return 2.0 * taps * n_total
-
-
 def algorithmic_bytes(lengths, fs, floor=71.0, ceil=800.0):
-    """Implementation-independent HBM bytes of the path per call, summed over
-    utterances: A = 8 L + 8 N + 16 (n/2 + 1)(2B + 1) + 16 B F + 24 F C (read x,
-    write y, the signal spectrum and one round trip of each band's product
-    spectrum at WORLD's global FFT size n, the B x F raw candidate matrix, and
-    the F x C candidate slots read and refined; C = 7 round(B / 10))."""
+    """Implementation-independent HBM bytes per utterance, summed:
+    A = 8 L + 8 N + 16 (n/2 + 1)(2B + 1) + 16 B F + 24 F C (read x, write y,
+    the signal spectrum and one round trip of each band's product spectrum at
+    WORLD's global FFT size n, the B x F raw candidate matrix, and the F x C
+    candidate slots read and refined; C = 7 round(B / 10))."""
     from harvest_cuda import _core
     total = 0.0
     for L in lengths:
@@ -72,124 +71,282 @@ def algorithmic_bytes(lengths, fs, floor=71.0, ceil=800.0):
     return total
 
 
-def measured_traffic(kernel):
-    """HBM bytes per dispatch of `kernel` on this workload, from the committed
-    FETCH_SIZE/WRITE_SIZE counter passes calibrated by a copy of known size
-    (profiles/config2_traffic.md); None when not profiled."""
+def measured_traffic(kernel, workload):
+    """HBM bytes per dispatch of `kernel` on `workload` from the committed
+    FETCH_SIZE/WRITE_SIZE counter passes (profiles/*traffic*.md, each line
+    `traffic_bytes_per_launch <kernel> <workload>: <bytes>`); None when not
+    profiled."""
     import re
-    path = ROOT / "profiles" / "config2_traffic.md"
-    if not path.exists():
-        return None
-    m = re.search(r"^traffic_bytes_per_launch %s: (\d+)$" % re.escape(kernel), path.read_text(), re.M)
-    return float(m.group(1)) if m else None
+    pat = re.compile(r"^traffic_bytes_per_launch %s %s: (\d+)$" % (re.escape(kernel),
+                                                                     re.escape(workload)), re.M)
+    for path in sorted((ROOT / "profiles").glob("*traffic*.md")):
+        m = pat.search(path.read_text())
+        if m:
+            return float(m.group(1))
+    return None
 
 
-def roofline(kernels_ms, work, lengths, fs, steps):
-    """Roofline object of the kernel with the largest average duration."""
-    avg = {k: v / steps for k, v in kernels_ms.items()}
-    name = max(avg, key=avg.get)
-    sec = avg[name] / 1000.0
-    if name == "kernel_filterbank":
-        flops = fir_flops(lengths, fs)
-        achieved = flops / sec / 1e12
-        return name, {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                      "traffic": measured_traffic("k_filter_events"),
-                      "kernel": "k_filter_events", "avg_ms": avg[name],
-                      "model": "2 FLOP per tap per output sample per band (FP64 FMA)"}
+def roofline(kernels_ms, work, launches, workload):
+    """Roofline object of the kernel with the largest time per step.  The
+    refinement kernel is bound by FP64 vector FMAs (no MFMA: its DFT bins are
+    sums over a window, not a dense contraction); its algorithmic FLOPs are
+    8 per bin-sample (two Goertzel steps: an FMA and a subtraction each, for
+    the main and the derivative window) plus 17 per window sample (window,
+    its central difference and the two products), counted on the device."""
+    name = max(kernels_ms, key=kernels_ms.get)
+    per_launch_ms = kernels_ms[name] / launches
     if name == "kernel_refine":
-        flops = 8.0 * work["bin_samples"] / steps + 17.0 * work["window_samples"] / steps
-        achieved = flops / sec / 1e12
-        return name, {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                      "traffic": measured_traffic("k_refine_waves"),
-                      "kernel": "k_refine_waves", "avg_ms": avg[name],
-                      "model": "8 FLOP per bin-sample (4 FMA) + 17 FLOP per window sample, "
-                               "cosines not counted"}
-    # HBM-bound stages: algorithmic bytes of the decimation scan (read x
-    # twice, write and read the pass buffer twice, write y).
-    total_x = sum(lengths)
-    bytes_ = 8.0 * (2 * total_x + 4 * total_x + total_x / 3)
-    achieved = bytes_ / sec / 1e9
-    return name, {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                  "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": name,
-                  "avg_ms": avg[name]}
+        flops = (8.0 * work["bin_samples"] + 17.0 * work["window_samples"]) / launches
+        achieved = flops / (per_launch_ms / 1000.0) / 1e12
+        return {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                "traffic": measured_traffic("k_refine_waves", workload),
+                "kernel": "k_refine_waves", "avg_launch_ms": per_launch_ms,
+                "launches_per_step": launches, "flop_per_launch": flops,
+                "model": "8 FLOP per bin-sample + 17 FLOP per window sample (FP64 vector FMA "
+                         "peak 78.6 TFLOP/s, the same as MI355X's FP64 matrix peak)"}
+    return {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+            "traffic": None, "kernel": KERNEL_NAMES.get(name, name), "avg_launch_ms": per_launch_ms,
+            "launches_per_step": launches}
 
 
-def world_baseline(x, fs):
+def cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def host_threads():
+    """The host's cores, at most 16 (the share of one GPU's host on the box)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def world_over(xs, fs, threads):
+    """WORLD Harvest of every utterance on `threads` host threads, longest
+    first (each call releases the GIL); returns (f0 list, seconds, kind)."""
     from oracle import oracle
-    if oracle.ref_available():
-        kind, fn = "reference", oracle.ref_harvest
-    else:
-        kind, fn = "port", oracle.harvest
+    kind, fn = ("reference", oracle.ref_harvest) if oracle.ref_available() else ("port", oracle.harvest)
+    order = sorted(range(len(xs)), key=lambda i: -len(xs[i]))
+    out = [None] * len(xs)
     t0 = time.perf_counter()
-    f0, _ = fn(x, fs)
-    dt = time.perf_counter() - t0
-    return f0, dt, kind
+    with ThreadPoolExecutor(threads) as pool:
+        for i, (f0, _) in zip(order, pool.map(lambda i: fn(xs[i], fs), order)):
+            out[i] = f0
+    return out, time.perf_counter() - t0, kind
 
 
-def parity(actual, expected):
+def accuracy(expected, actual):
+    """The acceptance bar over many tracks, and the reference benchmark's
+    figures (harvest_cuda.metrics.compare) over their concatenation."""
     import numpy as np
-    v = expected > 0
-    both = v & (actual > 0)
-    rel = float(np.max(np.abs(actual[both] - expected[both]) / expected[both])) if both.any() else 0.0
-    return {"vuv_identical": bool(np.array_equal(actual > 0, v)),
-            "vuv_mismatches": int(np.count_nonzero((actual > 0) != v)),
-            "max_rel_err": rel, "voiced_frames": int(v.sum())}
+    from harvest_cuda.metrics import compare
+    vuv_bad, worst, bad_utts = 0, 0.0, 0
+    for e, a in zip(expected, actual):
+        v = e > 0
+        mism = int(np.count_nonzero(v != (a > 0)))
+        both = v & (a > 0)
+        rel = float(np.max(np.abs(a[both] - e[both]) / e[both])) if both.any() else 0.0
+        vuv_bad += mism
+        worst = max(worst, rel)
+        bad_utts += mism > 0 or rel > 1e-6
+    acc = compare(np.concatenate(expected), np.concatenate(actual))
+    return {"utterances": len(expected), "utterances_failing": bad_utts,
+            "vuv_identical": vuv_bad == 0, "vuv_mismatches": vuv_bad, "max_rel_err": worst,
+            "within_1e-6": bad_utts == 0, "compare": acc}
 
 
-def batched_leg(device, warmup):
+def batched_leg(args, result):
     import numpy as np
-    import torch
-    from harvest_cuda import _core
+    import harvest_cuda
     from harvest_cuda.signals import config3
     t0 = time.perf_counter()
     xs, fs = config3()
-    log(f"config 3 generated in {time.perf_counter() - t0:.1f} s")
     lengths = [len(x) for x in xs]
     audio = sum(lengths) / fs
-    flat = torch.from_numpy(np.concatenate(xs)).to(device)
-    torch.cuda.synchronize()
-    base = flat.data_ptr()
-    offs = np.concatenate([[0], np.cumsum(lengths)])[:-1]
+    log(f"config 3: {len(xs)} utterances, {audio:.1f} audio-s, generated in "
+        f"{time.perf_counter() - t0:.1f} s")
 
-    def batched():
-        return _core.harvest_device(base, lengths, fs)
-
-    def plain():
-        return [_core.harvest_device(base + 8 * int(o), [n], fs)[0] for o, n in zip(offs, lengths)]
-
-    for _ in range(warmup):
-        batched()
-    plain()
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    rb = batched()
-    torch.cuda.synchronize()
-    tb = time.perf_counter() - t0
+    harvest_cuda.harvest_batch(xs, fs)
+    first_ms = (time.perf_counter() - t0) * 1000.0
+    for _ in range(args.warmup):  # with timings, so that their events exist before timing
+        harvest_cuda._core.harvest_batch(xs, fs, timings=True)
+    kernels = {k: 0.0 for k in KERNEL_KEYS}
+    work = {"window_samples": 0.0, "bin_samples": 0.0}
+    device_ms, launches = 0.0, 0
+    steps_ms = []
     t0 = time.perf_counter()
-    rp = plain()
+    last = t0
+    for _ in range(args.steps):
+        out, tm = harvest_cuda._core.harvest_batch(xs, fs, timings=True)
+        now = time.perf_counter()
+        steps_ms.append((now - last) * 1000.0)
+        last = now
+        for k in KERNEL_KEYS:
+            kernels[k] += tm[k]
+        work["window_samples"] += tm["work_window_samples"]
+        work["bin_samples"] += tm["work_bin_samples"]
+        device_ms += tm["preprocess"] + tm["filter_events"] + tm["refine"]
+        launches += int(tm.get("groups", 1))
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed / args.steps * 1000.0
+    log("batched step ms: " + " ".join(f"{v:.1f}" for v in steps_ms))
+    batched = out
+
+    # Plain: one harvest() per utterance, same scope.
+    plain_ms = []
+    plain = None
+    for rep in range(args.plain_repeats + 1):  # the first loop warms every shape's buffers
+        t1 = time.perf_counter()
+        plain = [harvest_cuda.harvest(x, fs) for x in xs]
+        if rep:
+            plain_ms.append((time.perf_counter() - t1) * 1000.0)
+    identical = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+                    for a, b in zip(batched, plain))
+    plain_med = statistics.median(plain_ms) if plain_ms else None
+    log(f"batched {ms_per_step:.1f} ms/step ({audio / ms_per_step * 1000:.0f} audio-s/s), plain "
+        f"{plain_med and round(plain_med, 1)} ms, bitwise equal {identical}")
+
+    steps = args.steps
+    workload = "config3"
+    result.update({
+        "value": audio / (ms_per_step / 1000.0), "ms_per_step": ms_per_step,
+        "config": {"workload": "512 float64 utterances at 24 kHz, lengths log-uniform 0.5-30 s "
+                               "(seed 1003), one batched call, NumPy in and out",
+                   "fs": fs, "utterances": len(xs), "audio_seconds": audio,
+                   "f0_floor": 71.0, "f0_ceil": 800.0, "frame_period": 5.0},
+        "step_ms": {"median": statistics.median(steps_ms), "min": min(steps_ms),
+                    "max": max(steps_ms)},
+        "first_call_ms": first_ms,
+        "plain": {"ms_per_loop_median": plain_med, "loops": len(plain_ms),
+                  "audio_sec_per_sec": audio / (plain_med / 1000.0) if plain_med else None,
+                  "calls_per_loop": len(xs),
+                  "note": "one harvest(x) call per utterance, NumPy in and out"},
+        "batched_over_plain": (plain_med / ms_per_step) if plain_med else None,
+        "batched_equals_plain_bitwise": bool(identical) if plain_ms else None,
+        "roofline": roofline({k: v / steps for k, v in kernels.items()},
+                             {k: v / steps for k, v in work.items()}, launches // steps, workload),
+        "kernels_ms_per_step": {k: v / steps for k, v in kernels.items()},
+        "device_ms_per_step": device_ms / steps,
+    })
+    a_bytes = algorithmic_bytes(lengths, fs)
+    dev_s = device_ms / steps / 1000.0
+    result["algorithmic_hbm"] = {
+        "bytes_per_step": a_bytes, "gbps_device_span": a_bytes / dev_s / 1e9,
+        "gbps_wall": a_bytes / (ms_per_step / 1000.0) / 1e9,
+        "frac_of_peak_device_span": a_bytes / dev_s / 1e9 / HBM_PEAK_GBS,
+        "model": "8L + 8N + 16(n/2+1)(2B+1) + 16BF + 24FC per utterance (global-n spectra; this "
+                 "design never materialises them, so the figure is a normalised throughput)"}
+    if args.cpu_baseline:
+        threads = host_threads()
+        ref, dt, kind = world_over(xs, fs, threads)
+        result["cpu_baseline"] = {
+            "value": audio / dt, "unit": "audio-sec/sec", "cores": threads, "kind": kind,
+            "cpu": cpu_model(), "seconds": dt,
+            "sample": f"all {len(xs)} utterances ({audio:.0f} audio-s), one WORLD Harvest call "
+                      f"each on a pool of {threads} threads, longest first"}
+        result["parity"] = dict(accuracy(ref, [f for f, _ in batched]),
+                                against="WORLD Harvest" if kind == "reference" else "oracle restatement")
+        log(f"cpu baseline ({kind}, {threads} threads): {audio / dt:.1f} audio-s/s; "
+            f"parity {result['parity']}")
+    return xs, fs
+
+
+def config2_leg(args):
+    import numpy as np
+    import torch
+    from harvest_cuda import _core
+    from harvest_cuda.signals import config2
+    x, fs = config2()
+    seconds = len(x) / fs
+    dx = torch.from_numpy(x).to("cuda:0")
     torch.cuda.synchronize()
-    tp = time.perf_counter() - t0
-    identical = all(np.array_equal(a[0], b[0]) for a, b in zip(rb, rp))
-    return {"workload": "config 3: 512 utterances at 24 kHz, lengths log-uniform 0.5-30 s",
-            "audio_seconds": audio, "batched_audio_sec_per_sec": audio / tb,
-            "plain_audio_sec_per_sec": audio / tp, "speedup": tp / tb,
-            "batched_ms": tb * 1000, "plain_ms": tp * 1000,
-            "batched_equals_plain": bool(identical)}
+    ptr = dx.data_ptr()
+    for _ in range(5):
+        _core.harvest_device(ptr, [len(x)], fs, timings=True)
+    steps = max(args.steps, 10)
+    kernels = {k: 0.0 for k in KERNEL_KEYS}
+    work = {"window_samples": 0.0, "bin_samples": 0.0}
+    device_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out, tm = _core.harvest_device(ptr, [len(x)], fs, timings=True)
+        for k in KERNEL_KEYS:
+            kernels[k] += tm[k]
+        work["window_samples"] += tm["work_window_samples"]
+        work["bin_samples"] += tm["work_bin_samples"]
+        device_ms += tm["preprocess"] + tm["filter_events"] + tm["refine"]
+    hbm_ms = (time.perf_counter() - t0) / steps * 1000.0
+    for _ in range(2):
+        _core.harvest(x, fs)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        f0, _t = _core.harvest(x, fs)
+    host_ms = (time.perf_counter() - t0) / steps * 1000.0
+    same = bool(np.array_equal(out[0][0], f0))
+    return {"workload": "one 60 s float64 signal at 48 kHz (seed 1002), plain drop-in call",
+            "steps": steps, "host_input": {"ms_per_call": host_ms,
+                                           "audio_sec_per_sec": seconds / (host_ms / 1000.0)},
+            "hbm_input": {"ms_per_call": hbm_ms, "audio_sec_per_sec": seconds / (hbm_ms / 1000.0)},
+            "device_ms": device_ms / steps,
+            "kernels_ms_per_call": {k: v / steps for k, v in kernels.items()},
+            "roofline": roofline({k: v / steps for k, v in kernels.items()},
+                                 {k: v / steps for k, v in work.items()}, 1, "config2"),
+            "host_equals_hbm_input": same}
+
+
+def config5_leg(args):
+    import numpy as np
+    import harvest_cuda
+    from harvest_cuda.signals import config5_recording
+    n = args.long
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(n) as pool:
+        recs = list(pool.map(lambda i: config5_recording(i)[0], range(n)))
+    fs = 44100
+    log(f"config 5: {n} recordings generated in {time.perf_counter() - t0:.1f} s")
+    harvest_cuda.harvest(recs[-1], fs)  # the first call per shape allocates ~15 GB
+    times, outs = [], []
+    last = time.perf_counter()
+    for f0, _ in harvest_cuda.harvest_stream(iter(recs), fs):
+        now = time.perf_counter()
+        times.append((now - last) * 1000.0)
+        last = now
+        outs.append(f0)
+    seconds = len(recs[0]) / fs
+    a_bytes = algorithmic_bytes([len(recs[0])], fs)
+    res = {"workload": f"{n} recordings of 30 min at 44.1 kHz (seed 1005) through harvest_stream",
+           "ms_per_recording": times, "audio_sec_per_sec": n * seconds / (sum(times) / 1000.0),
+           "algorithmic_gbps": a_bytes * n / (sum(times) / 1000.0) / 1e9}
+    golden = ROOT / "tests" / "golden" / "world_long.npz"
+    if golden.exists():
+        import hashlib
+        g = np.load(golden)
+        if hashlib.sha256(recs[0].tobytes()).hexdigest() == str(g["sha"]):
+            ref = g["f0"].astype(np.float64)
+            res["parity_recording0"] = dict(accuracy([ref], [outs[0]]), against="WORLD Harvest "
+                                            "(stored F0, float32)")
+    return res
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    # A process's first calls include one-time costs: one stall of 6-9 ms on
-    # one of roughly the first 30 calls (not tied to the number of launches,
-    # copies or events on the analysis stream: warming those at workspace
-    # creation does not move it).  The warm-up absorbs it.
-    ap.add_argument("--warmup", type=int, default=40)
-    ap.add_argument("--batched", type=int, default=1, help="also run the config-3 batched leg")
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--plain-repeats", type=int, default=3)
     ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--config2", type=int, default=1, help="also run the 60 s plain-call leg")
+    ap.add_argument("--long", type=int, default=2, help="30-min recordings of the long leg (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB float64 tensor 5 times on the GPU (a kernel of known "
                          "traffic, to calibrate the profiler's FETCH_SIZE/WRITE_SIZE) and exit")
@@ -210,94 +367,23 @@ def main():
               "(utterances would shard across GPUs with no exchange); run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
+    if args.steps < 1:
+        raise SystemExit("--steps must be at least 1")
 
-    import numpy as np
-    import torch
     import harvest_cuda
-    from harvest_cuda import _core
-    from harvest_cuda.signals import config2
-
     if not harvest_cuda.cuda_available(0):
         raise SystemExit("no supported gfx950 GPU visible")
-    device = torch.device("cuda:0")
-    x, fs = config2()
-    seconds = len(x) / fs
-    dx = torch.from_numpy(x).to(device)
-    torch.cuda.synchronize()
-    ptr = dx.data_ptr()
-
-    def step(timed):
-        return _core.harvest_device(ptr, [len(x)], fs, timings=timed)
-
-    for _ in range(args.warmup):  # with timings, so that their events exist before timing
-        step(True)
-    kernel_keys = ["kernel_decimate", "kernel_filterbank", "kernel_interp", "kernel_refine",
-                   "kernel_prune_compact"]
-    kernels = {k: 0.0 for k in kernel_keys}
-    work = {"window_samples": 0.0, "bin_samples": 0.0}
-    device_ms = 0.0  # hipEvent span of the device launch sequence
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    marks = []
-    for _ in range(args.steps):
-        out, tm = step(True)
-        marks.append(time.perf_counter())
-        for k in kernel_keys:
-            kernels[k] += tm[k]
-        work["window_samples"] += tm["work_window_samples"]
-        work["bin_samples"] += tm["work_bin_samples"]
-        device_ms += tm["preprocess"] + tm["filter_events"] + tm["refine"]
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = elapsed / args.steps * 1000.0
-    per = [(b - a) * 1000.0 for a, b in zip([t0] + marks[:-1], marks)]
-    log("step ms: " + " ".join(f"{v:.2f}" for v in per))
-    f0 = out[0][0]
-    name, roof = roofline(kernels, work, [len(x)], fs, args.steps)
-    log(f"plain config 2: {ms_per_step:.3f} ms/step, dominant {name} {roof['avg_ms']:.3f} ms")
-
-    result = {
-        "metric": METRIC, "value": seconds / (ms_per_step / 1000.0), "unit": "audio-sec/sec",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": "config 2: one 60 s float64 signal at 48 kHz, plain drop-in call, "
-                               "input resident in HBM, f0 returned to the host",
-                   "fs": fs, "seconds": seconds, "f0_floor": 71.0, "f0_ceil": 800.0,
-                   "frame_period": 5.0},
-        "roofline": roof,
-        "kernels_ms_per_step": {k: v / args.steps for k, v in kernels.items()},
-    }
-    # The same call with the waveform on the host (NumPy in, PCIe upload
-    # included): the drop-in API's rate, reported beside the primary value.
-    for _ in range(2):
-        _core.harvest(x, fs)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        _core.harvest(x, fs)
-    host_ms = (time.perf_counter() - t0) / args.steps * 1000.0
-    result["host_input"] = {"ms_per_step": host_ms, "audio_sec_per_sec": seconds / (host_ms / 1000.0),
-                            "note": "harvest(x) with x a host NumPy array: includes the H2D copy"}
-    a_bytes = algorithmic_bytes([len(x)], fs)
-    dev_s = device_ms / args.steps / 1000.0
-    result["algorithmic_hbm"] = {
-        "bytes_per_step": a_bytes, "device_ms": dev_s * 1000.0,
-        "gbps_device_span": a_bytes / dev_s / 1e9, "gbps_wall": a_bytes / (ms_per_step / 1000.0) / 1e9,
-        "frac_of_peak_device_span": a_bytes / dev_s / 1e9 / HBM_PEAK_GBS,
-        "model": "8L + 8N + 16(n/2+1)(2B+1) + 16BF + 24FC (global-n spectra; this design "
-                 "never materialises them, so the figure is a normalised throughput)"}
-    if args.cpu_baseline:
-        ref_f0, dt, kind = world_baseline(x, fs)
-        result["cpu_baseline"] = {"value": seconds / dt, "unit": "audio-sec/sec", "cores": 1,
-                                  "kind": kind,
-                                  "sample": "the full config-2 signal (60 s at 48 kHz), one call, "
-                                            "single thread (WORLD Harvest is single-threaded)"}
-        result["parity"] = dict(parity(f0, ref_f0), against="WORLD Harvest" if kind == "reference"
-                                else "oracle restatement")
-        log(f"cpu baseline ({kind}): {seconds / dt:.2f} audio-s/s; parity {result['parity']}")
-    if args.batched:
-        result["batched"] = batched_leg(device, 1)
-        log(f"batched: {result['batched']}")
+    result = {"metric": METRIC, "value": None, "unit": "audio-sec/sec", "n_gpus": 1,
+              "steps": args.steps, "warmup": args.warmup, "ms_per_step": None,
+              "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+              "data": "synthetic"}
+    batched_leg(args, result)
+    if args.config2:
+        result["config2"] = config2_leg(args)
+        log(f"config 2: {result['config2']['host_input']}")
+    if args.long > 0:
+        result["config5"] = config5_leg(args)
+        log(f"config 5: {result['config5']}")
     print(json.dumps(result), flush=True)
 
 

@@ -130,10 +130,10 @@ PYBIND11_MODULE(_core, m) {
       if (device >= kSecondSlot) throw std::invalid_argument("device index out of range");
       std::vector<int64_t> frames;
       for (int64_t n : lengths) frames.push_back(derive_params(n, fs, floor, ceil, period).frames);
-      if (!timings && pipeline_groups(frames) > 1) {
+      if (pipeline_groups(frames) > 1) {
         Cache::Lease second(cache(), device + kSecondSlot);
         out = analyze_batch(ws.get(), ptrs, lengths, fs, floor, ceil, period, host_threads(),
-                            nullptr, second.get());
+                            timings ? &t : nullptr, second.get());
       } else {
         out = analyze_batch(ws.get(), ptrs, lengths, fs, floor, ceil, period, host_threads(),
                             timings ? &t : nullptr);
@@ -165,10 +165,10 @@ PYBIND11_MODULE(_core, m) {
       std::vector<int64_t> frames;
       for (int64_t n : lengths) frames.push_back(derive_params(n, fs, floor, ceil, period).frames);
       const double* xd = reinterpret_cast<const double*>(x_device);
-      if (!timings && pipeline_groups(frames) > 1) {
+      if (pipeline_groups(frames) > 1) {
         Cache::Lease second(cache(), device + kSecondSlot);
         rs = analyze_batch_device(ws.get(), xd, lengths, fs, floor, ceil, period, host_threads(),
-                                  nullptr, second.get());
+                                  timings ? &t : nullptr, second.get());
       } else {
         rs = analyze_batch_device(ws.get(), xd, lengths, fs, floor, ceil, period, host_threads(),
                                   timings ? &t : nullptr);

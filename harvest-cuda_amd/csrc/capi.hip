@@ -1001,6 +1001,13 @@ hv_status hv_analyze(hv_workspace* ws, const double* y, int64_t y_length, int ff
       invalid("invalid analysis sizes");
     if (fft_size < y_length || (fft_size & (fft_size - 1)) != 0)
       invalid("fft_size must be a power of two no smaller than y_length");
+    if (!std::isfinite(fs_work) || fs_work <= 0) invalid("fs_work must be finite and positive");
+    if (!std::isfinite(f0_floor) || !std::isfinite(f0_ceil) || f0_floor <= 0 || f0_ceil <= f0_floor)
+      invalid("require finite 0 < f0_floor < f0_ceil");
+    if (f0_ceil >= fs_work / 2) invalid("f0_ceil must be below the internal Nyquist frequency");
+    for (int b = 0; b < bands; ++b)
+      if (!std::isfinite(boundaries[b]) || boundaries[b] <= 0)
+        invalid("band boundaries must be finite and positive");
     // Frames map to filterbank tiles with integer arithmetic: fs_work = num / den.
     int den = 0;
     for (int d = 1; d <= 12 && !den; ++d)

@@ -890,41 +890,64 @@ int pipeline_groups(const std::vector<int64_t>& frames) {
 }
 
 namespace {
-// Groups of consecutive utterances with about equal total frames.
-std::vector<int> group_bounds(const std::vector<int64_t>& frames, int groups) {
+// Length buckets of a batch: `order` lists the utterances longest first (or
+// in the given order for back-to-back device input), and group g holds
+// order[bounds[g] .. bounds[g + 1]), a run of about total / groups frames.
+struct Buckets {
+  std::vector<int> order, bounds;
+};
+
+Buckets make_buckets(const std::vector<int64_t>& frames, int groups, bool by_length) {
+  Buckets b;
+  const int count = (int)frames.size();
+  b.order.resize(count);
+  std::iota(b.order.begin(), b.order.end(), 0);
+  if (by_length)
+    std::stable_sort(b.order.begin(), b.order.end(),
+                     [&](int x, int y) { return frames[x] > frames[y]; });
   int64_t total = 0;
   for (int64_t f : frames) total += f;
-  std::vector<int> bounds{0};
+  b.bounds = {0};
   int64_t acc = 0;
-  for (int u = 0; u < (int)frames.size(); ++u) {
-    acc += frames[u];
-    const int g = (int)bounds.size();
-    if (g < groups && acc * groups >= total * g && u + 1 < (int)frames.size()) bounds.push_back(u + 1);
+  for (int k = 0; k < count; ++k) {
+    acc += frames[b.order[k]];
+    const int g = (int)b.bounds.size();
+    if (g < groups && acc * groups >= total * g && k + 1 < count) b.bounds.push_back(k + 1);
   }
-  bounds.push_back((int)frames.size());
-  return bounds;
+  b.bounds.push_back(count);
+  return b;
 }
 
-// `submit(w, first, n)` enqueues utterances [first, first + n) on workspace w.
+// Adds one group's device spans, kernel sums and work counters to `t`.
+void accumulate_device(hv_workspace* ws, const double* ms, Timings& t) {
+  Timings one;
+  record_device(ms, one);
+  record_stats(ws, one);
+  for (const auto& kv : one) t[kv.first] += kv.second;
+}
+
+// `submit(w, idx, n, timed)` enqueues utterances idx[0 .. n) on workspace w.
+// Groups alternate between the two workspaces: while one group's kernels
+// run, the host stages the next group's input and runs the previous group's
+// contours.
 template <class Submit>
 std::vector<Result> batch_pipelined(hv_workspace* ws0, hv_workspace* ws1, Submit&& submit,
-                                    const std::vector<int64_t>& lengths, int fs,
-                                    double f0_floor, double f0_ceil, double frame_period,
-                                    int host_threads) {
+                                    const std::vector<int64_t>& lengths, const Buckets& bk,
+                                    int fs, double f0_floor, double f0_ceil, double frame_period,
+                                    int host_threads, Timings* timings) {
+  Clock::time_point mark = Clock::now();
+  if (timings) timings->clear();
   const int count = (int)lengths.size();
   std::vector<Params> params(count);
-  std::vector<int64_t> frames(count);
-  for (int u = 0; u < count; ++u) {
+  for (int u = 0; u < count; ++u)
     params[u] = derive_params(lengths[u], fs, f0_floor, f0_ceil, frame_period);
-    frames[u] = params[u].frames;
-  }
-  const std::vector<int> bounds = group_bounds(frames, pipeline_groups(frames));
-  const int groups = (int)bounds.size() - 1;
+  const int groups = (int)bk.bounds.size() - 1;
   hv_workspace* ws[2] = {ws0, ws1};
   bool pending[2] = {false, false};
   std::vector<Result> results(count);
   auto enqueue = [&](int g) {
-    throw_status(submit(ws[g % 2], bounds[g], bounds[g + 1] - bounds[g]));
+    throw_status(submit(ws[g % 2], bk.order.data() + bk.bounds[g], bk.bounds[g + 1] - bk.bounds[g],
+                        timings != nullptr));
     pending[g % 2] = true;
   };
   try {
@@ -933,23 +956,28 @@ std::vector<Result> batch_pipelined(hv_workspace* ws0, hv_workspace* ws1, Submit
     for (int g = 0; g < groups; ++g) {
       hv_workspace* w = ws[g % 2];
       int64_t nnz = 0;
+      double ms[HV_TIMING_SLOTS];
       pending[g % 2] = false;
-      throw_status(hv_analyze_batch_collect(w, nullptr, nullptr, &nnz, nullptr));
+      throw_status(hv_analyze_batch_collect(w, nullptr, nullptr, &nnz, timings ? ms : nullptr));
+      if (timings) accumulate_device(w, ms, *timings);
       const View v = result_view(w);
-      const int first = bounds[g], n = bounds[g + 1] - first;
+      const int* idx = bk.order.data() + bk.bounds[g];
+      const int n = bk.bounds[g + 1] - bk.bounds[g];
       std::vector<int64_t> frame_off(n + 1, 0);
-      for (int k = 0; k < n; ++k) frame_off[k + 1] = frame_off[k] + frames[first + k];
-      std::vector<int> order(n);
-      std::iota(order.begin(), order.end(), 0);
-      std::sort(order.begin(), order.end(),
-                [&](int a, int b) { return frames[first + a] > frames[first + b]; });
+      for (int k = 0; k < n; ++k) frame_off[k + 1] = frame_off[k] + params[idx[k]].frames;
+      // Longest utterance first over the host pool.
+      std::vector<int> by_len(n);
+      std::iota(by_len.begin(), by_len.end(), 0);
+      std::stable_sort(by_len.begin(), by_len.end(), [&](int a, int b) {
+        return params[idx[a]].frames > params[idx[b]].frames;
+      });
       const int inner = std::max(1, std::min(kSectionThreads, host_threads / std::max(1, n)));
       parallel_for(n, host_threads, [&](int k) {
-        const int u = order[k];
-        CandidateRows rows{v.offsets + frame_off[u], v.f0, v.scores, w};
-        finish_utterance(rows, params[first + u].frames, v.best + frame_off[u], frame_period,
-                         params[first + u].out_frames, results[first + u], nullptr, nullptr,
-                         nullptr, inner);
+        const int local = by_len[k];
+        const int u = idx[local];
+        CandidateRows rows{v.offsets + frame_off[local], v.f0, v.scores, w};
+        finish_utterance(rows, params[u].frames, v.best + frame_off[local], frame_period,
+                         params[u].out_frames, results[u], nullptr, nullptr, nullptr, inner);
       });
       // The workspace's pinned results are read: it takes the group after next.
       if (g + 2 < groups) enqueue(g + 2);
@@ -962,7 +990,19 @@ std::vector<Result> batch_pipelined(hv_workspace* ws0, hv_workspace* ws1, Submit
     }
     throw;
   }
+  if (timings) {
+    (*timings)["groups"] = groups;
+    (*timings)["call"] = ms_since(mark);
+  }
   return results;
+}
+
+std::vector<int64_t> frames_of(const std::vector<int64_t>& lengths, int fs, double f0_floor,
+                               double f0_ceil, double frame_period) {
+  std::vector<int64_t> frames(lengths.size());
+  for (size_t u = 0; u < lengths.size(); ++u)
+    frames[u] = derive_params(lengths[u], fs, f0_floor, f0_ceil, frame_period).frames;
+  return frames;
 }
 }  // namespace
 
@@ -970,16 +1010,24 @@ std::vector<Result> analyze_batch(hv_workspace* ws, const std::vector<const doub
                                   const std::vector<int64_t>& lengths, int fs,
                                   double f0_floor, double f0_ceil, double frame_period,
                                   int host_threads, Timings* timings, hv_workspace* second) {
-  std::vector<int64_t> frames(lengths.size());
-  for (size_t u = 0; u < lengths.size(); ++u)
-    frames[u] = derive_params(lengths[u], fs, f0_floor, f0_ceil, frame_period).frames;
-  if (second && !timings && pipeline_groups(frames) > 1) {
-    auto submit = [&](hv_workspace* w, int first, int n) {
-      return hv_analyze_batch_submit(w, n, xs.data() + first, lengths.data() + first, fs,
-                                     f0_floor, f0_ceil, 0);
+  const std::vector<int64_t> frames = frames_of(lengths, fs, f0_floor, f0_ceil, frame_period);
+  const int groups = pipeline_groups(frames);
+  if (second && groups > 1) {
+    // Host input: any utterance order can be staged, so groups are length
+    // buckets (longest first).
+    const Buckets bk = make_buckets(frames, groups, true);
+    auto submit = [&](hv_workspace* w, const int* idx, int n, bool timed) {
+      std::vector<const double*> p(n);
+      std::vector<int64_t> l(n);
+      for (int k = 0; k < n; ++k) {
+        p[k] = xs[idx[k]];
+        l[k] = lengths[idx[k]];
+      }
+      return hv_analyze_batch_submit(w, n, p.data(), l.data(), fs, f0_floor, f0_ceil,
+                                     timed ? 1 : 0);
     };
-    return batch_pipelined(ws, second, submit, lengths, fs, f0_floor, f0_ceil, frame_period,
-                           host_threads);
+    return batch_pipelined(ws, second, submit, lengths, bk, fs, f0_floor, f0_ceil, frame_period,
+                           host_threads, timings);
   }
   auto call = [&](int32_t* offsets, double* best, int64_t* nnz, double* ms) {
     return hv_analyze_batch_async(ws, (int)xs.size(), xs.data(), lengths.data(), fs, f0_floor,
@@ -994,18 +1042,20 @@ std::vector<Result> analyze_batch_device(hv_workspace* ws, const double* x_devic
                                          double f0_floor, double f0_ceil, double frame_period,
                                          int host_threads, Timings* timings,
                                          hv_workspace* second) {
-  std::vector<int64_t> frames(lengths.size()), x_off(lengths.size() + 1, 0);
-  for (size_t u = 0; u < lengths.size(); ++u) {
-    frames[u] = derive_params(lengths[u], fs, f0_floor, f0_ceil, frame_period).frames;
-    x_off[u + 1] = x_off[u] + lengths[u];
-  }
-  if (second && !timings && pipeline_groups(frames) > 1) {
-    auto submit = [&](hv_workspace* w, int first, int n) {
-      return hv_analyze_batch_device_submit(w, n, x_device + x_off[first],
-                                            lengths.data() + first, fs, f0_floor, f0_ceil, 0);
+  const std::vector<int64_t> frames = frames_of(lengths, fs, f0_floor, f0_ceil, frame_period);
+  std::vector<int64_t> x_off(lengths.size() + 1, 0);
+  for (size_t u = 0; u < lengths.size(); ++u) x_off[u + 1] = x_off[u] + lengths[u];
+  const int groups = pipeline_groups(frames);
+  if (second && groups > 1) {
+    // Back-to-back device input: groups are runs of consecutive utterances.
+    const Buckets bk = make_buckets(frames, groups, false);
+    auto submit = [&](hv_workspace* w, const int* idx, int n, bool timed) {
+      std::vector<int64_t> l(lengths.begin() + idx[0], lengths.begin() + idx[0] + n);
+      return hv_analyze_batch_device_submit(w, n, x_device + x_off[idx[0]], l.data(), fs,
+                                            f0_floor, f0_ceil, timed ? 1 : 0);
     };
-    return batch_pipelined(ws, second, submit, lengths, fs, f0_floor, f0_ceil, frame_period,
-                           host_threads);
+    return batch_pipelined(ws, second, submit, lengths, bk, fs, f0_floor, f0_ceil, frame_period,
+                           host_threads, timings);
   }
   auto call = [&](int32_t* offsets, double* best, int64_t* nnz, double* ms) {
     return hv_analyze_batch_device_async(ws, (int)lengths.size(), x_device, lengths.data(), fs,

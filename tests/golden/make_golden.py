@@ -30,8 +30,9 @@ import numpy as np
 ROOT = Path(__file__).resolve().parents[2]
 sys.path.insert(0, str(ROOT / "harvest-cuda_amd"))
 sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "tests"))
 
-from harvest_cuda.signals import reference_signal  # noqa: E402
+from ref_signals import reference_signal  # noqa: E402
 from oracle import oracle  # noqa: E402
 
 OUT = Path(__file__).resolve().parent

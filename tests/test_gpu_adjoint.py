@@ -11,7 +11,8 @@ import pytest
 import torch
 
 import reexpress
-from harvest_cuda.signals import config4, config4_target, reference_signal
+from harvest_cuda.signals import config4, config4_target
+from ref_signals import reference_signal
 
 pytestmark = pytest.mark.gpu
 

@@ -15,7 +15,8 @@ import numpy as np
 import pytest
 from conftest import GOLDEN
 
-from harvest_cuda.signals import config1, config2, reference_signal
+from harvest_cuda.signals import config1, config2
+from ref_signals import reference_signal
 
 pytestmark = pytest.mark.gpu
 RTOL = 1e-6

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 from conftest import GOLDEN
 
-from harvest_cuda.signals import reference_signal
+from ref_signals import reference_signal
 
 
 def sha(x):
