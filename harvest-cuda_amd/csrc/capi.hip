@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -160,6 +161,38 @@ void chunk_power(const double* co, double* out) {
     std::memcpy(r, t, sizeof(r));
   }
   for (int k = 0; k < 9; ++k) out[k] = (double)r[k];
+}
+
+}  // namespace
+}  // namespace hv
+
+// Device memory of the allocation-free decimation entry point: one array per
+// GPU, held by the loaded code object until the process ends.  Host code
+// reaches each device's copy only through the address hipGetSymbolAddress
+// returns with that device current (scratch_base); the array's name in host
+// code is not a device address.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace hv {
+namespace {
+
+std::mutex g_scratch_mutex;
+std::vector<void*> g_scratch_base;  // per device; sized once by the first call
+
+// Address of this device's scratch array; the caller has made `device`
+// current.  Looked up once per device.
+void* scratch_base(int device, int devices) {
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_base.empty()) g_scratch_base.assign(std::max(devices, 1), nullptr);
+  if (device >= (int)g_scratch_base.size()) invalid("device is out of range");
+  if (!g_scratch_base[device]) {
+    void* ptr = nullptr;
+    check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!ptr) throw Failure(HV_EDEVICE, "scratch array has no device address");
+    g_scratch_base[device] = ptr;
+  }
+  return g_scratch_base[device];
 }
 
 }  // namespace
@@ -987,6 +1020,71 @@ hv_status hv_decimate(hv_workspace* ws, const double* x, int64_t length, int rat
     check(hipMemcpyAsync(hy, dy, y_length * sizeof(double), hipMemcpyDeviceToHost, s), "download y");
     check(hipStreamSynchronize(s), "decimate");
     std::memcpy(y, hy, y_length * sizeof(double));
+  });
+}
+
+hv_status hv_decimate_device_noalloc(int device, const double* x_device, int64_t length,
+                                     int ratio, double* y_device, int64_t y_length,
+                                     void* stream) {
+  return guarded([&] {
+    if (!x_device || !y_device) invalid("null pointer");
+    if (ratio < 1 || ratio > 12) invalid("ratio must be between 1 and 12");
+    if (length < 2LL * ratio || length > 0x7fffffffLL) invalid("invalid length");
+    if (y_length != (length + ratio - 1) / ratio) invalid("y_length must be ceil(length / ratio)");
+    if (device < 0) invalid("device must be a nonnegative visible GPU index");
+    int devices = 0;
+    check(hipGetDeviceCount(&devices), "hipGetDeviceCount");
+    if (device >= devices) invalid("device " + std::to_string(device) + " is out of range");
+    // The plan of one utterance, as layout() builds it for a batch of one.
+    Config cfg{};
+    fill_config(cfg, 1, ratio, 8000 * ratio, 8000.0, 71.0, 800.0, 0, 0, 0);
+    Utt u{};
+    u.length = length;
+    u.n = (int)y_length;
+    if (ratio > 1) {
+      const long long padded = length + 2LL * cfg.lag;
+      u.m = padded + 18;
+      const long long nout = (padded - 1) / ratio + 1;
+      u.nbeg = (int)(ratio - ratio * nout + padded);
+    }
+    u.chunks = (int)((u.m + kChunk - 1) / kChunk);
+    // Scratch layout: plan words, pass buffer, chunk states, carries, carry
+    // scratch; each part 256-byte aligned.
+    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+    constexpr int kUttWords = (int)(sizeof(Utt) / 8);
+    static_assert(sizeof(Utt) % 8 == 0 && kUttWords + 4 <= kStoreWords, "plan words");
+    const size_t at_pass = 256;
+    const size_t at_states = at_pass + up((size_t)u.m * sizeof(double));
+    const size_t at_carries = at_states + up((size_t)3 * u.chunks * sizeof(double));
+    const size_t at_tmp = at_carries + up((size_t)3 * u.chunks * sizeof(double));
+    const size_t need = at_tmp + up(carry_scratch_doubles(1, u.chunks) * sizeof(double));
+    if (need > kScratchBytes)
+      invalid("input needs " + std::to_string(need) + " bytes of scratch; the allocation-free "
+              "entry point holds " + std::to_string(kScratchBytes) +
+              " (inputs up to about 7.6 million samples): use hv_decimate");
+    DeviceGuard guard(device);
+    char* base = static_cast<char*>(scratch_base(device, devices));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // The utterance, then its chunk and output offset tables ([0, total]).
+    StoreWords words{};
+    std::memcpy(words.w, &u, sizeof(Utt));
+    words.w[kUttWords + 0] = 0;
+    words.w[kUttWords + 1] = (unsigned long long)u.chunks;
+    words.w[kUttWords + 2] = 0;
+    words.w[kUttWords + 3] = (unsigned long long)y_length;
+    auto* plan = reinterpret_cast<unsigned long long*>(base);
+    launch_store_words(s, words, kUttWords + 4, plan);
+    const Utt* d_utt = reinterpret_cast<const Utt*>(plan);
+    const long long* d_c_off = reinterpret_cast<const long long*>(plan + kUttWords);
+    const long long* d_y_off = reinterpret_cast<const long long*>(plan + kUttWords + 2);
+    if (ratio > 1)
+      launch_decimate(s, cfg, d_utt, d_c_off, u.chunks, u.chunks, x_device,
+                      reinterpret_cast<double*>(base + at_pass),
+                      reinterpret_cast<double*>(base + at_states),
+                      reinterpret_cast<double*>(base + at_carries),
+                      reinterpret_cast<double*>(base + at_tmp), y_device);
+    else
+      launch_copy_input(s, cfg, d_utt, d_y_off, y_length, x_device, y_device);
   });
 }
 

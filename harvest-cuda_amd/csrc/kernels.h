@@ -111,4 +111,13 @@ void launch_refine_entries(hipStream_t s, const Config& c, const Utt* utt,
                            const double* f0_init, const double* weight, const double* y,
                            double* refined, double* gy);
 
+// Up to kStoreWords 64-bit words passed by value, stored to dst[0, n) by one
+// small kernel: puts a launch plan into device memory with no host staging
+// buffer and no copy from host memory.
+constexpr int kStoreWords = 32;
+struct StoreWords {
+  unsigned long long w[kStoreWords];
+};
+void launch_store_words(hipStream_t s, const StoreWords& words, int n, unsigned long long* dst);
+
 }  // namespace hv

@@ -533,6 +533,12 @@ __global__ __launch_bounds__(kThreads) void k_fill_int(int* p, long long n, int 
   if (i < n) p[i] = v;
 }
 
+__global__ __launch_bounds__(64) void k_store_words(StoreWords words, int n,
+                                                    unsigned long long* dst) {
+  const int i = threadIdx.x;
+  if (i < n) dst[i] = words.w[i];
+}
+
 // ---------------------------------------------------------------------------
 // Filterbank + crossings.  One workgroup = one (tile, band): kTile outputs
 // from sample kTileStride * tile, of which the first kTileStride own their
@@ -2219,6 +2225,13 @@ void launch_phase_table(hipStream_t s, double* tab, long long P, int fine_bits) 
 void launch_window_table(hipStream_t s, double* tab, int hmax) {
   k_window_table<<<grid_for(65LL * (hmax + 1)), kThreads, 0, s>>>(tab, hmax);
   check_launch("window_table");
+}
+
+void launch_store_words(hipStream_t s, const StoreWords& words, int n, unsigned long long* dst) {
+  if (n <= 0) return;
+  if (n > kStoreWords) throw std::runtime_error("store_words: too many words");
+  k_store_words<<<1, 64, 0, s>>>(words, n, dst);
+  check_launch("store_words");
 }
 
 void launch_fill_int(hipStream_t s, int* p, long long n, int value) {

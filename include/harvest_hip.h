@@ -78,6 +78,21 @@ int hv_workspace_device(const hv_workspace* ws);
 hv_status hv_decimate(hv_workspace* ws, const double* x, int64_t length,
                       int ratio, double* y, int64_t y_length);
 
+/* hv_decimate for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc); it has no
+ * counterpart in the reference.  x_device[length] and y_device[y_length] are
+ * device memory on `device`; the kernels are queued on `stream` (a
+ * hipStream_t, NULL for the null stream) and the call returns without
+ * waiting: synchronise the stream before reading y_device.  No workspace is
+ * used and nothing is allocated: the plan and the scan's intermediate arrays
+ * live in one static 64 MiB device array per GPU, so inputs are limited to
+ * about 7.6 million samples (longer ones return HV_EINVAL).  Calls that
+ * overlap on one GPU share that array and race: serialise them.  Output is
+ * bitwise equal to hv_decimate's. */
+hv_status hv_decimate_device_noalloc(int device, const double* x_device, int64_t length,
+                                     int ratio, double* y_device, int64_t y_length,
+                                     void* stream);
+
 /* Analysis of one decimated, mean-removed waveform y[y_length] at the working
  * rate fs_work: filterbank, crossings, raw/official candidates, refinement,
  * pruning and compaction.  Replaces gpu_analyze (src/engine.h:30-34,
