@@ -190,6 +190,9 @@ def batched_leg(args, result):
         out, tm = harvest_cuda._core.harvest_batch(xs, fs, timings=True)
         now = time.perf_counter()
         steps_ms.append((now - last) * 1000.0)
+        log(f"  step {steps_ms[-1]:.1f} ms: call {tm.get('call', 0):.1f}, host submit "
+            f"{tm.get('host_submit', 0):.1f}, wait {tm.get('host_wait', 0):.1f} (max "
+            f"{tm.get('host_wait_max', 0):.1f}), contour {tm.get('host_contour', 0):.1f}")
         last = now
         for k in KERNEL_KEYS:
             kernels[k] += tm[k]
@@ -205,13 +208,14 @@ def batched_leg(args, result):
     # Plain: one harvest() per utterance, same scope.
     plain_ms = []
     plain = None
-    for rep in range(args.plain_repeats + 1):  # the first loop warms every shape's buffers
+    for rep in range(args.plain_repeats + 1 if args.plain_repeats > 0 else 0):
+        # the first loop warms every shape's buffers
         t1 = time.perf_counter()
         plain = [harvest_cuda.harvest(x, fs) for x in xs]
         if rep:
             plain_ms.append((time.perf_counter() - t1) * 1000.0)
-    identical = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
-                    for a, b in zip(batched, plain))
+    identical = plain is not None and all(
+        np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) for a, b in zip(batched, plain))
     plain_med = statistics.median(plain_ms) if plain_ms else None
     log(f"batched {ms_per_step:.1f} ms/step ({audio / ms_per_step * 1000:.0f} audio-s/s), plain "
         f"{plain_med and round(plain_med, 1)} ms, bitwise equal {identical}")

@@ -945,10 +945,16 @@ std::vector<Result> batch_pipelined(hv_workspace* ws0, hv_workspace* ws1, Submit
   hv_workspace* ws[2] = {ws0, ws1};
   bool pending[2] = {false, false};
   std::vector<Result> results(count);
+  // Host time of each phase (with timings): staging and enqueueing a group,
+  // waiting for its device work, and its contours.
+  double host_submit = 0.0, host_wait = 0.0, host_contour = 0.0, worst_wait = 0.0;
+  Clock::time_point phase = Clock::now();
   auto enqueue = [&](int g) {
+    if (timings) ms_since(phase);
     throw_status(submit(ws[g % 2], bk.order.data() + bk.bounds[g], bk.bounds[g + 1] - bk.bounds[g],
                         timings != nullptr));
     pending[g % 2] = true;
+    if (timings) host_submit += ms_since(phase);
   };
   try {
     enqueue(0);
@@ -958,8 +964,15 @@ std::vector<Result> batch_pipelined(hv_workspace* ws0, hv_workspace* ws1, Submit
       int64_t nnz = 0;
       double ms[HV_TIMING_SLOTS];
       pending[g % 2] = false;
+      if (timings) ms_since(phase);
       throw_status(hv_analyze_batch_collect(w, nullptr, nullptr, &nnz, timings ? ms : nullptr));
-      if (timings) accumulate_device(w, ms, *timings);
+      if (timings) {
+        const double waited = ms_since(phase);
+        host_wait += waited;
+        worst_wait = std::max(worst_wait, waited);
+        accumulate_device(w, ms, *timings);
+        ms_since(phase);
+      }
       const View v = result_view(w);
       const int* idx = bk.order.data() + bk.bounds[g];
       const int n = bk.bounds[g + 1] - bk.bounds[g];
@@ -979,6 +992,7 @@ std::vector<Result> batch_pipelined(hv_workspace* ws0, hv_workspace* ws1, Submit
         finish_utterance(rows, params[u].frames, v.best + frame_off[local], frame_period,
                          params[u].out_frames, results[u], nullptr, nullptr, nullptr, inner);
       });
+      if (timings) host_contour += ms_since(phase);
       // The workspace's pinned results are read: it takes the group after next.
       if (g + 2 < groups) enqueue(g + 2);
     }
@@ -992,6 +1006,10 @@ std::vector<Result> batch_pipelined(hv_workspace* ws0, hv_workspace* ws1, Submit
   }
   if (timings) {
     (*timings)["groups"] = groups;
+    (*timings)["host_submit"] = host_submit;
+    (*timings)["host_wait"] = host_wait;
+    (*timings)["host_wait_max"] = worst_wait;
+    (*timings)["host_contour"] = host_contour;
     (*timings)["call"] = ms_since(mark);
   }
   return results;

@@ -72,6 +72,16 @@ __device__ unsigned long long block_scan_u64(unsigned long long v,
   return before + incl - v;
 }
 
+constexpr int kXcds = 8;  // XCDs that workgroups are dispatched to in turn
+
+// Workgroup b of G is dispatched to XCD b mod 8; renumbered so that each XCD
+// takes one contiguous range of logical blocks (its L2 then holds one slice
+// of the data those blocks share).
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned G) {
+  const unsigned x = b % kXcds, per = G / kXcds, rem = G % kXcds;
+  return x * per + min(x, rem) + b / kXcds;
+}
+
 // ---------------------------------------------------------------------------
 // Decimation.  WORLD runs a 3rd-order IIR forward over the padded signal,
 // then again over the reversed output, and keeps every r-th sample
@@ -654,13 +664,17 @@ __device__ void tile_crossings(const Config& c, const Utt& u, const Quirk& qk, l
 template <bool kLds>
 __global__ __launch_bounds__(kThreads) void k_filter_events(
     Config c, const Utt* utt, const long long* t_off, long long total_tiles,
-    int band0, int slot0, const double* y, const double* coef, const int* coef_off,
+    int band0, int nbands, int slot0, const double* y, const double* coef, const int* coef_off,
     const int* half, const Quirk* quirk, const double2* phase, double* slab, int* counts) {
   extern __shared__ double smem[];
   __shared__ unsigned long long wave_sums[kThreads / 64];
-  const long long g = blockIdx.x;
-  const int bg = slot0 + blockIdx.y;  // the band's slab slot within its group
-  const int b = band0 + blockIdx.y;
+  // Logical block = tile * nbands + band: the bands of a tile run back to back
+  // on one XCD, whose L2 then serves the tile's samples to all of them.
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  const long long g = lb / nbands;
+  const int bi = (int)(lb % nbands);
+  const int bg = slot0 + bi;  // the band's slab slot within its group
+  const int b = band0 + bi;
   const int ui = find_segment(t_off, c.count, g);
   const Utt u = utt[ui];
   const long long i0 = (g - u.t_off) * kTileStride;
@@ -953,8 +967,9 @@ __global__ __launch_bounds__(kThreads) void k_filter_fft(
   __shared__ unsigned long long wave_sums[kThreads / 64];
   LDS double* re = (LDS double*)buf;
   LDS double* im = re + kFftM;
-  const long long g = blockIdx.x / nbands;
-  const int bi = (int)(blockIdx.x % nbands);
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);  // tile * nbands + band
+  const long long g = lb / nbands;
+  const int bi = (int)(lb % nbands);
   const int bg = slot0 + bi;
   const int b = band0 + bi;
   const int ui = find_segment(t_off, c.count, g);
@@ -1042,6 +1057,9 @@ __global__ __launch_bounds__(kThreads) void k_interp(
     long long total_frames, int band0, const double* boundaries,
     const double* slab, const int* counts, const int* offs, const int* totals,
     double* raw) {
+  // Logical block = band * total_tiles + tile: each XCD walks a contiguous
+  // run of one band's tiles, whose crossing slabs (and the adjacent tiles'
+  // margins) it reads.
   // One crossing type at a time: its edges of ranks [wlo, whi] are staged,
   // then the interval locations loc_k = (e_k + e_{k+1}) / 2 / fs and values
   // val_k = fs / (e_{k+1} - e_k) (harvest.cpp:222-236) are computed once, so
@@ -1050,8 +1068,9 @@ __global__ __launch_bounds__(kThreads) void k_interp(
   constexpr int kWin = kSlabCap + 2 * kWindowMargin + 2;
   constexpr int kReps = kInterpFramesPerThread;
   __shared__ double edge[kWin], loc[kWin], val[kWin];
-  const long long g = blockIdx.x;
-  const int bg = blockIdx.y;
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  const long long g = lb % total_tiles;
+  const int bg = (int)(lb / total_tiles);
   const int b = band0 + bg;
   const int ui = find_segment(t_off, c.count, g);
   const Utt u = utt[ui];
@@ -1253,7 +1272,6 @@ constexpr int kPer = 1;          // bins per lane group per pass
 constexpr int kFrameBins = 128;  // bin results of one frame kept in LDS (< 256)
 constexpr int kStage = 128;      // windowed samples staged per pass (multiple of 64)
 constexpr int kMaxWords = 128;   // bitmap words (bins 0..8191)
-constexpr int kXcds = 8;         // XCDs that workgroups are dispatched to in turn
 constexpr int kYPre = 1;         // 64-sample chunks of the next window loaded ahead (more
                                  // cost registers: 6 chunks ran 8 % slower)
 
@@ -1337,39 +1355,26 @@ __device__ __forceinline__ double lane_value(double v, int src) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// Whole-wave DPP shifts (CDNA row masks all set): lane l receives v of lane
-// l + 1 (from_next_lane) or l - 1 (from_prev_lane); the lane without a source
-// receives `edge`.
-template <int kCtrl>
-__device__ __forceinline__ double wave_shift(double v, double edge) {
-  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-  const unsigned long long o = (unsigned long long)__double_as_longlong(edge);
-  const unsigned lo =
-      (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)o, (int)(unsigned)b, kCtrl, 0xF, 0xF, false);
-  const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32),
-                                                            (int)(unsigned)(b >> 32), kCtrl, 0xF,
-                                                            0xF, false);
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+// FFT size of GetMeanF0 for half window h: 2^(2 + floor(log2(2h + 1)))
+// (harvest.cpp:598-599; 2h + 1 is odd, so log / kLog2 there is never within
+// rounding of an integer).  window_fft_log2 is its exponent.
+__host__ __device__ inline int window_fft_log2(int h) {
+  return 2 + (31 - __builtin_clz((unsigned)(2 * h + 1)));
 }
-__device__ __forceinline__ double from_next_lane(double v, double edge) {
-  return wave_shift<0x130>(v, edge);  // wave_shl:1
-}
-__device__ __forceinline__ double from_prev_lane(double v, double edge) {
-  return wave_shift<0x138>(v, edge);  // wave_shr:1
-}
+__host__ __device__ inline int window_fft(int h) { return 1 << window_fft_log2(h); }
 
-// Workgroup b of G is dispatched to XCD b mod 8; renumbered so that each XCD
-// takes one contiguous range (its L2 then holds one slice of the signal).
-__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned G) {
-  const unsigned x = b % kXcds, per = G / kXcds, rem = G % kXcds;
-  return x * per + min(x, rem) + b / kXcds;
+// q / d and q % d for 0 <= q < 2^16 and 1 <= d < 2^16 from m = ceil(2^32 / d):
+// floor(q m / 2^32) = floor(q / d + q e / (d 2^32)) with e = m d - 2^32 < d,
+// and q e / (d 2^32) < 2^-16 <= 1 / d stays below the gap to the next
+// integer, so the quotient is exact.
+// d = 1 is encoded as m = 0 (2^32 does not fit).
+__host__ __device__ inline unsigned small_div_magic(unsigned d) {
+  // ceil of 2^32 / d in FP64: a quotient that is not an integer is at least
+  // 1 / d from one, far beyond its rounding.
+  return d <= 1 ? 0u : (unsigned)ceil(4294967296.0 / (double)d);
 }
-
-// FFT size of GetMeanF0 for half window h: 2^(2 + floor(log2(2h + 1))).
-__host__ __device__ inline int window_fft(int h) {
-  int lg = 0;
-  while ((2 << lg) <= 2 * h + 1) ++lg;
-  return 1 << (lg + 2);
+__device__ __forceinline__ int small_div(int q, unsigned magic) {
+  return magic ? (int)__umulhi((unsigned)q, magic) : q;
 }
 
 // Exclusive prefix sum of v over the wave; returns the wave total.
@@ -1397,14 +1402,21 @@ __device__ __forceinline__ void twiddle_at(const double* tw, int stride, int n, 
   }
 }
 
-// Four waves per SIMD: at most 128 VGPRs (the compiler spills a few dwords)
-// and about 10 KB of LDS per wave at 71-800 Hz; 17 % faster than three.
+// Four waves per SIMD: at most 128 VGPRs and about 10 KB of LDS per wave at
+// 71-800 Hz; 17 % faster than three.  kS, kHS, kW > 0 fix the slot count,
+// hash size and bitmap words at compile time (the default analysis settings):
+// every LDS offset then folds into its ds_* instruction, which frees the
+// scalar registers that would hold the carved pointers.  kS = 0 takes them
+// from the arguments; both instances compute the same values.
+template <int kS, int kHS, int kW>
 __global__ __launch_bounds__(64 * kRefineWaves, 4) void k_refine_waves(
-    Config c, const Utt* utt, const long long* f_off, long long total_frames, int HS, int W,
-    const double* y, const double* official, const int* count, const double* twiddle,
+    Config c, const Utt* utt, const long long* f_off, long long total_frames, int hs_arg,
+    int w_arg, const double* y, const double* official, const int* count, const double* twiddle,
     const double* wintab, double* rf0, double* rscore, unsigned long long* work) {
   extern __shared__ double lds_raw[];
-  const int S = c.slots;
+  const int S = kS > 0 ? kS : c.slots;
+  const int HS = kS > 0 ? kHS : hs_arg;
+  const int W = kS > 0 ? kW : w_arg;
   // The wave index is uniform: readfirstlane lets the compiler keep the
   // frame's scalars (frame, utterance, counts) in scalar registers.
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -1421,7 +1433,9 @@ __global__ __launch_bounds__(64 * kRefineWaves, 4) void k_refine_waves(
   const Utt u = utt[ui];
   const long long i = gf - u.f_off;
   const double fs = c.fs_work;
-  const double t = (double)i / 1000.0;
+  // The frame time is uniform: held in scalar registers, not in a VGPR pair
+  // that would be spilled across the window loop.
+  const double t = lane_value((double)i / 1000.0, 0);
   const double* yu = y + u.y_off;
   auto y_at = [&](int idx) -> double {  // GetBaseIndex's clamping (harvest.cpp:434-441)
     return yu[idx < 0 ? 0 : (idx > u.n - 1 ? u.n - 1 : idx)];
@@ -1435,8 +1449,9 @@ __global__ __launch_bounds__(64 * kRefineWaves, 4) void k_refine_waves(
   for (int k = lane; k < HS; k += 64) L.hkey[k] = -1;
   if (lane == 0) L.misc[0] = 0;
   wave_sync();
+  const unsigned cnt_magic = small_div_magic((unsigned)cnt);
   for (int s = lane; s < ncand; s += 64) {
-    const int g = s / cnt, jj = s % cnt;
+    const int g = small_div(s, cnt_magic), jj = s - g * cnt;
     long long src = i;
     if (g >= 1 && g <= 3) src = i - g;
     else if (g >= 4) src = i + (g - 3);
@@ -1538,21 +1553,28 @@ __global__ __launch_bounds__(64 * kRefineWaves, 4) void k_refine_waves(
 
   // FixF0 (harvest.cpp:507-536) and GetRefinedF0's gate (:610-614) of the
   // slot at grouped position p from its harmonics' results res[jidx].
+  // The divisions of FixF0 that feed no integer decision become products
+  // with reciprocals (1 / (2 pi), 1 / (h + 1), 1 / f0): within two ulps of
+  // WORLD's quotients, which the spectra's own rounding far exceeds.
+  constexpr double kInvTwoPi = 0.5 / kPi;
   auto finish = [&](int p) {
     const int s = L.wslots[p];
     const double f0 = L.f0[s];
     const int H = L.nh[s];
-    const double inv_n = 1.0 / window_fft(L.h[s]);  // exact: a power of two
+    const double inv_n = ldexp(1.0, -window_fft_log2(L.h[s]));  // 1 / n, exact
+    const double inv_f0 = 1.0 / f0;
     double numer = 0.0, den = 0.0, dev = 0.0;
     for (int hh = 0; hh < H; ++hh) {
+      const double inv_h = hh == 0 ? 1.0 : hh == 1 ? 0.5 : hh == 2 ? 1.0 / 3.0
+                         : hh == 3 ? 0.25 : hh == 4 ? 0.2 : 1.0 / 6.0;
       const int k = L.pk[6 * p + hh];
       const int j = L.jidx[6 * p + hh];
       const double pw = L.res[2 * j], nu = L.res[2 * j + 1];
-      const double ifr = pw == 0.0 ? 0.0 : (double)k * fs * inv_n + nu / pw * fs * 0.5 / kPi;
+      const double ifr = pw == 0.0 ? 0.0 : (double)k * fs * inv_n + nu / pw * fs * kInvTwoPi;
       const double amp = sqrt(pw);
       numer += amp * ifr;
       den += amp * (hh + 1.0);
-      dev += fabs((ifr / (hh + 1.0) - f0) / f0);
+      dev += fabs((ifr * inv_h - f0) * inv_f0);
     }
     double refined = numer / (den + kGuard);
     double score = 1.0 / (dev / H + kGuard);
@@ -1598,6 +1620,9 @@ __global__ __launch_bounds__(64 * kRefineWaves, 4) void k_refine_waves(
     }
     // Sample m = lane: e^{2 pi i (a + m) / len} = e^{2 pi i a / len} e^{2 pi i m / len}.
     const double wc0 = fma(ac, tpre.x, -(as * tpre.y)), ws0 = fma(as, tpre.x, ac * tpre.y);
+    // (cos, sin) delta and sin 2 delta, delta = 2 pi / len, for the derivative window.
+    const double2 e1 = wrot[65 * hw + 1], e2 = wrot[65 * hw + 2];
+    const double dc = e1.x, ds = e1.y, dA = 0.5 * e1.y, dB = 0.16 * e2.y;
     const int nstage = (len + kStage - 1) / kStage;
     bool staged = false;
 
@@ -1621,9 +1646,7 @@ __global__ __launch_bounds__(64 * kRefineWaves, 4) void k_refine_waves(
           p1[qq] = p2[qq] = q1[qq] = q2[qq] = 0.0;
         }
         const bool fill = !staged || nstage > 1;
-        double wc = wc0, ws = ws0, cur = 0.0, prev_last = 0.0;
-        auto blackman_cos = [](double cs) { return 0.42 + 0.5 * cs + 0.08 * (2.0 * cs * cs - 1.0); };
-        if (fill) cur = blackman_cos(wc);
+        double wc = wc0, ws = ws0;  // (cos, sin) phi_m of this lane's sample m
         for (int c0 = 0; c0 < len; c0 += kStage) {
           const int cend = min(c0 + kStage, len);
           if (fill) {
@@ -1631,29 +1654,33 @@ __global__ __launch_bounds__(64 * kRefineWaves, 4) void k_refine_waves(
             // `have` (a prefetched chunk), else it is loaded.
             auto chunk = [&](int s0, double xv, bool have) {
               const int m = s0 + lane;
+              if (m < len) {
+                // GetMainWindow: 0.42 + 0.5 cos phi + 0.08 cos 2 phi.
+                const double w = fma(wc, fma(0.16, wc, 0.5), 0.34);
+                // GetDiffWindow's central difference -(w[m+1] - w[m-1]) / 2 =
+                // sin phi (sin(delta) / 2 + 0.16 sin(2 delta) cos phi), delta
+                // = 2 pi / len; its ends take WORLD's one-sided values
+                // d[0] = -w[1] / 2 and d[len - 1] = w[len - 2] / 2.
+                double d = ws * fma(dB, wc, dA);
+                if (m == 0 || m == len - 1) {
+                  const double side = m == 0 ? 1.0 : -1.0;  // neighbour phi + side delta
+                  const double ce = fma(wc, dc, -(side * ws * ds));
+                  d = -0.5 * side * fma(ce, fma(0.16, ce, 0.5), 0.34);
+                }
+                if (!have) xv = y_at(basic + m - 1);
+                L.xwd[2 * (m - c0)] = xv * w;
+                L.xwd[2 * (m - c0) + 1] = xv * d;
+              }
+              // On to sample m + 64.
               const double nc = fma(wc, gc, -(ws * gs)), ns = fma(ws, gc, wc * gs);
               wc = nc;
               ws = ns;
-              const double nxt = blackman_cos(wc);  // sample m + 64
-              // Neighbours w[m + 1] and w[m - 1]; w[-1] = w[len] = 0 gives
-              // d[0] = -w[1] / 2 and d[len - 1] = w[len - 2] / 2.
-              double wn = from_next_lane(cur, lane_value(nxt, 0));
-              const double wp = from_prev_lane(cur, prev_last);
-              prev_last = lane_value(cur, 63);
-              if (m == len - 1) wn = 0.0;
-              if (m < len) {
-                const double d = -(wn - wp) / 2.0;
-                if (!have) xv = y_at(basic + m - 1);
-                L.xwd[2 * (m - c0)] = xv * cur;
-                L.xwd[2 * (m - c0) + 1] = xv * d;
-              }
-              cur = nxt;
             };
 #pragma unroll
             for (int q = 0; q < kYPre; ++q)
               if (64 * q >= c0 && 64 * q < cend) chunk(64 * q, ypre[q], true);
             for (int s0 = max(c0, 64 * kYPre); s0 < cend; s0 += 64) chunk(s0, 0.0, false);
-            if (lane == 0) wsamp += (unsigned long long)(cend - c0);
+            wsamp += (unsigned long long)(cend - c0);  // uniform: kept in SGPRs
             wave_sync();
           }
           // Lane sub takes staged samples sub + 8 r: `full` of them on every
@@ -1721,10 +1748,8 @@ __global__ __launch_bounds__(64 * kRefineWaves, 4) void k_refine_waves(
           }
         }
       }
-      if (lane == 0) {
-        bsamp += (unsigned long long)nbt * len;
-        nbins += (unsigned long long)nbt;
-      }
+      bsamp += (unsigned long long)nbt * len;  // uniform
+      nbins += (unsigned long long)nbt;
       wave_sync();
     };
 
@@ -2076,6 +2101,7 @@ __global__ __launch_bounds__(kThreads) void k_compact(Config c, const Utt* utt,
   const long long i = gf - u.f_off;
   int pos = offsets[gf];
   const unsigned long long below = (1ull << lane) - 1ull;
+  const unsigned cnt_magic = small_div_magic((unsigned)max(cnt, 1));
   for (int k0 = 0; k0 < ncand; k0 += 64) {
     const int k = k0 + lane;
     const double f = k < ncand ? pf0[gf * c.slots + k] : 0.0;
@@ -2084,7 +2110,7 @@ __global__ __launch_bounds__(kThreads) void k_compact(Config c, const Utt* utt,
       const int at = pos + __popcll(nz & below);
       of0[at] = f;
       oscore[at] = pscore[gf * c.slots + k];
-      const int g = k / cnt, jj = k % cnt;
+      const int g = small_div(k, cnt_magic), jj = k - g * cnt;
       long long src = i;
       if (g >= 1 && g <= 3) src = i - g;
       else if (g >= 4) src = i + (g - 3);
@@ -2250,14 +2276,14 @@ void launch_filter_events(hipStream_t s, const Config& c, const Utt* utt,
   const double2* ph = reinterpret_cast<const double2*>(phase);
   const int taps8 = (2 * max_half + 1 + 7) & ~7;
   const size_t lds_bytes = sizeof(double) * pad8_size(kTile + taps8 + 10);
-  dim3 grid((unsigned)total_tiles, (unsigned)count);
+  const unsigned grid = (unsigned)(total_tiles * count);
   if (lds_bytes <= 64 * 1024) {
     k_filter_events<true><<<grid, kThreads, lds_bytes, s>>>(
-        c, utt, t_off, total_tiles, band0, slot0, y, coef, coef_off, half, quirk, ph, slab,
+        c, utt, t_off, total_tiles, band0, count, slot0, y, coef, coef_off, half, quirk, ph, slab,
         counts);
   } else {
     k_filter_events<false><<<grid, kThreads, sizeof(double) * pad8_size(kTile), s>>>(
-        c, utt, t_off, total_tiles, band0, slot0, y, coef, coef_off, half, quirk, ph, slab,
+        c, utt, t_off, total_tiles, band0, count, slot0, y, coef, coef_off, half, quirk, ph, slab,
         counts);
   }
   check_launch("filter_events");
@@ -2315,7 +2341,7 @@ void launch_interp(hipStream_t s, const Config& c, const Utt* utt,
   if (per_tile > (long long)kInterpFramesPerThread * kThreads)
     throw std::runtime_error("interp: " + std::to_string(per_tile) +
                              " frames per tile exceed the kernel's threads");
-  dim3 grid((unsigned)total_tiles, (unsigned)group);
+  const unsigned grid = (unsigned)(total_tiles * group);
   k_interp<<<grid, kThreads, 0, s>>>(c, utt, t_off, total_tiles, total_frames, band0,
                                      boundaries, slab, counts, offs, totals, raw);
   check_launch("interp");
@@ -2345,8 +2371,12 @@ void launch_refine(hipStream_t s, const Config& c, const Utt* utt,
                              " bytes of LDS per workgroup; the device allows " +
                              std::to_string(c.lds_limit));
   const unsigned grid = (unsigned)((total_frames + kRefineWaves - 1) / kRefineWaves);
-  k_refine_waves<<<grid, 64 * kRefineWaves, bytes, s>>>(c, utt, f_off, total_frames, HS, W, y, official,
-                                               count, twiddle, wintab, rf0, rscore, work);
+  if (c.slots == 105 && HS == 128 && W == 9)  // 71-800 Hz at a working rate of 7350-8000 Hz
+    k_refine_waves<105, 128, 9><<<grid, 64 * kRefineWaves, bytes, s>>>(
+        c, utt, f_off, total_frames, HS, W, y, official, count, twiddle, wintab, rf0, rscore, work);
+  else
+    k_refine_waves<0, 0, 0><<<grid, 64 * kRefineWaves, bytes, s>>>(
+        c, utt, f_off, total_frames, HS, W, y, official, count, twiddle, wintab, rf0, rscore, work);
   check_launch("refine");
 }
 
