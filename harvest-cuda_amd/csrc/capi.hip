@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/harvest_hip.h"
+#include "device_check.h"
 #include "host_copy.h"
 #include "kernels.h"
 
@@ -23,44 +24,36 @@ namespace {
 
 thread_local std::string g_error;
 
-struct Failure : std::runtime_error {
-  hv_status status;
-  Failure(hv_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+// The HIP runtime as device_check.h's runtime policy.
+struct HipRuntime {
+  using Error = hipError_t;
+  static constexpr Error kSuccess = hipSuccess;
+  static constexpr Error kOutOfMemory = hipErrorOutOfMemory;
+  static constexpr Error kNoDevice = hipErrorNoDevice;
+  static constexpr Error kInsufficientDriver = hipErrorInsufficientDriver;
+  static Error device_count(int* n) { return hipGetDeviceCount(n); }
+  static Error properties(int device, std::string* arch, std::string* name) {
+    hipDeviceProp_t prop;
+    const Error e = hipGetDeviceProperties(&prop, device);
+    if (e == hipSuccess) {
+      *arch = prop.gcnArchName;
+      *name = prop.name;
+    }
+    return e;
+  }
+  static Error get_device(int* d) { return hipGetDevice(d); }
+  static Error set_device(int d) { return hipSetDevice(d); }
+  static const char* error_name(Error e) { return hipGetErrorName(e); }
+  static const char* error_string(Error e) { return hipGetErrorString(e); }
+  static void clear_error() { (void)hipGetLastError(); }
 };
+using Device = DeviceOps<HipRuntime>;
+using DeviceGuard = Device::Guard;
 
 void check(hipError_t e, const char* what) {
-  if (e == hipSuccess) return;
-  (void)hipGetLastError();
-  hv_status st = HV_EDEVICE;
-  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) st = HV_ENOMEM;
-  if (e == hipErrorNoDevice || e == hipErrorInsufficientDriver) st = HV_ENODEV;
-  std::string m = std::string("HIP: ") + hipGetErrorName(e) + ": " + hipGetErrorString(e) +
-                  " (" + what + ")";
-  if (st == HV_ENOMEM)
-    m += ". GPU memory allocation failed; try shorter audio or release unused GPU allocations.";
-  else if (st == HV_ENODEV)
-    m += ". Check the ROCm driver and HIP_VISIBLE_DEVICES before starting Python.";
-  throw Failure(st, m);
+  // hipMalloc reports exhaustion as hipErrorMemoryAllocation as well.
+  Device::check(e == hipErrorMemoryAllocation ? hipErrorOutOfMemory : e, what);
 }
-
-[[noreturn]] void invalid(const std::string& m) { throw Failure(HV_EINVAL, m); }
-
-// Saves the caller's current device and restores it on every exit path
-// (counterpart of CudaContextGuard, src/cuda_device.h:92-108).
-struct DeviceGuard {
-  int previous = -1;
-  bool switched = false;
-  explicit DeviceGuard(int device) {
-    check(hipGetDevice(&previous), "hipGetDevice");
-    if (previous != device) {
-      check(hipSetDevice(device), "hipSetDevice");
-      switched = true;
-    }
-  }
-  ~DeviceGuard() {
-    if (switched) (void)hipSetDevice(previous);
-  }
-};
 
 struct DevBuf {
   void* p = nullptr;
@@ -940,39 +933,13 @@ extern "C" {
 
 const char* hv_last_error(void) { return g_error.c_str(); }
 
-int hv_device_supported(int device) {
-  if (device < 0) return 0;
-  int n = 0;
-  if (hipGetDeviceCount(&n) != hipSuccess) {
-    (void)hipGetLastError();
-    return 0;
-  }
-  if (device >= n) return 0;
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
-    (void)hipGetLastError();
-    return 0;
-  }
-  return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
-}
+int hv_device_supported(int device) { return Device::supported(device); }
 
 hv_status hv_workspace_create(int device, hv_workspace** out) {
   return guarded([&] {
     if (!out) invalid("null output pointer");
     *out = nullptr;
-    if (device < 0) invalid("device must be a nonnegative visible GPU index");
-    int n = 0;
-    check(hipGetDeviceCount(&n), "hipGetDeviceCount");
-    if (n == 0) throw Failure(HV_ENODEV, "HIP: no GPU is visible. Check the ROCm driver and HIP_VISIBLE_DEVICES before starting Python.");
-    if (device >= n)
-      invalid("device " + std::to_string(device) + " is out of range; " + std::to_string(n) +
-              " GPU(s) are visible through HIP_VISIBLE_DEVICES");
-    hipDeviceProp_t prop;
-    check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
-    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
-      throw Failure(HV_EUNSUPPORTED, std::string("Harvest requires an MI355X-class gfx950 GPU; device ") +
-                                         std::to_string(device) + " (" + prop.name + ") is " +
-                                         prop.gcnArchName);
+    Device::validate(device);
     DeviceGuard guard(device);
     *out = new hv_workspace(device);
   });
@@ -1031,10 +998,9 @@ hv_status hv_decimate_device_noalloc(int device, const double* x_device, int64_t
     if (ratio < 1 || ratio > 12) invalid("ratio must be between 1 and 12");
     if (length < 2LL * ratio || length > 0x7fffffffLL) invalid("invalid length");
     if (y_length != (length + ratio - 1) / ratio) invalid("y_length must be ceil(length / ratio)");
-    if (device < 0) invalid("device must be a nonnegative visible GPU index");
+    Device::validate(device);
     int devices = 0;
     check(hipGetDeviceCount(&devices), "hipGetDeviceCount");
-    if (device >= devices) invalid("device " + std::to_string(device) + " is out of range");
     // The plan of one utterance, as layout() builds it for a batch of one.
     Config cfg{};
     fill_config(cfg, 1, ratio, 8000 * ratio, 8000.0, 71.0, 800.0, 0, 0, 0);
