@@ -8,6 +8,7 @@
 #include <pybind11/stl.h>
 
 #include <memory>
+#include <mutex>
 #include <optional>
 #include <stdexcept>
 #include <thread>
@@ -29,12 +30,28 @@ using Cache = WorkspaceCache<hv_workspace>;
 // second workspace a pipelined batch uses on that device.
 constexpr int kSecondSlot = 1 << 16;
 
+// Backend tuning applied to every workspace the cache creates (and to the
+// cached ones when it changes); set from the package's tuning table.
+std::mutex g_tuning_mutex;
+hv_tuning g_tuning{64, 0, 4096};
+
+hv_tuning current_tuning() {
+  std::lock_guard<std::mutex> lock(g_tuning_mutex);
+  return g_tuning;
+}
+
 Cache& cache() {
   // Owned by the module for the life of the process; cleared explicitly.
   static Cache* instance = new Cache(
       [](int key) {
         hv_workspace* ws = nullptr;
         throw_status(hv_workspace_create(key % kSecondSlot, &ws));
+        const hv_tuning t = current_tuning();
+        const hv_status st = hv_set_tuning(ws, &t);
+        if (st != HV_OK) {
+          hv_workspace_destroy(ws);
+          throw_status(st);
+        }
         return ws;
       },
       [](hv_workspace* ws) { hv_workspace_destroy(ws); });
@@ -426,6 +443,61 @@ PYBIND11_MODULE(_core, m) {
     d["fs_work"] = p.fs_work;
     return d;
   });
+  m.def("_set_tuning", [](int fft_min_half, int graphs, int slab_budget_mib) {
+    const hv_tuning t{fft_min_half, graphs, slab_budget_mib};
+    if (fft_min_half < 1 || (graphs != 0 && graphs != 1) || slab_budget_mib < 64)
+      throw std::invalid_argument("invalid tuning");
+    {
+      std::lock_guard<std::mutex> lock(g_tuning_mutex);
+      g_tuning = t;
+    }
+    // Cached workspaces take it on their next lease by being dropped now
+    // (those in use finish their call first).
+    py::gil_scoped_release release;
+    cache().clear();
+  }, py::arg("fft_min_half"), py::arg("graphs"), py::arg("slab_budget_mib"),
+     "Backend tuning of every workspace (include/harvest_hip.h hv_tuning).");
+  m.def("_get_tuning", [] {
+    const hv_tuning t = current_tuning();
+    py::dict d;
+    d["fft_min_half"] = t.fft_min_half;
+    d["graphs"] = t.graphs;
+    d["slab_budget_mib"] = t.slab_budget_mib;
+    return d;
+  });
+  m.def("_set_batch_tuning", [](int max_groups, int64_t min_group_frames, int64_t split_frames,
+                                double first_weight, double last_weight) {
+    BatchTuning t;
+    t.max_groups = max_groups;
+    t.min_group_frames = min_group_frames;
+    t.split_frames = split_frames;
+    t.first_weight = first_weight;
+    t.last_weight = last_weight;
+    set_batch_tuning(t);
+  }, py::arg("max_groups"), py::arg("min_group_frames"), py::arg("split_frames"),
+     py::arg("first_weight"), py::arg("last_weight"),
+     "How large batches are cut into length buckets (engine.h BatchTuning).");
+  m.def("_get_batch_tuning", [] {
+    const BatchTuning t = batch_tuning();
+    py::dict d;
+    d["max_groups"] = t.max_groups;
+    d["min_group_frames"] = t.min_group_frames;
+    d["split_frames"] = t.split_frames;
+    d["first_weight"] = t.first_weight;
+    d["last_weight"] = t.last_weight;
+    return d;
+  });
+  m.def("_graph_stats", [](int device) {
+    int64_t replays = 0, captures = 0, r = 0, c = 0;
+    py::gil_scoped_release release;
+    for (int key : {device, device + kSecondSlot}) {
+      Cache::Lease ws(cache(), key);
+      throw_status(hv_graph_stats(ws.get(), &r, &c));
+      replays += r;
+      captures += c;
+    }
+    return std::make_pair(replays, captures);
+  }, py::arg("device") = 0, "(graph replays, graph captures) of the device's workspaces.");
   m.def("_host_cache_allocated", [] { return HostCache::instance().allocated(); });
   m.def("_cached_workspaces", [] { return cache().size(); });
 }

@@ -227,6 +227,7 @@ struct hv_workspace {
   PinnedBuf h_x, h_meta, h_offsets, h_best, h_cand, h_work, h_ent, h_gx;
   // Filter-bank cache key (working rate, floor, ceil).
   double key_fs = -1.0, key_floor = -1.0, key_ceil = -1.0;
+  int key_fft_min = -1;
   std::vector<double> boundaries;
   std::vector<int> halves;
   int max_half = 0;
@@ -241,6 +242,17 @@ struct hv_workspace {
   // then scores (cand_ev[1]).  The _async entry points return before they
   // complete; hv_wait_candidates waits for them.
   hipEvent_t cand_ev[2] = {nullptr, nullptr};
+  hv_tuning tune{kFftMinHalf, 0, (int)(kSlabBudget >> 20)};
+  // Executable graphs of the device launch sequence, one per plan shape and
+  // buffer set (least recently used first out).
+  struct Graph {
+    std::vector<long long> key;
+    hipGraphExec_t exec = nullptr;
+    unsigned long long used = 0;
+  };
+  std::vector<Graph> graphs;
+  unsigned long long graph_clock = 0;
+  long long graph_hits = 0, graph_captures = 0;
 
   explicit hv_workspace(int dev) : device(dev) {
     hipDeviceProp_t prop;
@@ -253,6 +265,8 @@ struct hv_workspace {
   }
   ~hv_workspace() {
     if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& g : graphs)
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     for (auto& e : cand_ev)
@@ -274,6 +288,8 @@ struct Plan {
   long long total_tiles = 0, total_chunks = 0, total_pblocks = 0;
   long long max_chunks = 0;  // largest utterance's IIR chunk count
 };
+
+hipStream_t s_of(hv_workspace* ws) { return ws->stream; }
 
 // Records `launch` between two events attributed to timing slot `slot`.
 template <class F> void timed(hv_workspace* ws, int slot, F&& launch) {
@@ -326,7 +342,7 @@ void prepare_filters(hv_workspace* ws, double fs_work, double f0_floor,
     for (int i = 0; i < bands; ++i) bounds[i] = af * std::pow(2.0, (i + 1) / 40.0);
   }
   if (ws->key_fs == fs_work && ws->key_floor == f0_floor && ws->key_ceil == f0_ceil &&
-      ws->boundaries == bounds)
+      ws->key_fft_min == ws->tune.fft_min_half && ws->boundaries == bounds)
     return;
   const int B = (int)bounds.size();
   std::vector<int> halves(B), offs(B);
@@ -360,11 +376,12 @@ void prepare_filters(hv_workspace* ws, double fs_work, double f0_floor,
         "upload filters");
   check(hipMemcpyAsync(dbound, bounds.data(), B * sizeof(double), hipMemcpyHostToDevice,
                        ws->stream), "upload filters");
-  // FFT bands: the leading (longest) bands with h >= kFftMinHalf, when the
-  // longest fits a 4096-point block.
+  // FFT bands: the leading (longest) bands with h >= the tuned minimum
+  // (kFftMinHalf by default), when the longest fits a 4096-point block.
   int nf = 0;
+  const int fft_min = ws->tune.fft_min_half;
   if (halves[0] <= kFftMaxHalf)
-    while (nf < B && halves[nf] >= kFftMinHalf && halves[nf] <= halves[0]) ++nf;
+    while (nf < B && halves[nf] >= fft_min && halves[nf] <= halves[0]) ++nf;
   if (nf > 0) {
     if (!ws->tw4096_ready) {
       launch_twiddle(ws->stream, ws->tw4096.ensure<double>(2 * 4096), 4096);
@@ -379,6 +396,7 @@ void prepare_filters(hv_workspace* ws, double fs_work, double f0_floor,
   ws->key_fs = fs_work;
   ws->key_floor = f0_floor;
   ws->key_ceil = f0_ceil;
+  ws->key_fft_min = fft_min;
   ws->boundaries = bounds;
   ws->halves = halves;
   ws->max_half = max_half;
@@ -539,105 +557,237 @@ void upload_plan(hv_workspace* ws, const Plan& p) {
   check(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ws->stream), "upload plan");
 }
 
-// Everything after the decimated, mean-removed y is on the device.
-// Enqueues the analysis after preprocessing, through the download of offsets,
-// best and the work counters; the host does not wait (analyze_collect does).
-void analyze_submit(hv_workspace* ws, const Plan& p) {
+// Device buffers of one analysis, all sized before anything is enqueued (so
+// that a captured launch sequence never allocates).
+struct Bufs {
+  const double* dx = nullptr;  // input: the caller's device batch or ws->x
+  double *y, *pass, *states, *carries, *carry_tmp, *partial, *mean;
+  Quirk* quirk;
+  double *partial3, *sums3;
+  int group;  // bands per slab group
+  double* slab;
+  int *counts, *offs, *totals;
+  double *raw, *tspec;
+  int* count;
+  double *official, *rf0, *rsc, *pf0, *psc, *dbest, *cinit;
+  unsigned long long* work;
+  int *row_nnz, *doffsets, *scan_tmp;
+  long long nb;
+  int32_t* hoff;
+  double* hbest;
+  unsigned long long* hwork;
+};
+
+// HV_STAT_SLOTS work counters, then a flag the refinement sets if a
+// candidate's window exceeds the bound its LDS was sized for; each spread
+// over kWorkSpread addresses.
+constexpr int kWorkWords = (HV_STAT_SLOTS + 1) * kWorkSpread;
+
+Bufs size_buffers(hv_workspace* ws, const Plan& p, const double* dx) {
+  const Config& c = p.cfg;
+  const int U = c.count, B = c.bands;
+  Bufs b{};
+  b.dx = dx;
+  b.y = ws->y.ensure<double>(p.total_y);
+  b.pass = ws->pass.ensure<double>(p.total_m);
+  b.states = ws->states.ensure<double>(3 * p.total_chunks);
+  b.carries = ws->carries.ensure<double>(3 * p.total_chunks);
+  b.carry_tmp = ws->carry_tmp.ensure<double>(carry_scratch_doubles(U, p.max_chunks));
+  b.partial = ws->partial.ensure<double>(p.total_pblocks);
+  b.mean = ws->mean.ensure<double>(U);
+  b.quirk = ws->quirk.ensure<Quirk>((size_t)U * B);
+  b.partial3 = ws->partial3.ensure<double>(3 * p.total_pblocks);
+  b.sums3 = ws->sums3.ensure<double>(3 * U);
+  // Band groups bounded by the slab budget.
+  const size_t budget = (size_t)ws->tune.slab_budget_mib << 20;
+  const size_t per_band = (size_t)4 * p.total_tiles * kSlabCap * sizeof(double);
+  b.group = (int)std::max<size_t>(1, std::min<size_t>(B, budget / std::max<size_t>(per_band, 1)));
+  b.slab = ws->slab.ensure<double>((size_t)b.group * 4 * p.total_tiles * kSlabCap);
+  b.counts = ws->counts.ensure<int>((size_t)b.group * 4 * p.total_tiles);
+  b.offs = ws->offs.ensure<int>((size_t)b.group * 4 * p.total_tiles);
+  b.totals = ws->totals.ensure<int>((size_t)b.group * 4 * U);
+  b.raw = ws->raw.ensure<double>((size_t)B * p.total_frames);
+  b.tspec = ws->fft_bands > 0 ? ws->tile_spec.ensure<double>((size_t)p.total_tiles * kSpecDoubles)
+                              : nullptr;
+  b.count = ws->count.ensure<int>(U);
+  b.official = ws->official.ensure<double>((size_t)p.total_frames * c.capacity);
+  const size_t slots = (size_t)p.total_frames * c.slots;
+  b.rf0 = ws->rf0.ensure<double>(slots);
+  b.rsc = ws->rsc.ensure<double>(slots);
+  b.work = ws->work.ensure<unsigned long long>(kWorkWords);
+  b.pf0 = ws->pf0.ensure<double>(slots);
+  b.psc = ws->psc.ensure<double>(slots);
+  b.dbest = ws->best.ensure<double>(p.total_frames);
+  b.row_nnz = ws->row_nnz.ensure<int>(p.total_frames);
+  b.doffsets = ws->offsets.ensure<int>(p.total_frames + 1);
+  b.nb = scan_block_count(p.total_frames);
+  b.scan_tmp = ws->scan_tmp.ensure<int>(b.nb + 1);
+  b.cinit = ws->cinit.ensure<double>(slots);
+  b.hoff = ws->h_offsets.ensure<int32_t>(p.total_frames + 1);
+  b.hbest = ws->h_best.ensure<double>(p.total_frames);
+  b.hwork = ws->h_work.ensure<unsigned long long>(kWorkWords);
+  return b;
+}
+
+// Decimation (or the copy at ratio 1) and mean removal: y on the device.
+void enqueue_preprocess(hv_workspace* ws, const Plan& p, const Bufs& b, bool events) {
+  hipStream_t s = ws->stream;
+  timed(ws, 3, [&] {
+    if (p.cfg.ratio > 1)
+      launch_decimate(s, p.cfg, ws->d_utt, ws->d_c_off, p.total_chunks, p.max_chunks, b.dx,
+                      b.pass, b.states, b.carries, b.carry_tmp, b.y);
+    else
+      launch_copy_input(s, p.cfg, ws->d_utt, ws->d_y_off, p.total_y, b.dx, b.y);
+  });
+  launch_remove_mean(s, p.cfg, ws->d_utt, ws->d_p_off, p.total_pblocks, ws->d_y_off, p.total_y,
+                     b.y, b.partial, b.mean);
+  if (events) check(hipEventRecord(ws->ev[1], s), "hipEventRecord");
+}
+
+// Everything after the decimated, mean-removed y, through the download of
+// offsets, best and the work counters; the host does not wait.
+void enqueue_analysis(hv_workspace* ws, const Plan& p, const Bufs& b, bool events) {
   const Config& c = p.cfg;
   const int U = c.count, B = c.bands;
   hipStream_t s = ws->stream;
   const Utt* dutt = ws->d_utt;
-  double* y = ws->y.get<double>();
   const double* coef = ws->coef.get<double>();
   const int* coef_off = ws->coef_off.get<int>();
   const int* half = ws->half.get<int>();
-
-  Quirk* quirk = ws->quirk.ensure<Quirk>((size_t)U * B);
-  launch_quirk(s, c, dutt, ws->d_p_off, p.total_pblocks, y, coef, coef_off, half,
-               ws->partial3.ensure<double>(3 * p.total_pblocks), ws->sums3.ensure<double>(3 * U),
-               quirk);
-
-  // Band groups bounded by the slab budget.
-  const size_t per_band = (size_t)4 * p.total_tiles * kSlabCap * sizeof(double);
-  int group = (int)std::max<size_t>(1, std::min<size_t>(B, kSlabBudget / std::max<size_t>(per_band, 1)));
-  double* slab = ws->slab.ensure<double>((size_t)group * 4 * p.total_tiles * kSlabCap);
-  int* counts = ws->counts.ensure<int>((size_t)group * 4 * p.total_tiles);
-  int* offs = ws->offs.ensure<int>((size_t)group * 4 * p.total_tiles);
-  int* totals = ws->totals.ensure<int>((size_t)group * 4 * U);
-  double* raw = ws->raw.ensure<double>((size_t)B * p.total_frames);
+  launch_quirk(s, c, dutt, ws->d_p_off, p.total_pblocks, b.y, coef, coef_off, half, b.partial3,
+               b.sums3, b.quirk);
   // The long bands [0, nfft) by FFT (one spectrum per tile, shared by the
   // bands), the others by the direct FIR.
   const int nfft = ws->fft_bands, lead = ws->fft_D;
-  double* tspec = nullptr;
   if (nfft > 0) {
-    tspec = ws->tile_spec.ensure<double>((size_t)p.total_tiles * kSpecDoubles);
     timed(ws, 4, [&] {
-      launch_tile_spectra(s, c, dutt, ws->d_t_off, p.total_tiles, lead, y,
-                          ws->tw4096.get<double>(), tspec);
+      launch_tile_spectra(s, c, dutt, ws->d_t_off, p.total_tiles, lead, b.y,
+                          ws->tw4096.get<double>(), b.tspec);
     });
   }
-  for (int b0 = 0; b0 < B; b0 += group) {
-    const int g = std::min(group, B - b0);
+  for (int b0 = 0; b0 < B; b0 += b.group) {
+    const int g = std::min(b.group, B - b0);
     const int nf = std::max(0, std::min(g, nfft - b0));
     int mh = 0;
-    for (int b = b0 + nf; b < b0 + g; ++b) mh = std::max(mh, ws->halves[b]);
+    for (int k = b0 + nf; k < b0 + g; ++k) mh = std::max(mh, ws->halves[k]);
     timed(ws, 4, [&] {
-      launch_filter_fft(s, c, dutt, ws->d_t_off, p.total_tiles, b0, nf, 0, lead, tspec,
-                        ws->band_spec.get<double>(), half, quirk, ws->tw4096.get<double>(),
-                        ws->phase.get<double>(), slab, counts);
-      launch_filter_events(s, c, dutt, ws->d_t_off, p.total_tiles, b0 + nf, g - nf, nf, mh, y,
-                           coef, coef_off, half, quirk, ws->phase.get<double>(), slab, counts);
+      launch_filter_fft(s, c, dutt, ws->d_t_off, p.total_tiles, b0, nf, 0, lead, b.tspec,
+                        ws->band_spec.get<double>(), half, b.quirk, ws->tw4096.get<double>(),
+                        ws->phase.get<double>(), b.slab, b.counts);
+      launch_filter_events(s, c, dutt, ws->d_t_off, p.total_tiles, b0 + nf, g - nf, nf, mh, b.y,
+                           coef, coef_off, half, b.quirk, ws->phase.get<double>(), b.slab,
+                           b.counts);
     });
     timed(ws, 5, [&] {
-      launch_tile_scan(s, c, dutt, p.total_tiles, g, counts, offs, totals);
+      launch_tile_scan(s, c, dutt, p.total_tiles, g, b.counts, b.offs, b.totals);
       launch_interp(s, c, dutt, ws->d_t_off, p.total_tiles, p.total_frames, b0, g,
-                    ws->bounds.get<double>(), slab, counts, offs, totals, raw);
+                    ws->bounds.get<double>(), b.slab, b.counts, b.offs, b.totals, b.raw);
     });
   }
-  int* count = ws->count.ensure<int>(U);
-  launch_fill_int(s, count, U, 0);
-  double* official = ws->official.ensure<double>((size_t)p.total_frames * c.capacity);
-  launch_official(s, c, ws->d_f_off, p.total_frames, raw, official, count);
-  check(hipEventRecord(ws->ev[2], s), "hipEventRecord");
-
-  const size_t slots = (size_t)p.total_frames * c.slots;
-  double* rf0 = ws->rf0.ensure<double>(slots);
-  double* rsc = ws->rsc.ensure<double>(slots);
-  // HV_STAT_SLOTS work counters, then a flag the refinement sets if a
-  // candidate's window exceeds the bound its LDS was sized for; each spread
-  // over kWorkSpread addresses.
-  constexpr int kWorkWords = (HV_STAT_SLOTS + 1) * kWorkSpread;
-  unsigned long long* work = ws->work.ensure<unsigned long long>(kWorkWords);
-  launch_fill_int(s, reinterpret_cast<int*>(work), 2 * kWorkWords, 0);
+  launch_fill_int(s, b.count, U, 0);
+  launch_official(s, c, ws->d_f_off, p.total_frames, b.raw, b.official, b.count);
+  if (events) check(hipEventRecord(ws->ev[2], s), "hipEventRecord");
+  launch_fill_int(s, reinterpret_cast<int*>(b.work), 2 * kWorkWords, 0);
   timed(ws, 6, [&] {
-    launch_refine(s, c, dutt, ws->d_f_off, p.total_frames, y, official, count,
-                  ws->twiddle.get<double>(), ws->wintab.get<double>(), rf0, rsc, work);
+    launch_refine(s, c, dutt, ws->d_f_off, p.total_frames, b.y, b.official, b.count,
+                  ws->twiddle.get<double>(), ws->wintab.get<double>(), b.rf0, b.rsc, b.work);
   });
-  double* pf0 = ws->pf0.ensure<double>(slots);
-  double* psc = ws->psc.ensure<double>(slots);
-  double* dbest = ws->best.ensure<double>(p.total_frames);
-  int* row_nnz = ws->row_nnz.ensure<int>(p.total_frames);
-  int* doffsets = ws->offsets.ensure<int>(p.total_frames + 1);
-  const long long nb = scan_block_count(p.total_frames);
-  int* scan_tmp = ws->scan_tmp.ensure<int>(nb + 1);
   timed(ws, 7, [&] {
-    launch_prune_best(s, c, dutt, ws->d_f_off, p.total_frames, count, rf0, rsc,
-                      pf0, psc, dbest, row_nnz);
-    launch_exclusive_scan(s, row_nnz, doffsets, p.total_frames, scan_tmp, nb + 1);
+    launch_prune_best(s, c, dutt, ws->d_f_off, p.total_frames, b.count, b.rf0, b.rsc, b.pf0,
+                      b.psc, b.dbest, b.row_nnz);
+    launch_exclusive_scan(s, b.row_nnz, b.doffsets, p.total_frames, b.scan_tmp, b.nb + 1);
     // Compacted candidates reuse the refinement buffers (no longer needed).
-    launch_compact(s, c, dutt, ws->d_f_off, p.total_frames, count, official,
-                   pf0, psc, doffsets, rf0, rsc, ws->cinit.ensure<double>(slots));
+    launch_compact(s, c, dutt, ws->d_f_off, p.total_frames, b.count, b.official, b.pf0, b.psc,
+                   b.doffsets, b.rf0, b.rsc, b.cinit);
   });
-  int32_t* hoff = ws->h_offsets.ensure<int32_t>(p.total_frames + 1);
-  double* hbest = ws->h_best.ensure<double>(p.total_frames);
-  check(hipMemcpyAsync(hoff, doffsets, (p.total_frames + 1) * sizeof(int32_t),
+  check(hipMemcpyAsync(b.hoff, b.doffsets, (p.total_frames + 1) * sizeof(int32_t),
                        hipMemcpyDeviceToHost, s), "download offsets");
-  check(hipMemcpyAsync(hbest, dbest, p.total_frames * sizeof(double), hipMemcpyDeviceToHost, s),
-        "download best");
-  unsigned long long* hwork = ws->h_work.ensure<unsigned long long>(kWorkWords);
-  check(hipMemcpyAsync(hwork, work, kWorkWords * sizeof(unsigned long long),
+  check(hipMemcpyAsync(b.hbest, b.dbest, p.total_frames * sizeof(double), hipMemcpyDeviceToHost,
+                       s), "download best");
+  check(hipMemcpyAsync(b.hwork, b.work, kWorkWords * sizeof(unsigned long long),
                        hipMemcpyDeviceToHost, s), "download counters");
-  check(hipEventRecord(ws->ev[3], s), "hipEventRecord");
+}
+
+// What a captured launch sequence depends on: every size and pointer its
+// kernels and copies take, and the settings folded into the launch config.
+std::vector<long long> graph_key(hv_workspace* ws, const Plan& p, const Bufs& b) {
+  const Config& c = p.cfg;
+  std::vector<long long> k = {c.count, c.ratio, c.fs, c.bands, c.capacity, c.twiddle_n,
+                              c.hmax, (long long)c.phase_n, ws->fft_bands, ws->fft_D, b.group,
+                              p.total_x, p.total_m, p.total_y, p.total_frames, p.total_tiles,
+                              p.total_chunks, p.total_pblocks, p.max_chunks};
+  long long bits;
+  for (double v : {c.fs_work, c.f0_floor, c.f0_ceil}) {
+    std::memcpy(&bits, &v, sizeof(bits));
+    k.push_back(bits);
+  }
+  const void* ptrs[] = {b.dx, ws->d_utt, ws->coef.p, ws->coef_off.p, ws->half.p, ws->bounds.p,
+                        ws->twiddle.p, ws->wintab.p, ws->tw4096.p, ws->band_spec.p, ws->phase.p,
+                        b.y, b.pass, b.states, b.carries, b.carry_tmp, b.partial, b.mean, b.quirk,
+                        b.partial3, b.sums3, b.slab, b.counts, b.offs, b.totals, b.raw, b.tspec,
+                        b.count, b.official, b.rf0, b.rsc, b.pf0, b.psc, b.dbest, b.cinit, b.work,
+                        b.row_nnz, b.doffsets, b.scan_tmp, b.hoff, b.hbest, b.hwork};
+  for (const void* q : ptrs) k.push_back((long long)(uintptr_t)q);
+  return k;
+}
+
+constexpr size_t kMaxGraphs = 8;
+
+// The device launch sequence as one graph launch: replayed when a graph of
+// the same key exists, else captured from the stream (nothing in it
+// allocates or clears memory with a memset: counters are cleared by fill
+// kernels), instantiated and cached.
+void launch_as_graph(hv_workspace* ws, const Plan& p, const Bufs& b) {
+  hipStream_t s = ws->stream;
+  std::vector<long long> key = graph_key(ws, p, b);
+  ++ws->graph_clock;
+  for (auto& g : ws->graphs)
+    if (g.key == key) {
+      g.used = ws->graph_clock;
+      ++ws->graph_hits;
+      check(hipGraphLaunch(g.exec, s), "hipGraphLaunch");
+      return;
+    }
+  hipGraph_t graph = nullptr;
+  check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  try {
+    enqueue_preprocess(ws, p, b, false);
+    enqueue_analysis(ws, p, b, false);
+  } catch (...) {
+    (void)hipStreamEndCapture(s, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  check(hipStreamEndCapture(s, &graph), "hipStreamEndCapture");
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  check(e, "hipGraphInstantiate");
+  if (ws->graphs.size() >= kMaxGraphs) {
+    auto lru = std::min_element(ws->graphs.begin(), ws->graphs.end(),
+                                [](const auto& x, const auto& y) { return x.used < y.used; });
+    (void)hipGraphExecDestroy(lru->exec);
+    ws->graphs.erase(lru);
+  }
+  ws->graphs.push_back({std::move(key), exec, ws->graph_clock});
+  ++ws->graph_captures;
+  check(hipGraphLaunch(exec, s), "hipGraphLaunch");
+}
+
+// Enqueues the device work of a staged batch (see stage_batch) and marks the
+// workspace submitted: as a graph when graphs are on and no timings were
+// asked for, else launch by launch with span events.
+// With dx == nullptr, y is already in place (hv_analyze): no preprocessing,
+// and the sequence runs launch by launch.
+void analyze_submit(hv_workspace* ws, const Plan& p, const double* dx) {
+  const Bufs b = size_buffers(ws, p, dx);
+  if (dx && ws->tune.graphs && !ws->timing) {
+    launch_as_graph(ws, p, b);
+  } else {
+    if (dx) enqueue_preprocess(ws, p, b, true);
+    enqueue_analysis(ws, p, b, true);
+  }
+  check(hipEventRecord(ws->ev[3], s_of(ws)), "hipEventRecord");
   ws->pending_frames = p.total_frames;
   ws->submitted = true;
 }
@@ -694,9 +844,9 @@ void analyze_collect(hv_workspace* ws, int32_t* offsets, double* best, int64_t* 
   ws->timing = false;
 }
 
-void analyze_device(hv_workspace* ws, const Plan& p, int32_t* offsets, double* best,
-                    int64_t* nnz, double* timings, bool defer = false) {
-  analyze_submit(ws, p);
+void analyze_device(hv_workspace* ws, const Plan& p, const double* dx, int32_t* offsets,
+                    double* best, int64_t* nnz, double* timings, bool defer = false) {
+  analyze_submit(ws, p, dx);
   analyze_collect(ws, offsets, best, nnz, timings, defer);
 }
 
@@ -742,9 +892,9 @@ void check_common(int fs, double f0_floor, double f0_ceil) {
 // already points to the concatenated batch in device memory.
 // Plan, upload and preprocessing shared by the analysis and the frozen-decision
 // entry point: on return the decimated, mean-removed y is on the device.
-Plan preprocess_batch(hv_workspace* ws, int count, const double* const* xs,
-                      const int64_t* lengths, int fs, double f0_floor, double f0_ceil,
-                      double* timings, bool device_x) {
+Plan stage_batch(hv_workspace* ws, int count, const double* const* xs, const int64_t* lengths,
+                 int fs, double f0_floor, double f0_ceil, double* timings, bool device_x,
+                 const double** staged) {
   if (count < 1) invalid("count must be positive");
   check_common(fs, f0_floor, f0_ceil);
   const int ratio = std::max(std::min(host_round(fs / 8000.0), 12), 1);
@@ -801,24 +951,18 @@ Plan preprocess_batch(hv_workspace* ws, int count, const double* const* xs,
     }
     dx = d;
   }
-  double* y = ws->y.ensure<double>(p.total_y);
-  double* pass = ws->pass.ensure<double>(p.total_m);
-  double* states = ws->states.ensure<double>(3 * p.total_chunks);
-  double* carries = ws->carries.ensure<double>(3 * p.total_chunks);
-  timed(ws, 3, [&] {
-    if (ratio > 1)
-      launch_decimate(s, p.cfg, ws->d_utt, ws->d_c_off, p.total_chunks, p.max_chunks, dx, pass,
-                      states, carries,
-                      ws->carry_tmp.ensure<double>(carry_scratch_doubles(count, p.max_chunks)),
-                      y);
-    else
-      launch_copy_input(s, p.cfg, ws->d_utt, ws->d_y_off, p.total_y, dx,
-                        y);
-  });
-  launch_remove_mean(s, p.cfg, ws->d_utt, ws->d_p_off, p.total_pblocks,
-                     ws->d_y_off, p.total_y, y,
-                     ws->partial.ensure<double>(p.total_pblocks), ws->mean.ensure<double>(count));
-  check(hipEventRecord(ws->ev[1], s), "hipEventRecord");
+  *staged = dx;
+  return p;
+}
+
+// stage_batch, then decimation and mean removal launch by launch: on return
+// the decimated, mean-removed y is on the device (frozen-decision entry).
+Plan preprocess_batch(hv_workspace* ws, int count, const double* const* xs,
+                      const int64_t* lengths, int fs, double f0_floor, double f0_ceil,
+                      double* timings, bool device_x) {
+  const double* dx = nullptr;
+  Plan p = stage_batch(ws, count, xs, lengths, fs, f0_floor, f0_ceil, timings, device_x, &dx);
+  enqueue_preprocess(ws, p, size_buffers(ws, p, dx), true);
   return p;
 }
 
@@ -828,9 +972,10 @@ void run_batch(hv_workspace* ws, int count, const double* const* xs, const int64
   if (!ws) invalid("null workspace");
   if (ws->submitted) invalid("the workspace has a submitted analysis to collect first");
   DeviceGuard guard(ws->device);
-  const Plan p = preprocess_batch(ws, count, xs, lengths, fs, f0_floor, f0_ceil, timings,
-                                  device_x);
-  analyze_device(ws, p, offsets, best, nnz, timings, defer);
+  const double* dx = nullptr;
+  const Plan p = stage_batch(ws, count, xs, lengths, fs, f0_floor, f0_ceil, timings, device_x,
+                             &dx);
+  analyze_device(ws, p, dx, offsets, best, nnz, timings, defer);
 }
 
 void submit_batch(hv_workspace* ws, int count, const double* const* xs, const int64_t* lengths,
@@ -839,9 +984,10 @@ void submit_batch(hv_workspace* ws, int count, const double* const* xs, const in
   if (ws->submitted) invalid("the workspace has a submitted analysis to collect first");
   DeviceGuard guard(ws->device);
   double on = 0.0;
-  const Plan p = preprocess_batch(ws, count, xs, lengths, fs, f0_floor, f0_ceil,
-                                  timings ? &on : nullptr, device_x);
-  analyze_submit(ws, p);
+  const double* dx = nullptr;
+  const Plan p = stage_batch(ws, count, xs, lengths, fs, f0_floor, f0_ceil,
+                             timings ? &on : nullptr, device_x, &dx);
+  analyze_submit(ws, p, dx);
 }
 
 // Frozen-decision refinement of entries (value and/or adjoint), then the
@@ -1096,7 +1242,7 @@ hv_status hv_analyze(hv_workspace* ws, const double* y, int64_t y_length, int ff
     check(hipMemcpyAsync(ws->y.ensure<double>(y_length), hy, y_length * sizeof(double),
                          hipMemcpyHostToDevice, s), "upload y");
     check(hipEventRecord(ws->ev[1], s), "hipEventRecord");
-    analyze_device(ws, p, offsets, best, nnz, timings_ms);
+    analyze_device(ws, p, nullptr, offsets, best, nnz, timings_ms);
   });
 }
 
@@ -1209,6 +1355,32 @@ hv_status hv_refine_entries(hv_workspace* ws, int count, const double* const* xs
       if (!xs[u]) invalid("null utterance pointer");
     refine_entries(ws, count, xs, lengths, fs, f0_floor, f0_ceil, n_entries, frame, f0_initial,
                    weight, refined, grad_x);
+  });
+}
+
+hv_status hv_get_tuning(hv_workspace* ws, hv_tuning* out) {
+  return guarded([&] {
+    if (!ws || !out) invalid("null pointer");
+    *out = ws->tune;
+  });
+}
+
+hv_status hv_set_tuning(hv_workspace* ws, const hv_tuning* in) {
+  return guarded([&] {
+    if (!ws || !in) invalid("null pointer");
+    if (ws->submitted) invalid("the workspace has a submitted analysis to collect first");
+    if (in->fft_min_half < 1) invalid("fft_min_half must be positive");
+    if (in->graphs != 0 && in->graphs != 1) invalid("graphs must be 0 or 1");
+    if (in->slab_budget_mib < 64) invalid("slab_budget_mib must be at least 64");
+    ws->tune = *in;  // a new band split takes effect at the next call (filter cache key)
+  });
+}
+
+hv_status hv_graph_stats(hv_workspace* ws, int64_t* replays, int64_t* captures) {
+  return guarded([&] {
+    if (!ws || !replays || !captures) invalid("null pointer");
+    *replays = ws->graph_hits;
+    *captures = ws->graph_captures;
   });
 }
 

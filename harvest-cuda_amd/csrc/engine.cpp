@@ -882,11 +882,32 @@ std::vector<Result> batch_common(hv_workspace* ws, Call&& call,
 }
 }  // namespace
 
+namespace {
+std::mutex g_tuning_mutex;
+BatchTuning g_tuning;
+}  // namespace
+
+void set_batch_tuning(const BatchTuning& t) {
+  if (t.max_groups < 1 || t.min_group_frames < 1 || t.split_frames < 0 || !(t.first_weight > 0) ||
+      !(t.last_weight > 0) || !std::isfinite(t.first_weight) || !std::isfinite(t.last_weight))
+    throw std::invalid_argument("invalid batch tuning");
+  std::lock_guard<std::mutex> lock(g_tuning_mutex);
+  g_tuning = t;
+}
+
+BatchTuning batch_tuning() {
+  std::lock_guard<std::mutex> lock(g_tuning_mutex);
+  return g_tuning;
+}
+
 int pipeline_groups(const std::vector<int64_t>& frames) {
+  const BatchTuning t = batch_tuning();
   int64_t total = 0;
   for (int64_t f : frames) total += f;
-  if (frames.size() < 4 || total < 200000) return 1;
-  return (int)std::min<int64_t>({6, (int64_t)frames.size(), total / 50000});
+  if (frames.size() < 4 || total < t.split_frames) return 1;
+  return (int)std::max<int64_t>(
+      1, std::min<int64_t>({(int64_t)t.max_groups, (int64_t)frames.size(),
+                            total / t.min_group_frames}));
 }
 
 namespace {
@@ -907,12 +928,24 @@ Buckets make_buckets(const std::vector<int64_t>& frames, int groups, bool by_len
                      [&](int x, int y) { return frames[x] > frames[y]; });
   int64_t total = 0;
   for (int64_t f : frames) total += f;
+  // Cumulative frame shares of the group boundaries.
+  const BatchTuning t = batch_tuning();
+  std::vector<double> weight(groups, 1.0);
+  if (groups > 1) {
+    weight[0] = t.first_weight;
+    weight[groups - 1] = t.last_weight;
+  }
+  double sum = 0.0;
+  for (double w : weight) sum += w;
+  std::vector<double> edge(groups, 0.0);  // edge[g]: share before group g
+  for (int g = 1; g < groups; ++g) edge[g] = edge[g - 1] + weight[g - 1] / sum;
   b.bounds = {0};
   int64_t acc = 0;
   for (int k = 0; k < count; ++k) {
     acc += frames[b.order[k]];
     const int g = (int)b.bounds.size();
-    if (g < groups && acc * groups >= total * g && k + 1 < count) b.bounds.push_back(k + 1);
+    if (g < groups && (double)acc >= edge[g] * (double)total && k + 1 < count)
+      b.bounds.push_back(k + 1);
   }
   b.bounds.push_back(count);
   return b;

@@ -69,8 +69,25 @@ std::vector<Result> analyze_batch_device(hv_workspace* ws, const double* x_devic
                                          int host_threads, Timings* timings = nullptr,
                                          hv_workspace* second = nullptr);
 
-// Number of pipeline groups for a batch: 1 (no split) below 4 utterances or
-// 200 000 frames in total, else up to 6 groups of at least 50 000 frames.
+// How a large batch is cut into length buckets (pipeline groups): a batch
+// of at least 4 utterances and `split_frames` 1 ms frames is split into up
+// to `max_groups` groups of at least `min_group_frames` frames; groups take
+// shares of the frames in proportion to their weights (the first and the
+// last group `first_weight` and `last_weight`, the others 1), so that a
+// small first group starts the device early and a small last group shortens
+// the host contour left after the device finishes.  Process-wide; the
+// package applies its autotuned table (harvest_cuda/tuning.json).
+struct BatchTuning {
+  int max_groups = 6;
+  int64_t min_group_frames = 50000;
+  int64_t split_frames = 200000;
+  double first_weight = 1.0;
+  double last_weight = 1.0;
+};
+void set_batch_tuning(const BatchTuning& t);
+BatchTuning batch_tuning();
+
+// Number of pipeline groups for a batch under the current BatchTuning.
 int pipeline_groups(const std::vector<int64_t>& frames);
 
 // Contour stages, exposed for the bit-exactness tests against WORLD.

@@ -16,6 +16,9 @@ except ImportError:  # pragma: no cover - torch is optional for the NumPy API
 
 from ._core import clear_cache, cuda_available, harvest, harvest_batch  # noqa: E402
 from .stream import harvest_stream  # noqa: E402
+from . import tuning as _tuning  # noqa: E402
+
+_tuning.apply()  # the autotuned table (harvest_cuda/tuning.json), if present
 
 __all__ = ["harvest", "harvest_batch", "harvest_stream", "cuda_available", "clear_cache"]
 __version__ = "0.1.0"

@@ -1,0 +1,137 @@
+"""Autotuner of the batched form (run on an MI355X):
+
+    python -m harvest_cuda.autotune [--out harvest_cuda/tuning.json]
+
+It measures, on a calibration workload drawn like the batched benchmark
+(256 utterances at 24 kHz, lengths log-uniform over 0.5-30 s) but from
+another seed (2003), so that the benchmark's own utterances are not tuned
+on:
+  1. the FFT/direct filterbank split (fft_min_half): the filterbank kernels'
+     time on one 60 s utterance at 24 kHz for each candidate split, keeping
+     only splits whose F0 equals the default's to 1e-9 with identical
+     voicing;
+  2. the length buckets (max_groups, first_weight, last_weight): the wall
+     time of one batched call, NumPy in and out, median of 3, over a grid;
+  3. HIP graph capture on or off at the chosen buckets.
+The winners and every measurement go to the table, which the package
+applies at import (harvest_cuda/tuning.py).
+"""
+from __future__ import annotations
+
+import argparse
+import itertools
+import json
+import statistics
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+from . import _core, tuning
+from .signals import _voiced_unvoiced, config3
+
+FFT_SPLITS = (32, 48, 64, 80, 96, 128, 160, 1001)
+GROUPS = (2, 3, 4, 6, 8, 12)
+WEIGHTS = (0.4, 0.7, 1.0)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def tune_fft_split(repeats: int = 10) -> tuple[int, list]:
+    import torch
+    rng = np.random.default_rng(np.random.SeedSequence(2003).spawn(1)[0])
+    fs = 24000
+    x = np.ascontiguousarray(_voiced_unvoiced(rng, 60 * fs, fs, 25.0))
+    dx = torch.from_numpy(x).to("cuda:0")
+    torch.cuda.synchronize()
+    rows, ref = [], None
+    for split in FFT_SPLITS:
+        tuning.set_backend(fft_min_half=split)
+        for _ in range(3):
+            _core.harvest_device(dx.data_ptr(), [len(x)], fs, timings=True)
+        ms = []
+        for _ in range(repeats):
+            out, tm = _core.harvest_device(dx.data_ptr(), [len(x)], fs, timings=True)
+            ms.append(tm["kernel_filterbank"])
+        f0 = out[0][0]
+        if ref is None and split == 64:
+            ref = f0
+        rows.append({"fft_min_half": split, "filterbank_ms": statistics.median(ms), "f0": f0})
+        log(f"fft_min_half {split}: filterbank {rows[-1]['filterbank_ms']:.4f} ms")
+    ok = []
+    for r in rows:
+        f0 = r.pop("f0")
+        v = ref > 0
+        same_vuv = bool(np.array_equal(v, f0 > 0))
+        rel = float(np.max(np.abs(f0[v] - ref[v]) / ref[v])) if v.any() else 0.0
+        r.update(same_vuv=same_vuv, max_rel_vs_default=rel)
+        if same_vuv and rel <= 1e-9:
+            ok.append(r)
+    best = min(ok, key=lambda r: r["filterbank_ms"])["fft_min_half"]
+    return best, rows
+
+
+def batch_wall(xs, fs, repeats: int) -> float:
+    _core.harvest_batch(xs, fs)
+    ms = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        _core.harvest_batch(xs, fs)
+        ms.append((time.perf_counter() - t0) * 1000.0)
+    return statistics.median(ms)
+
+
+def tune_buckets(xs, fs, repeats: int = 3) -> tuple[dict, list]:
+    rows = []
+    for g, first, last in itertools.product(GROUPS, WEIGHTS, WEIGHTS):
+        tuning.set_batch(max_groups=g, min_group_frames=20000, split_frames=200000,
+                         first_weight=first, last_weight=last)
+        ms = batch_wall(xs, fs, repeats)
+        rows.append({"max_groups": g, "first_weight": first, "last_weight": last, "ms": ms})
+        log(f"groups {g} first {first} last {last}: {ms:.1f} ms")
+    best = min(rows, key=lambda r: r["ms"])
+    chosen = {"max_groups": best["max_groups"], "min_group_frames": 20000, "split_frames": 200000,
+              "first_weight": best["first_weight"], "last_weight": best["last_weight"]}
+    return chosen, rows
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--out", default=str(tuning.TABLE))
+    ap.add_argument("--count", type=int, default=256)
+    args = ap.parse_args()
+    if not _core.cuda_available(0):
+        raise SystemExit("no supported gfx950 GPU visible")
+    t0 = time.perf_counter()
+    tuning.set_backend(fft_min_half=64, graphs=0, slab_budget_mib=4096)
+    split, fft_rows = tune_fft_split()
+    tuning.set_backend(fft_min_half=split)
+    xs, fs = config3(seed=2003, count=args.count)
+    audio = sum(len(x) for x in xs) / fs
+    log(f"calibration batch: {len(xs)} utterances, {audio:.0f} audio-s")
+    batch, bucket_rows = tune_buckets(xs, fs)
+    tuning.set_batch(**batch)
+    graph_rows = []
+    for graphs in (0, 1):
+        tuning.set_backend(graphs=graphs)
+        ms = batch_wall(xs, fs, 5)
+        graph_rows.append({"graphs": graphs, "ms": ms})
+        log(f"graphs {graphs}: {ms:.1f} ms")
+    graphs = min(graph_rows, key=lambda r: r["ms"])["graphs"]
+    table = {
+        "backend": {"fft_min_half": split, "graphs": graphs, "slab_budget_mib": 4096},
+        "batch": batch,
+        "calibration": {"utterances": len(xs), "audio_seconds": audio, "fs": fs, "seed": 2003,
+                        "seconds_to_tune": time.perf_counter() - t0},
+        "measured": {"fft_split": fft_rows, "buckets": bucket_rows, "graphs": graph_rows},
+    }
+    Path(args.out).write_text(json.dumps(table, indent=1) + "\n")
+    log(f"wrote {args.out}: backend {table['backend']}, batch {batch}")
+    print(json.dumps({"backend": table["backend"], "batch": batch}))
+
+
+if __name__ == "__main__":
+    main()

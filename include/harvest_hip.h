@@ -54,6 +54,33 @@ typedef enum hv_status {
  *   [2] distinct refinement bins */
 #define HV_STAT_SLOTS 3
 
+/* Tuning of a workspace.  No reference counterpart: the package's
+ * autotuner (harvest_cuda/autotune.py) picks the values on the GPU and
+ * applies its committed table to every workspace it creates.
+ *   fft_min_half     bands whose filter half length is at least this, when
+ *                    the longest band fits a 4096-point block, run the
+ *                    overlap-save FFT filterbank; shorter ones the direct FIR
+ *                    (default 64; a value above 1000 makes every band direct);
+ *   graphs           1: each analysis' device launch sequence (decimation
+ *                    through the download of offsets and best) is captured
+ *                    into a HIP graph once per plan shape and buffer set and
+ *                    replayed after that; analyses that collect timings run
+ *                    launch by launch (default 0);
+ *   slab_budget_mib  device memory of the crossing slabs of one band group
+ *                    (default 4096; at least 64).
+ * Results do not depend on the tuning beyond the rounding of the two
+ * filterbank forms (fft_min_half). */
+typedef struct hv_tuning {
+  int fft_min_half;
+  int graphs;
+  int slab_budget_mib;
+} hv_tuning;
+hv_status hv_get_tuning(hv_workspace* ws, hv_tuning* out);
+hv_status hv_set_tuning(hv_workspace* ws, const hv_tuning* in);
+/* Graph launches that replayed a cached graph and graphs captured, since the
+ * workspace was created.  No reference counterpart. */
+hv_status hv_graph_stats(hv_workspace* ws, int64_t* replays, int64_t* captures);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char* hv_last_error(void);
 
