@@ -26,9 +26,15 @@ namespace {
 using Input = py::array_t<double, py::array::c_style>;
 using Cache = WorkspaceCache<hv_workspace>;
 
-// Cache keys: the device index, and the device index plus kSecondSlot for the
-// second workspace a pipelined batch uses on that device.
+// Cache keys: the device index, and the device index plus k kSecondSlot for
+// the k-th extra workspace a pipelined batch uses on that device.
 constexpr int kSecondSlot = 1 << 16;
+
+// Leases of the extra workspaces of a pipelined batch (BatchTuning::depth).
+struct ExtraLeases {
+  std::vector<std::unique_ptr<WorkspaceCache<hv_workspace>::Lease>> leases;
+  std::vector<hv_workspace*> ws;
+};
 
 // Backend tuning applied to every workspace the cache creates (and to the
 // cached ones when it changes); set from the package's tuning table.
@@ -56,6 +62,16 @@ Cache& cache() {
       },
       [](hv_workspace* ws) { hv_workspace_destroy(ws); });
   return *instance;
+}
+
+ExtraLeases extra_workspaces(int device) {
+  ExtraLeases e;
+  const int depth = batch_tuning().depth;
+  for (int k = 1; k < depth; ++k) {
+    e.leases.push_back(std::make_unique<Cache::Lease>(cache(), device + k * kSecondSlot));
+    e.ws.push_back(e.leases.back()->get());
+  }
+  return e;
 }
 
 const double* data_1d(const Input& x) {
@@ -148,9 +164,9 @@ PYBIND11_MODULE(_core, m) {
       std::vector<int64_t> frames;
       for (int64_t n : lengths) frames.push_back(derive_params(n, fs, floor, ceil, period).frames);
       if (pipeline_groups(frames) > 1) {
-        Cache::Lease second(cache(), device + kSecondSlot);
+        ExtraLeases more = extra_workspaces(device);
         out = analyze_batch(ws.get(), ptrs, lengths, fs, floor, ceil, period, host_threads(),
-                            timings ? &t : nullptr, second.get());
+                            timings ? &t : nullptr, more.ws);
       } else {
         out = analyze_batch(ws.get(), ptrs, lengths, fs, floor, ceil, period, host_threads(),
                             timings ? &t : nullptr);
@@ -183,9 +199,9 @@ PYBIND11_MODULE(_core, m) {
       for (int64_t n : lengths) frames.push_back(derive_params(n, fs, floor, ceil, period).frames);
       const double* xd = reinterpret_cast<const double*>(x_device);
       if (pipeline_groups(frames) > 1) {
-        Cache::Lease second(cache(), device + kSecondSlot);
+        ExtraLeases more = extra_workspaces(device);
         rs = analyze_batch_device(ws.get(), xd, lengths, fs, floor, ceil, period, host_threads(),
-                                  timings ? &t : nullptr, second.get());
+                                  timings ? &t : nullptr, more.ws);
       } else {
         rs = analyze_batch_device(ws.get(), xd, lengths, fs, floor, ceil, period, host_threads(),
                                   timings ? &t : nullptr);
@@ -466,16 +482,20 @@ PYBIND11_MODULE(_core, m) {
     return d;
   });
   m.def("_set_batch_tuning", [](int max_groups, int64_t min_group_frames, int64_t split_frames,
-                                double first_weight, double last_weight) {
+                                double first_weight, double last_weight, int serialize,
+                                int depth) {
     BatchTuning t;
     t.max_groups = max_groups;
     t.min_group_frames = min_group_frames;
     t.split_frames = split_frames;
     t.first_weight = first_weight;
     t.last_weight = last_weight;
+    t.serialize = serialize;
+    t.depth = depth;
     set_batch_tuning(t);
   }, py::arg("max_groups"), py::arg("min_group_frames"), py::arg("split_frames"),
-     py::arg("first_weight"), py::arg("last_weight"),
+     py::arg("first_weight"), py::arg("last_weight"), py::arg("serialize") = 0,
+     py::arg("depth") = 2,
      "How large batches are cut into length buckets (engine.h BatchTuning).");
   m.def("_get_batch_tuning", [] {
     const BatchTuning t = batch_tuning();
@@ -485,12 +505,15 @@ PYBIND11_MODULE(_core, m) {
     d["split_frames"] = t.split_frames;
     d["first_weight"] = t.first_weight;
     d["last_weight"] = t.last_weight;
+    d["serialize"] = t.serialize;
+    d["depth"] = t.depth;
     return d;
   });
   m.def("_graph_stats", [](int device) {
     int64_t replays = 0, captures = 0, r = 0, c = 0;
     py::gil_scoped_release release;
-    for (int key : {device, device + kSecondSlot}) {
+    for (int key : {device, device + kSecondSlot, device + 2 * kSecondSlot,
+                    device + 3 * kSecondSlot}) {
       Cache::Lease ws(cache(), key);
       throw_status(hv_graph_stats(ws.get(), &r, &c));
       replays += r;

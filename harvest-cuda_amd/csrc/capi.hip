@@ -242,7 +242,17 @@ struct hv_workspace {
   // then scores (cand_ev[1]).  The _async entry points return before they
   // complete; hv_wait_candidates waits for them.
   hipEvent_t cand_ev[2] = {nullptr, nullptr};
+  // The candidates are copied into h_cand by the launch sequence itself:
+  // f0 at [0, capacity), scores at [capacity, 2 capacity), capacity sized by
+  // the last analysis' candidates per frame (an analysis that finds more
+  // copies them at collect).
+  double cand_per_frame = 20.0;
+  long long cand_capacity = 0;  // of the last analysis
   hv_tuning tune{kFftMinHalf, 0, (int)(kSlabBudget >> 20)};
+  // hv_order_after: an event on another workspace's stream that the next
+  // submitted analysis' device work waits for (after its uploads).
+  hipEvent_t after_ev = nullptr;
+  bool after_pending = false;
   // Executable graphs of the device launch sequence, one per plan shape and
   // buffer set (least recently used first out).
   struct Graph {
@@ -262,6 +272,7 @@ struct hv_workspace {
     for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
     for (auto& e : cand_ev)
       check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    check(hipEventCreateWithFlags(&after_ev, hipEventDisableTiming), "hipEventCreate");
   }
   ~hv_workspace() {
     if (stream) (void)hipStreamSynchronize(stream);
@@ -271,6 +282,7 @@ struct hv_workspace {
       if (e) (void)hipEventDestroy(e);
     for (auto& e : cand_ev)
       if (e) (void)hipEventDestroy(e);
+    if (after_ev) (void)hipEventDestroy(after_ev);
     for (auto& e : timer_events)
       if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
@@ -576,6 +588,8 @@ struct Bufs {
   int32_t* hoff;
   double* hbest;
   unsigned long long* hwork;
+  long long capacity;  // candidates the sequence copies into h_cand (0: none)
+  double* hcand;       // h_cand when capacity > 0
 };
 
 // HV_STAT_SLOTS work counters, then a flag the refinement sets if a
@@ -626,7 +640,17 @@ Bufs size_buffers(hv_workspace* ws, const Plan& p, const double* dx) {
   b.hoff = ws->h_offsets.ensure<int32_t>(p.total_frames + 1);
   b.hbest = ws->h_best.ensure<double>(p.total_frames);
   b.hwork = ws->h_work.ensure<unsigned long long>(kWorkWords);
+  b.capacity = 0;
+  b.hcand = nullptr;
   return b;
+}
+
+// Sizes the in-sequence candidate copy of a two-step analysis.
+void size_candidate_copy(hv_workspace* ws, const Plan& p, Bufs& b) {
+  const long long slots = p.total_frames * (long long)p.cfg.slots;
+  b.capacity = std::min<long long>(slots, std::max<long long>(
+      1024, (long long)std::ceil(ws->cand_per_frame * (double)p.total_frames)));
+  b.hcand = ws->h_cand.ensure<double>(2 * (size_t)b.capacity);
 }
 
 // Decimation (or the copy at ratio 1) and mean removal: y on the device.
@@ -700,6 +724,18 @@ void enqueue_analysis(hv_workspace* ws, const Plan& p, const Bufs& b, bool event
     launch_compact(s, c, dutt, ws->d_f_off, p.total_frames, b.count, b.official, b.pf0, b.psc,
                    b.doffsets, b.rf0, b.rsc, b.cinit);
   });
+  // Two-step (pipelined) analyses copy the candidates into pinned host
+  // memory in the same sequence (the first `capacity` of each array; the
+  // device's total is known only at collect): the host finds them in place
+  // when the sequence ends, with no download queued behind the other
+  // workspace's kernels.  Direct calls download them at collect instead,
+  // overlapping the first contour steps.
+  if (b.capacity > 0) {
+    check(hipMemcpyAsync(b.hcand, b.rf0, b.capacity * sizeof(double), hipMemcpyDeviceToHost, s),
+          "download candidates");
+    check(hipMemcpyAsync(b.hcand + b.capacity, b.rsc, b.capacity * sizeof(double),
+                         hipMemcpyDeviceToHost, s), "download candidates");
+  }
   check(hipMemcpyAsync(b.hoff, b.doffsets, (p.total_frames + 1) * sizeof(int32_t),
                        hipMemcpyDeviceToHost, s), "download offsets");
   check(hipMemcpyAsync(b.hbest, b.dbest, p.total_frames * sizeof(double), hipMemcpyDeviceToHost,
@@ -726,8 +762,10 @@ std::vector<long long> graph_key(hv_workspace* ws, const Plan& p, const Bufs& b)
                         b.y, b.pass, b.states, b.carries, b.carry_tmp, b.partial, b.mean, b.quirk,
                         b.partial3, b.sums3, b.slab, b.counts, b.offs, b.totals, b.raw, b.tspec,
                         b.count, b.official, b.rf0, b.rsc, b.pf0, b.psc, b.dbest, b.cinit, b.work,
-                        b.row_nnz, b.doffsets, b.scan_tmp, b.hoff, b.hbest, b.hwork};
+                        b.row_nnz, b.doffsets, b.scan_tmp, b.hoff, b.hbest, b.hwork,
+                        b.hcand};
   for (const void* q : ptrs) k.push_back((long long)(uintptr_t)q);
+  k.push_back(b.capacity);
   return k;
 }
 
@@ -779,8 +817,9 @@ void launch_as_graph(hv_workspace* ws, const Plan& p, const Bufs& b) {
 // asked for, else launch by launch with span events.
 // With dx == nullptr, y is already in place (hv_analyze): no preprocessing,
 // and the sequence runs launch by launch.
-void analyze_submit(hv_workspace* ws, const Plan& p, const double* dx) {
-  const Bufs b = size_buffers(ws, p, dx);
+void analyze_submit(hv_workspace* ws, const Plan& p, const double* dx, bool copy_candidates) {
+  Bufs b = size_buffers(ws, p, dx);
+  if (copy_candidates) size_candidate_copy(ws, p, b);
   if (dx && ws->tune.graphs && !ws->timing) {
     launch_as_graph(ws, p, b);
   } else {
@@ -788,6 +827,7 @@ void analyze_submit(hv_workspace* ws, const Plan& p, const double* dx) {
     enqueue_analysis(ws, p, b, true);
   }
   check(hipEventRecord(ws->ev[3], s_of(ws)), "hipEventRecord");
+  ws->cand_capacity = b.capacity;
   ws->pending_frames = p.total_frames;
   ws->submitted = true;
 }
@@ -814,20 +854,34 @@ void analyze_collect(hv_workspace* ws, int32_t* offsets, double* best, int64_t* 
   }
   if (hwork[HV_STAT_SLOTS])
     throw std::runtime_error("refinement window beyond its precomputed bound");
-  // The candidates go straight to the pinned staging buffer, where the host
-  // contour reads them in place (hv_result_view) or callers copy them out.
-  // The stream is idle here, so no earlier download still targets h_cand.
+  // The candidates end in h_cand (f0 at [0, capacity), scores at
+  // [capacity, ...)), where the host contour reads them in place
+  // (hv_result_view) or callers copy them out.
   const long long n = hoff[frames];
-  double* hc = ws->h_cand.ensure<double>(2 * std::max(n, 1LL));
-  if (n > 0)
-    check(hipMemcpyAsync(hc, rf0, n * sizeof(double), hipMemcpyDeviceToHost, s),
-          "download candidates");
-  check(hipEventRecord(ws->cand_ev[0], s), "hipEventRecord");
-  if (n > 0)
-    check(hipMemcpyAsync(hc + n, rsc, n * sizeof(double), hipMemcpyDeviceToHost, s),
-          "download candidates");
-  check(hipEventRecord(ws->cand_ev[1], s), "hipEventRecord");
-  if (!defer) check(hipStreamSynchronize(s), "download candidates");
+  if (ws->cand_capacity > 0 && n <= ws->cand_capacity) {
+    // Copied by the sequence: both parts are in place already.
+    check(hipEventRecord(ws->cand_ev[0], s), "hipEventRecord");
+    check(hipEventRecord(ws->cand_ev[1], s), "hipEventRecord");
+  } else {
+    // Downloaded now, f0 then scores (an in-sequence copy that was too small
+    // is redone in full).  The stream is idle, so nothing else targets h_cand.
+    const bool overflow = ws->cand_capacity > 0;
+    const long long cap = std::max(n, 1LL);
+    double* hc = ws->h_cand.ensure<double>(2 * (size_t)cap);
+    if (n > 0)
+      check(hipMemcpyAsync(hc, rf0, n * sizeof(double), hipMemcpyDeviceToHost, s),
+            "download candidates");
+    check(hipEventRecord(ws->cand_ev[0], s), "hipEventRecord");
+    if (n > 0)
+      check(hipMemcpyAsync(hc + cap, rsc, n * sizeof(double), hipMemcpyDeviceToHost, s),
+            "download candidates");
+    check(hipEventRecord(ws->cand_ev[1], s), "hipEventRecord");
+    ws->cand_capacity = cap;
+    if (!defer || overflow) check(hipStreamSynchronize(s), "download candidates");
+  }
+  // The next in-sequence copy takes 10 % more than this analysis' density
+  // (at least 20 candidates per frame; the usual density is 15-19).
+  ws->cand_per_frame = std::max(20.0, 1.1 * (double)n / (double)std::max(frames, 1LL));
   if (offsets) std::memcpy(offsets, hoff, (frames + 1) * sizeof(int32_t));
   if (best) std::memcpy(best, hbest, frames * sizeof(double));
   ws->last_nnz = n;
@@ -846,7 +900,7 @@ void analyze_collect(hv_workspace* ws, int32_t* offsets, double* best, int64_t* 
 
 void analyze_device(hv_workspace* ws, const Plan& p, const double* dx, int32_t* offsets,
                     double* best, int64_t* nnz, double* timings, bool defer = false) {
-  analyze_submit(ws, p, dx);
+  analyze_submit(ws, p, dx, false);
   analyze_collect(ws, offsets, best, nnz, timings, defer);
 }
 
@@ -987,7 +1041,11 @@ void submit_batch(hv_workspace* ws, int count, const double* const* xs, const in
   const double* dx = nullptr;
   const Plan p = stage_batch(ws, count, xs, lengths, fs, f0_floor, f0_ceil,
                              timings ? &on : nullptr, device_x, &dx);
-  analyze_submit(ws, p, dx);
+  if (ws->after_pending) {
+    check(hipStreamWaitEvent(ws->stream, ws->after_ev, 0), "hipStreamWaitEvent");
+    ws->after_pending = false;
+  }
+  analyze_submit(ws, p, dx, true);
 }
 
 // Frozen-decision refinement of entries (value and/or adjoint), then the
@@ -1376,6 +1434,17 @@ hv_status hv_set_tuning(hv_workspace* ws, const hv_tuning* in) {
   });
 }
 
+hv_status hv_order_after(hv_workspace* ws, hv_workspace* before) {
+  return guarded([&] {
+    if (!ws || !before) invalid("null workspace");
+    if (ws == before) invalid("a workspace is ordered after itself already");
+    if (ws->device != before->device) invalid("workspaces on different devices");
+    DeviceGuard guard(ws->device);
+    check(hipEventRecord(ws->after_ev, before->stream), "hipEventRecord");
+    ws->after_pending = true;
+  });
+}
+
 hv_status hv_graph_stats(hv_workspace* ws, int64_t* replays, int64_t* captures) {
   return guarded([&] {
     if (!ws || !replays || !captures) invalid("null pointer");
@@ -1403,7 +1472,7 @@ hv_status hv_fetch_candidates(hv_workspace* ws, double* f0, double* scores) {
     const long long n = ws->last_nnz;
     const double* h = ws->h_cand.get<double>();
     std::memcpy(f0, h, n * sizeof(double));
-    std::memcpy(scores, h + n, n * sizeof(double));
+    std::memcpy(scores, h + ws->cand_capacity, n * sizeof(double));
   });
 }
 
@@ -1415,7 +1484,7 @@ hv_status hv_result_view(hv_workspace* ws, const int32_t** offsets, const double
     *offsets = ws->h_offsets.get<int32_t>();
     *best = ws->h_best.get<double>();
     *f0 = ws->h_cand.get<double>();
-    *scores = ws->h_cand.get<double>() + ws->last_nnz;
+    *scores = ws->h_cand.get<double>() + ws->cand_capacity;
     *nnz = ws->last_nnz;
   });
 }

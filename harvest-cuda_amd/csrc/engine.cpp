@@ -889,7 +889,8 @@ BatchTuning g_tuning;
 
 void set_batch_tuning(const BatchTuning& t) {
   if (t.max_groups < 1 || t.min_group_frames < 1 || t.split_frames < 0 || !(t.first_weight > 0) ||
-      !(t.last_weight > 0) || !std::isfinite(t.first_weight) || !std::isfinite(t.last_weight))
+      !(t.last_weight > 0) || !std::isfinite(t.first_weight) || !std::isfinite(t.last_weight) ||
+      (t.serialize != 0 && t.serialize != 1) || t.depth < 2 || t.depth > 4)
     throw std::invalid_argument("invalid batch tuning");
   std::lock_guard<std::mutex> lock(g_tuning_mutex);
   g_tuning = t;
@@ -964,7 +965,7 @@ void accumulate_device(hv_workspace* ws, const double* ms, Timings& t) {
 // run, the host stages the next group's input and runs the previous group's
 // contours.
 template <class Submit>
-std::vector<Result> batch_pipelined(hv_workspace* ws0, hv_workspace* ws1, Submit&& submit,
+std::vector<Result> batch_pipelined(const std::vector<hv_workspace*>& ws, Submit&& submit,
                                     const std::vector<int64_t>& lengths, const Buckets& bk,
                                     int fs, double f0_floor, double f0_ceil, double frame_period,
                                     int host_threads, Timings* timings) {
@@ -975,28 +976,29 @@ std::vector<Result> batch_pipelined(hv_workspace* ws0, hv_workspace* ws1, Submit
   for (int u = 0; u < count; ++u)
     params[u] = derive_params(lengths[u], fs, f0_floor, f0_ceil, frame_period);
   const int groups = (int)bk.bounds.size() - 1;
-  hv_workspace* ws[2] = {ws0, ws1};
-  bool pending[2] = {false, false};
+  const int depth = (int)ws.size();
+  std::vector<char> pending(depth, 0);
   std::vector<Result> results(count);
   // Host time of each phase (with timings): staging and enqueueing a group,
   // waiting for its device work, and its contours.
   double host_submit = 0.0, host_wait = 0.0, host_contour = 0.0, worst_wait = 0.0;
   Clock::time_point phase = Clock::now();
+  const bool serialize = batch_tuning().serialize != 0;
   auto enqueue = [&](int g) {
     if (timings) ms_since(phase);
-    throw_status(submit(ws[g % 2], bk.order.data() + bk.bounds[g], bk.bounds[g + 1] - bk.bounds[g],
-                        timings != nullptr));
-    pending[g % 2] = true;
+    if (serialize && g > 0) throw_status(hv_order_after(ws[g % depth], ws[(g - 1) % depth]));
+    throw_status(submit(ws[g % depth], bk.order.data() + bk.bounds[g],
+                        bk.bounds[g + 1] - bk.bounds[g], timings != nullptr));
+    pending[g % depth] = 1;
     if (timings) host_submit += ms_since(phase);
   };
   try {
-    enqueue(0);
-    if (groups > 1) enqueue(1);
+    for (int g = 0; g < std::min(depth, groups); ++g) enqueue(g);
     for (int g = 0; g < groups; ++g) {
-      hv_workspace* w = ws[g % 2];
+      hv_workspace* w = ws[g % depth];
       int64_t nnz = 0;
       double ms[HV_TIMING_SLOTS];
-      pending[g % 2] = false;
+      pending[g % depth] = 0;
       if (timings) ms_since(phase);
       throw_status(hv_analyze_batch_collect(w, nullptr, nullptr, &nnz, timings ? ms : nullptr));
       if (timings) {
@@ -1026,12 +1028,12 @@ std::vector<Result> batch_pipelined(hv_workspace* ws0, hv_workspace* ws1, Submit
                          params[u].out_frames, results[u], nullptr, nullptr, nullptr, inner);
       });
       if (timings) host_contour += ms_since(phase);
-      // The workspace's pinned results are read: it takes the group after next.
-      if (g + 2 < groups) enqueue(g + 2);
+      // The workspace's pinned results are read: it takes the next group.
+      if (g + depth < groups) enqueue(g + depth);
     }
   } catch (...) {
-    // Leave both workspaces usable: collect (and drop) what is still enqueued.
-    for (int k = 0; k < 2; ++k) {
+    // Leave every workspace usable: collect (and drop) what is still enqueued.
+    for (int k = 0; k < depth; ++k) {
       int64_t nnz = 0;
       if (pending[k]) (void)hv_analyze_batch_collect(ws[k], nullptr, nullptr, &nnz, nullptr);
     }
@@ -1060,10 +1062,11 @@ std::vector<int64_t> frames_of(const std::vector<int64_t>& lengths, int fs, doub
 std::vector<Result> analyze_batch(hv_workspace* ws, const std::vector<const double*>& xs,
                                   const std::vector<int64_t>& lengths, int fs,
                                   double f0_floor, double f0_ceil, double frame_period,
-                                  int host_threads, Timings* timings, hv_workspace* second) {
+                                  int host_threads, Timings* timings,
+                                  const std::vector<hv_workspace*>& more) {
   const std::vector<int64_t> frames = frames_of(lengths, fs, f0_floor, f0_ceil, frame_period);
   const int groups = pipeline_groups(frames);
-  if (second && groups > 1) {
+  if (!more.empty() && groups > 1) {
     // Host input: any utterance order can be staged, so groups are length
     // buckets (longest first).
     const Buckets bk = make_buckets(frames, groups, true);
@@ -1077,7 +1080,9 @@ std::vector<Result> analyze_batch(hv_workspace* ws, const std::vector<const doub
       return hv_analyze_batch_submit(w, n, p.data(), l.data(), fs, f0_floor, f0_ceil,
                                      timed ? 1 : 0);
     };
-    return batch_pipelined(ws, second, submit, lengths, bk, fs, f0_floor, f0_ceil, frame_period,
+    std::vector<hv_workspace*> all{ws};
+    all.insert(all.end(), more.begin(), more.end());
+    return batch_pipelined(all, submit, lengths, bk, fs, f0_floor, f0_ceil, frame_period,
                            host_threads, timings);
   }
   auto call = [&](int32_t* offsets, double* best, int64_t* nnz, double* ms) {
@@ -1092,12 +1097,12 @@ std::vector<Result> analyze_batch_device(hv_workspace* ws, const double* x_devic
                                          const std::vector<int64_t>& lengths, int fs,
                                          double f0_floor, double f0_ceil, double frame_period,
                                          int host_threads, Timings* timings,
-                                         hv_workspace* second) {
+                                         const std::vector<hv_workspace*>& more) {
   const std::vector<int64_t> frames = frames_of(lengths, fs, f0_floor, f0_ceil, frame_period);
   std::vector<int64_t> x_off(lengths.size() + 1, 0);
   for (size_t u = 0; u < lengths.size(); ++u) x_off[u + 1] = x_off[u] + lengths[u];
   const int groups = pipeline_groups(frames);
-  if (second && groups > 1) {
+  if (!more.empty() && groups > 1) {
     // Back-to-back device input: groups are runs of consecutive utterances.
     const Buckets bk = make_buckets(frames, groups, false);
     auto submit = [&](hv_workspace* w, const int* idx, int n, bool timed) {
@@ -1105,7 +1110,9 @@ std::vector<Result> analyze_batch_device(hv_workspace* ws, const double* x_devic
       return hv_analyze_batch_device_submit(w, n, x_device + x_off[idx[0]], l.data(), fs,
                                             f0_floor, f0_ceil, timed ? 1 : 0);
     };
-    return batch_pipelined(ws, second, submit, lengths, bk, fs, f0_floor, f0_ceil, frame_period,
+    std::vector<hv_workspace*> all{ws};
+    all.insert(all.end(), more.begin(), more.end());
+    return batch_pipelined(all, submit, lengths, bk, fs, f0_floor, f0_ceil, frame_period,
                            host_threads, timings);
   }
   auto call = [&](int32_t* offsets, double* best, int64_t* nnz, double* ms) {

@@ -51,23 +51,23 @@ Result analyze(hv_workspace* ws, const double* x, int64_t length, int fs, double
                double f0_ceil, double frame_period, int host_threads = 1,
                Timings* timings = nullptr);
 // Batched analysis: one device launch sequence for all utterances, then the
-// host contour of every utterance on a pool of host threads.  Given a second
-// workspace on the same device, a large batch (see pipeline_groups) is split
-// into groups of consecutive utterances analysed alternately on the two
-// workspaces, so that the host contours and candidate download of one group
-// overlap the device work of the next.  Results do not depend on the split.
+// host contour of every utterance on a pool of host threads.  Given more
+// workspaces on the same device, a large batch (see pipeline_groups) is cut
+// into length buckets analysed in turn on the workspaces, so that the host
+// staging and contours of some buckets overlap the device work of others.
+// Results do not depend on the split.
 std::vector<Result> analyze_batch(hv_workspace* ws, const std::vector<const double*>& xs,
                                   const std::vector<int64_t>& lengths, int fs,
                                   double f0_floor, double f0_ceil, double frame_period,
                                   int host_threads, Timings* timings = nullptr,
-                                  hv_workspace* second = nullptr);
+                                  const std::vector<hv_workspace*>& more = {});
 
 // Batched analysis of waveforms already in device memory (back to back).
 std::vector<Result> analyze_batch_device(hv_workspace* ws, const double* x_device,
                                          const std::vector<int64_t>& lengths, int fs,
                                          double f0_floor, double f0_ceil, double frame_period,
                                          int host_threads, Timings* timings = nullptr,
-                                         hv_workspace* second = nullptr);
+                                         const std::vector<hv_workspace*>& more = {});
 
 // How a large batch is cut into length buckets (pipeline groups): a batch
 // of at least 4 utterances and `split_frames` 1 ms frames is split into up
@@ -83,6 +83,13 @@ struct BatchTuning {
   int64_t split_frames = 200000;
   double first_weight = 1.0;
   double last_weight = 1.0;
+  // 1: a group's device work waits for the previous group's (the two
+  // workspaces' kernels do not run concurrently; uploads, downloads and host
+  // work still overlap the device).
+  int serialize = 0;
+  // Workspaces a pipelined batch keeps in flight (2-4): groups g, g + 1, ...,
+  // g + depth - 1 are enqueued while group g's contours run.
+  int depth = 2;
 };
 void set_batch_tuning(const BatchTuning& t);
 BatchTuning batch_tuning();

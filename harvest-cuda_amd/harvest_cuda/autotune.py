@@ -10,9 +10,11 @@ on:
      time on one 60 s utterance at 24 kHz for each candidate split, keeping
      only splits whose F0 equals the default's to 1e-9 with identical
      voicing;
-  2. the length buckets (max_groups, first_weight, last_weight): the wall
+  2. the length buckets (max_groups, first_weight, last_weight) and whether
+     the buckets' device work runs one after another (serialize): the wall
      time of one batched call, NumPy in and out, median of 3, over a grid;
-  3. HIP graph capture on or off at the chosen buckets.
+  3. how many workspaces keep buckets in flight (depth 2-4);
+  4. HIP graph capture on or off at the chosen buckets.
 The winners and every measurement go to the table, which the package
 applies at import (harvest_cuda/tuning.py).
 """
@@ -86,15 +88,17 @@ def batch_wall(xs, fs, repeats: int) -> float:
 
 def tune_buckets(xs, fs, repeats: int = 3) -> tuple[dict, list]:
     rows = []
-    for g, first, last in itertools.product(GROUPS, WEIGHTS, WEIGHTS):
+    for serialize, g, first, last in itertools.product((0, 1), GROUPS, WEIGHTS, WEIGHTS):
         tuning.set_batch(max_groups=g, min_group_frames=20000, split_frames=200000,
-                         first_weight=first, last_weight=last)
+                         first_weight=first, last_weight=last, serialize=serialize)
         ms = batch_wall(xs, fs, repeats)
-        rows.append({"max_groups": g, "first_weight": first, "last_weight": last, "ms": ms})
-        log(f"groups {g} first {first} last {last}: {ms:.1f} ms")
+        rows.append({"serialize": serialize, "max_groups": g, "first_weight": first,
+                     "last_weight": last, "ms": ms})
+        log(f"serialize {serialize} groups {g} first {first} last {last}: {ms:.1f} ms")
     best = min(rows, key=lambda r: r["ms"])
     chosen = {"max_groups": best["max_groups"], "min_group_frames": 20000, "split_frames": 200000,
-              "first_weight": best["first_weight"], "last_weight": best["last_weight"]}
+              "first_weight": best["first_weight"], "last_weight": best["last_weight"],
+              "serialize": best["serialize"]}
     return chosen, rows
 
 
@@ -114,6 +118,14 @@ def main():
     log(f"calibration batch: {len(xs)} utterances, {audio:.0f} audio-s")
     batch, bucket_rows = tune_buckets(xs, fs)
     tuning.set_batch(**batch)
+    depth_rows = []
+    for depth in (2, 3, 4):
+        tuning.set_batch(depth=depth)
+        ms = batch_wall(xs, fs, 5)
+        depth_rows.append({"depth": depth, "ms": ms})
+        log(f"depth {depth}: {ms:.1f} ms")
+    batch["depth"] = min(depth_rows, key=lambda r: r["ms"])["depth"]
+    tuning.set_batch(**batch)
     graph_rows = []
     for graphs in (0, 1):
         tuning.set_backend(graphs=graphs)
@@ -126,7 +138,8 @@ def main():
         "batch": batch,
         "calibration": {"utterances": len(xs), "audio_seconds": audio, "fs": fs, "seed": 2003,
                         "seconds_to_tune": time.perf_counter() - t0},
-        "measured": {"fft_split": fft_rows, "buckets": bucket_rows, "graphs": graph_rows},
+        "measured": {"fft_split": fft_rows, "buckets": bucket_rows, "depth": depth_rows,
+                     "graphs": graph_rows},
     }
     Path(args.out).write_text(json.dumps(table, indent=1) + "\n")
     log(f"wrote {args.out}: backend {table['backend']}, batch {batch}")

@@ -4,7 +4,10 @@
   backend: fft_min_half (FFT/direct filterbank split), graphs (HIP graph
            capture of the launch sequence), slab_budget_mib;
   batch:   how large batches are cut into length buckets (max_groups,
-           min_group_frames, split_frames, first_weight, last_weight);
+           min_group_frames, split_frames, first_weight, last_weight) and
+           whether the buckets' device work runs one after another
+           (serialize), and how many workspaces hold buckets in flight
+           (depth);
 and the measurements behind the choice.  ``apply()`` runs at import.
 """
 from __future__ import annotations
@@ -16,7 +19,8 @@ from . import _core
 
 TABLE = Path(__file__).resolve().parent / "tuning.json"
 BACKEND_KEYS = ("fft_min_half", "graphs", "slab_budget_mib")
-BATCH_KEYS = ("max_groups", "min_group_frames", "split_frames", "first_weight", "last_weight")
+BATCH_KEYS = ("max_groups", "min_group_frames", "split_frames", "first_weight", "last_weight",
+              "serialize", "depth")
 
 
 def load(path: Path = TABLE) -> dict:
@@ -34,7 +38,8 @@ def set_batch(**kw) -> None:
     cur.update(kw)
     _core._set_batch_tuning(int(cur["max_groups"]), int(cur["min_group_frames"]),
                             int(cur["split_frames"]), float(cur["first_weight"]),
-                            float(cur["last_weight"]))
+                            float(cur["last_weight"]), int(cur.get("serialize", 0)),
+                            int(cur.get("depth", 2)))
 
 
 def current() -> dict:

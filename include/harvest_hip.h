@@ -206,6 +206,13 @@ hv_status hv_analyze_batch_device_submit(hv_workspace* ws, int count, const doub
 hv_status hv_analyze_batch_collect(hv_workspace* ws, int32_t* offsets, double* best,
                                    int64_t* nnz, double* timings_ms);
 
+/* The device work of the next analysis submitted on `ws` (after its input
+ * uploads) starts only when everything enqueued on `before` so far has
+ * finished: the host engine uses it to run the groups of a pipelined batch
+ * on the device one after another (their uploads and the host work still
+ * overlap), when the tuning asks for that.  No reference counterpart. */
+hv_status hv_order_after(hv_workspace* ws, hv_workspace* before);
+
 /* Wait until the last analysis' candidate f0 (part 0) or f0 and scores
  * (part 1) are in the pinned buffers hv_result_view exposes.  Returns at once
  * after the synchronous entry points.  No reference counterpart. */

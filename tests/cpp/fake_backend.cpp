@@ -232,6 +232,11 @@ hv_status hv_result_view(hv_workspace* ws, const int32_t** offsets, const double
   return HV_OK;
 }
 
+hv_status hv_order_after(hv_workspace* ws, hv_workspace* before) {
+  if (!ws || !before || ws == before) return fail(HV_EINVAL, "bad workspaces");
+  return HV_OK;  // the stand-in runs each analysis at collect, in call order
+}
+
 hv_status hv_last_stats(hv_workspace* ws, int64_t* stats) {
   if (!ws || !stats) return fail(HV_EINVAL, "null pointer");
   for (int k = 0; k < HV_STAT_SLOTS; ++k) stats[k] = 0;

@@ -91,14 +91,46 @@ int main() {
 
   require(same(analyze_batch(a, w.ptrs, w.lengths, w.fs, 71.0, 800.0, 5.0, 16), ref),
           "one-group batch equals plain calls");
-  require(same(analyze_batch(a, w.ptrs, w.lengths, w.fs, 71.0, 800.0, 5.0, 16, nullptr, b), ref),
+  require(same(analyze_batch(a, w.ptrs, w.lengths, w.fs, 71.0, 800.0, 5.0, 16, nullptr, {b}), ref),
           "pipelined batch equals plain calls");
+  {
+    hv_workspace* c = fake_workspace_create(0);
+    hv_workspace* d = fake_workspace_create(0);
+    require(same(analyze_batch(a, w.ptrs, w.lengths, w.fs, 71.0, 800.0, 5.0, 16, nullptr,
+                               {b, c, d}), ref),
+            "four workspaces in flight");
+    fake_workspace_destroy(c);
+    fake_workspace_destroy(d);
+  }
   Timings t;
-  require(same(analyze_batch(a, w.ptrs, w.lengths, w.fs, 71.0, 800.0, 5.0, 16, &t, b), ref),
+  require(same(analyze_batch(a, w.ptrs, w.lengths, w.fs, 71.0, 800.0, 5.0, 16, &t, {b}), ref),
           "pipelined batch with timings equals plain calls");
+  {
+    // Other bucket shapes and serialised groups give the same results.
+    const BatchTuning saved = batch_tuning();
+    BatchTuning bt;
+    bt.max_groups = 9;
+    bt.min_group_frames = 1000;
+    bt.first_weight = 0.3;
+    bt.last_weight = 0.5;
+    bt.serialize = 1;
+    set_batch_tuning(bt);
+    require(pipeline_groups(frames) == 9, "nine buckets");
+    require(same(analyze_batch(a, w.ptrs, w.lengths, w.fs, 71.0, 800.0, 5.0, 16, nullptr, {b}), ref),
+            "serialised weighted buckets equal plain calls");
+    set_batch_tuning(saved);
+    bool threw = false;
+    try {
+      bt.first_weight = 0.0;
+      set_batch_tuning(bt);
+    } catch (const std::invalid_argument&) {
+      threw = true;
+    }
+    require(threw && batch_tuning().max_groups == saved.max_groups, "invalid tuning rejected");
+  }
   require(t.count("groups") && t["groups"] > 1, "timings report the groups");
   require(same(analyze_batch_device(a, w.flat.data(), w.lengths, w.fs, 71.0, 800.0, 5.0, 16,
-                                    nullptr, b), ref),
+                                    nullptr, {b}), ref),
           "pipelined back-to-back batch equals plain calls");
 
   // Concurrent callers on their own workspaces share the host pool.
@@ -114,7 +146,7 @@ int main() {
           if (k == 3)
             good &= same(plain(x, w, 4), ref);
           else
-            good &= same(analyze_batch(x, w.ptrs, w.lengths, w.fs, 71.0, 800.0, 5.0, 8, nullptr, y),
+            good &= same(analyze_batch(x, w.ptrs, w.lengths, w.fs, 71.0, 800.0, 5.0, 8, nullptr, {y}),
                          ref);
         }
         ok[k] = good;
@@ -131,12 +163,12 @@ int main() {
   fake_fail_next_submits(b, 1);
   bool threw = false;
   try {
-    analyze_batch(a, w.ptrs, w.lengths, w.fs, 71.0, 800.0, 5.0, 16, nullptr, b);
+    analyze_batch(a, w.ptrs, w.lengths, w.fs, 71.0, 800.0, 5.0, 16, nullptr, {b});
   } catch (const std::runtime_error& e) {
     threw = std::strstr(e.what(), "injected device failure") != nullptr;
   }
   require(threw, "the device failure surfaces as runtime_error");
-  require(same(analyze_batch(a, w.ptrs, w.lengths, w.fs, 71.0, 800.0, 5.0, 16, nullptr, b), ref),
+  require(same(analyze_batch(a, w.ptrs, w.lengths, w.fs, 71.0, 800.0, 5.0, 16, nullptr, {b}), ref),
           "both workspaces are usable after the failure");
   fake_workspace_destroy(a);
   fake_workspace_destroy(b);
