@@ -87,28 +87,74 @@ def measured_traffic(kernel, workload):
     return None
 
 
-def roofline(kernels_ms, work, launches, workload):
-    """Roofline object of the kernel with the largest time per step.  The
-    refinement kernel is bound by FP64 vector FMAs (no MFMA: its DFT bins are
-    sums over a window, not a dense contraction); its algorithmic FLOPs are
-    8 per bin-sample (two Goertzel steps: an FMA and a subtraction each, for
-    the main and the derivative window) plus 17 per window sample (window,
-    its central difference and the two products), counted on the device."""
-    name = max(kernels_ms, key=kernels_ms.get)
-    per_launch_ms = kernels_ms[name] / launches
-    if name == "kernel_refine":
-        flops = (8.0 * work["bin_samples"] + 17.0 * work["window_samples"]) / launches
-        achieved = flops / (per_launch_ms / 1000.0) / 1e12
-        return {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                "traffic": measured_traffic("k_refine_waves", workload),
-                "kernel": "k_refine_waves", "avg_launch_ms": per_launch_ms,
-                "launches_per_step": launches, "flop_per_launch": flops,
-                "model": "8 FLOP per bin-sample + 17 FLOP per window sample (FP64 vector FMA "
-                         "peak 78.6 TFLOP/s, the same as MI355X's FP64 matrix peak)"}
-    return {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
-            "traffic": None, "kernel": KERNEL_NAMES.get(name, name), "avg_launch_ms": per_launch_ms,
-            "launches_per_step": launches}
+def filterbank_flops(lengths, fs, fft_min_half, floor=71.0, ceil=800.0):
+    """FP64 FLOPs of the filterbank as this build computes it, summed over
+    utterances: bands whose half length h is at least fft_min_half run by
+    overlap-save FFT (per 2046-sample tile one forward real 4096-point
+    transform, 2.5 n log2 n with n = 4096, then per band the spectrum
+    product, 6 FLOP per bin for 2049 bins, and the inverse transform), the
+    others by the direct FIR (2 FLOP per tap per output sample)."""
+    import math
+    from harvest_cuda import _core
+    p = _core._params(lengths[0], fs, floor, ceil, 5.0)
+    fs_w = p["fs_work"]
+    halves = []
+    for b in range(p["bands"]):
+        x = fs_w / (floor * 0.9 * 2.0 ** ((b + 1) / 40.0)) * 2.0
+        halves.append(int(x + 0.5))
+    nfft = 0
+    if halves[0] <= 1000:
+        while nfft < len(halves) and fft_min_half <= halves[nfft] <= halves[0]:
+            nfft += 1
+    taps = sum(2 * h + 1 for h in halves[nfft:])
+    fft4096 = 2.5 * 4096 * 12
+    total = 0.0
+    for L in lengths:
+        n = _core._params(L, fs, floor, ceil, 5.0)["y_length"]
+        tiles = math.ceil(n / 2046)
+        total += 2.0 * taps * n
+        if nfft:
+            total += tiles * fft4096 + tiles * nfft * (6 * 2049 + fft4096)
+    return total
+
+
+def roofline(kernels_ms, work, launches, workload, lengths, fs):
+    """Roofline objects of the refinement and filterbank kernels, and of
+    the one with the larger time per step.  Both are bound by FP64 vector
+    FMAs (no MFMA: neither is a dense contraction), against the 78.6 TFLOP/s
+    FP64 vector peak (MI355X's FP64 matrix peak is the same).  The
+    refinement's algorithmic FLOPs are 8 per bin-sample (a Goertzel step, an
+    FMA and a subtraction, for the main and the derivative window) plus 17
+    per window sample, counted on the device; the filterbank's are
+    filterbank_flops().  `launches` is the analyses per step (one per bucket):
+    a launch here is one analysis' refinement kernel, or its filterbank
+    stage (tile spectra, FFT bands, direct bands)."""
+    from harvest_cuda import tuning
+    fft_min = tuning.current()["backend"]["fft_min_half"]
+    out = {}
+    per = kernels_ms["kernel_refine"] / launches
+    flops = (8.0 * work["bin_samples"] + 17.0 * work["window_samples"]) / launches
+    ach = flops / (per / 1000.0) / 1e12
+    out["kernel_refine"] = {
+        "bound": "fp64_valu", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "frac": ach / FP64_PEAK_TFLOPS, "traffic": measured_traffic("k_refine_waves", workload),
+        "kernel": "k_refine_waves", "avg_launch_ms": per, "launches_per_step": launches,
+        "flop_per_launch": flops, "bin_samples_per_launch": work["bin_samples"] / launches,
+        "window_samples_per_launch": work["window_samples"] / launches,
+        "model": "8 FLOP per bin-sample + 17 FLOP per window sample"}
+    per = kernels_ms["kernel_filterbank"] / launches
+    flops = filterbank_flops(lengths, fs, fft_min) / launches
+    ach = flops / (per / 1000.0) / 1e12
+    out["kernel_filterbank"] = {
+        "bound": "fp64_valu", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "frac": ach / FP64_PEAK_TFLOPS,
+        "traffic": measured_traffic("k_filter_fft+k_filter_events", workload),
+        "kernel": "k_spectrum+k_filter_fft+k_filter_events", "avg_launch_ms": per,
+        "launches_per_step": launches, "flop_per_launch": flops,
+        "model": f"FFT bands (h >= {fft_min}): 2.5 n log2 n per 4096-point transform + 6 FLOP "
+                 "per bin; direct bands: 2 FLOP per tap per output sample"}
+    name = max(("kernel_refine", "kernel_filterbank"), key=lambda k: kernels_ms[k])
+    return out[name], out
 
 
 def cpu_model():
@@ -243,11 +289,12 @@ def batched_leg(args, result):
                   "note": "one harvest(x) call per utterance, NumPy in and out"},
         "batched_over_plain": (plain_med / ms_per_step) if plain_med else None,
         "batched_equals_plain_bitwise": bool(identical) if plain_ms else None,
-        "roofline": roofline({k: v / steps for k, v in kernels.items()},
-                             {k: v / steps for k, v in work.items()}, launches // steps, workload),
         "kernels_ms_per_step": {k: v / steps for k, v in kernels.items()},
         "device_ms_per_step": device_ms / steps,
     })
+    result["roofline"], result["rooflines"] = roofline(
+        {k: v / steps for k, v in kernels.items()}, {k: v / steps for k, v in work.items()},
+        launches // steps, workload, lengths, fs)
     a_bytes = algorithmic_bytes(lengths, fs)
     dev_s = device_ms / steps / 1000.0
     result["algorithmic_hbm"] = {
@@ -309,8 +356,9 @@ def config2_leg(args):
             "hbm_input": {"ms_per_call": hbm_ms, "audio_sec_per_sec": seconds / (hbm_ms / 1000.0)},
             "device_ms": device_ms / steps,
             "kernels_ms_per_call": {k: v / steps for k, v in kernels.items()},
-            "roofline": roofline({k: v / steps for k, v in kernels.items()},
-                                 {k: v / steps for k, v in work.items()}, 1, "config2"),
+            "rooflines": roofline({k: v / steps for k, v in kernels.items()},
+                                  {k: v / steps for k, v in work.items()}, 1, "config2",
+                                  [len(x)], fs)[1],
             "host_equals_hbm_input": same}
 
 
@@ -355,6 +403,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--plain-repeats", type=int, default=3)
     ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--batched", type=int, default=1,
+                    help="run the batched leg (0: skip it, to profile the other legs alone)")
     ap.add_argument("--config2", type=int, default=1, help="also run the 60 s plain-call leg")
     ap.add_argument("--long", type=int, default=2, help="30-min recordings of the long leg (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -387,7 +437,8 @@ def main():
               "steps": args.steps, "warmup": args.warmup, "ms_per_step": None,
               "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
               "data": "synthetic"}
-    batched_leg(args, result)
+    if args.batched:
+        batched_leg(args, result)
     if args.config2:
         result["config2"] = config2_leg(args)
         log(f"config 2: {result['config2']['host_input']}")
