@@ -233,6 +233,8 @@ struct hv_workspace {
   int max_half = 0;
   int twiddle_n = -1;
   int wintab_h = -1;  // largest half window the window table covers
+  DevBuf winpair;     // (w, d) of integral-offset windows (k_window_pairs)
+  int winpair_h = -1; // largest half window it covers
   long long last_nnz = -1;
   long long last_frames = 0;
   bool submitted = false;       // an analysis is enqueued and not yet collected
@@ -439,12 +441,27 @@ void prepare_phase(hv_workspace* ws, Plan& p) {
   p.cfg.phase_bits = kFineBits;
 }
 
+// Whether every refinement window depends on its half length alone: at 1 ms
+// frames t fs_work is an integer when fs_work is a multiple of 1000 Hz, so
+// the Blackman phase offset is -(h + 1) at every frame (k_window_pairs).  The
+// table holds (hmax + 1)^2 entries; very low floors keep the per-frame form.
+constexpr int kWindowPairsMaxH = 1023;
+bool use_window_pairs(double fs_work, int hmax) {
+  return fs_work == std::floor(fs_work) && std::fmod(fs_work, 1000.0) == 0.0 &&
+         hmax <= kWindowPairsMaxH;
+}
+
 void prepare_twiddles(hv_workspace* ws, double fs_work, double f0_floor) {
   const int wh = refine_hmax(fs_work, f0_floor);
   if (wh > ws->wintab_h) {
     double* tab = ws->wintab.ensure<double>(2 * 65 * ((size_t)wh + 1));
     launch_window_table(ws->stream, tab, wh);
     ws->wintab_h = wh;
+  }
+  if (use_window_pairs(fs_work, wh) && wh > ws->winpair_h) {
+    double2* tab = ws->winpair.ensure<double2>(((size_t)wh + 1) * ((size_t)wh + 1));
+    launch_window_pairs(ws->stream, tab, wh);
+    ws->winpair_h = wh;
   }
   // Largest refinement window: candidates are >= f0_floor (harvest.cpp:250-252).
   const double hmax = 1.5 * fs_work / (f0_floor * (1.0 - 1e-9)) + 2.0;
@@ -480,6 +497,7 @@ void fill_config(Config& c, int count, int ratio, int fs, double fs_work, double
   c.twiddle_n = twiddle_n;
   c.lds_limit = 64 * 1024;
   c.hmax = refine_hmax(fs_work, f0_floor);
+  c.win_pairs = use_window_pairs(fs_work, c.hmax) ? 1 : 0;
   const double* co = kDecimation[ratio >= 2 && ratio <= 12 ? ratio : 0];
   c.a0 = co[0];
   c.a1 = co[1];
@@ -714,7 +732,8 @@ void enqueue_analysis(hv_workspace* ws, const Plan& p, const Bufs& b, bool event
   launch_fill_int(s, reinterpret_cast<int*>(b.work), 2 * kWorkWords, 0);
   timed(ws, 6, [&] {
     launch_refine(s, c, dutt, ws->d_f_off, p.total_frames, b.y, b.official, b.count,
-                  ws->twiddle.get<double>(), ws->wintab.get<double>(), b.rf0, b.rsc, b.work);
+                  ws->twiddle.get<double>(), ws->wintab.get<double>(),
+                  c.win_pairs ? ws->winpair.get<double2>() : nullptr, b.rf0, b.rsc, b.work);
   });
   timed(ws, 7, [&] {
     launch_prune_best(s, c, dutt, ws->d_f_off, p.total_frames, b.count, b.rf0, b.rsc, b.pf0,
@@ -758,7 +777,7 @@ std::vector<long long> graph_key(hv_workspace* ws, const Plan& p, const Bufs& b)
     k.push_back(bits);
   }
   const void* ptrs[] = {b.dx, ws->d_utt, ws->coef.p, ws->coef_off.p, ws->half.p, ws->bounds.p,
-                        ws->twiddle.p, ws->wintab.p, ws->tw4096.p, ws->band_spec.p, ws->phase.p,
+                        ws->twiddle.p, ws->wintab.p, ws->winpair.p, ws->tw4096.p, ws->band_spec.p, ws->phase.p,
                         b.y, b.pass, b.states, b.carries, b.carry_tmp, b.partial, b.mean, b.quirk,
                         b.partial3, b.sums3, b.slab, b.counts, b.offs, b.totals, b.raw, b.tspec,
                         b.count, b.official, b.rf0, b.rsc, b.pf0, b.psc, b.dbest, b.cinit, b.work,

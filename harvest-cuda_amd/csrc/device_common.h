@@ -63,6 +63,7 @@ struct Config {
   int twiddle_n;     // size of the refinement twiddle table (0: none)
   int lds_limit;     // dynamic LDS bytes a workgroup may use on this device
   int hmax;          // largest refinement half window (candidates >= f0_floor)
+  int win_pairs;     // 1: the refinement windows depend on h alone (see k_window_pairs)
   long long phase_n; // size P of the filterbank's two-level phase table (>= every fft)
   int phase_bits;    // its fine part holds 2^phase_bits entries
   double a0, a1, a2, b0, b1;  // decimation IIR

@@ -36,6 +36,9 @@ void launch_fill_int(hipStream_t s, int* p, long long n, int value);
 void launch_twiddle(hipStream_t s, double* tw, int n);
 // (cos, sin) of 2 pi m / (2h + 1) for h <= hmax, m <= 64, 65 (h + 1) entries.
 void launch_window_table(hipStream_t s, double* tab, int hmax);
+// (w, d) of the refinement windows with phase offset -(h + 1) for h <= hmax,
+// (hmax + 1)^2 entries, window h at [h^2, (h + 1)^2).
+void launch_window_pairs(hipStream_t s, double2* tab, int hmax);
 
 // Filterbank + crossing detection for bands [band0, band0 + group).
 void launch_filter_events(hipStream_t s, const Config& c, const Utt* utt,
@@ -80,8 +83,8 @@ void launch_official(hipStream_t s, const Config& c, const long long* f_off,
 void launch_refine(hipStream_t s, const Config& c, const Utt* utt,
                    const long long* f_off, long long total_frames,
                    const double* y, const double* official, const int* count,
-                   const double* twiddle, const double* wintab, double* rf0, double* rscore,
-                   unsigned long long* work);
+                   const double* twiddle, const double* wintab, const double2* winpair,
+                   double* rf0, double* rscore, unsigned long long* work);
 void launch_prune_best(hipStream_t s, const Config& c, const Utt* utt,
                        const long long* f_off, long long total_frames,
                        const int* count, const double* rf0,

@@ -523,6 +523,39 @@ __global__ __launch_bounds__(kThreads) void k_window_table(double* tab, int hmax
   tab[2 * id + 1] = sn;
 }
 
+// Refinement windows of frames whose Blackman phase offset is the integer
+// -(h + 1): GetMainWindow (harvest.cpp:446-456) evaluates
+// 0.42 + 0.5 cos phi + 0.08 cos 2 phi at phi = 2 pi ((basic + m - 1) / fs - t)
+// / (len / fs) with basic = round(t fs - h), so phi = 2 pi (m - h - 1) / len
+// whenever t fs is an integer, which holds at every 1 ms frame when fs_work is
+// a multiple of 1000 Hz.  Entry h^2 + m (m <= 2h, len = 2h + 1) holds
+// (w[m], d[m]): w from sincospi of the exact rational argument, d from w by
+// GetDiffWindow's differences (:459-468): d[0] = -w[1] / 2,
+// d[m] = -(w[m + 1] - w[m - 1]) / 2, d[len - 1] = w[len - 2] / 2.
+__device__ __forceinline__ double main_window_at(int m, int h) {
+  double c1, c2, sn;
+  sincospi(2.0 * (double)(m - h - 1) / (2 * h + 1), &sn, &c1);
+  sincospi(4.0 * (double)(m - h - 1) / (2 * h + 1), &sn, &c2);
+  return 0.42 + 0.5 * c1 + 0.08 * c2;
+}
+
+__global__ __launch_bounds__(kThreads) void k_window_pairs(double2* tab, int hmax) {
+  const long long id = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (id >= (long long)(hmax + 1) * (hmax + 1)) return;
+  int h = (int)sqrt((double)id);
+  while ((long long)h * h > id) --h;
+  while ((long long)(h + 1) * (h + 1) <= id) ++h;
+  const int m = (int)(id - (long long)h * h), last = 2 * h;
+  double w = 0.0, d = 0.0;
+  if (h > 0) {
+    w = main_window_at(m, h);
+    if (m == 0) d = -main_window_at(1, h) / 2.0;
+    else if (m == last) d = main_window_at(last - 1, h) / 2.0;
+    else d = -(main_window_at(m + 1, h) - main_window_at(m - 1, h)) / 2.0;
+  }
+  tab[id] = make_double2(w, d);
+}
+
 // Two-level phase table of size P = 2^bits_total: fine entries
 // (cos, sin)(2 pi a / P), a < 2^fine_bits, then coarse entries
 // (cos, sin)(2 pi b 2^fine_bits / P), b < P / 2^fine_bits.
@@ -1407,12 +1440,15 @@ __device__ __forceinline__ void twiddle_at(const double* tw, int stride, int n, 
 // hash size and bitmap words at compile time (the default analysis settings):
 // every LDS offset then folds into its ds_* instruction, which frees the
 // scalar registers that would hold the carved pointers.  kS = 0 takes them
-// from the arguments; both instances compute the same values.
-template <int kS, int kHS, int kW>
+// from the arguments; both instances compute the same values.  kPairs reads
+// each window's (w, d) from the k_window_pairs table (c.win_pairs), otherwise
+// they are generated per frame by rotation.
+template <int kS, int kHS, int kW, bool kPairs>
 __global__ __launch_bounds__(64 * kRefineWaves, 4) void k_refine_waves(
     Config c, const Utt* utt, const long long* f_off, long long total_frames, int hs_arg,
     int w_arg, const double* y, const double* official, const int* count, const double* twiddle,
-    const double* wintab, double* rf0, double* rscore, unsigned long long* work) {
+    const double* wintab, const double2* winpair, double* rf0, double* rscore,
+    unsigned long long* work) {
   extern __shared__ double lds_raw[];
   const int S = kS > 0 ? kS : c.slots;
   const int HS = kS > 0 ? kHS : hs_arg;
@@ -1495,6 +1531,9 @@ __global__ __launch_bounds__(64 * kRefineWaves, 4) void k_refine_waves(
   if (lane < nwin) {
     const int hw = L.win_h[lane];
     pw_basic = mround((t + (-hw) / fs) * fs + 0.001);
+  }
+  if (!kPairs && lane < nwin) {
+    const int hw = L.win_h[lane];
     const double a = (double)(pw_basic - 1) - t * fs;
     sincospi(2.0 * a / (2 * hw + 1), &pw_as, &pw_ac);
     const double2 g = wrot[65 * hw + 64];
@@ -1507,7 +1546,7 @@ __global__ __launch_bounds__(64 * kRefineWaves, 4) void k_refine_waves(
   };
   // The first 64 kYPre samples of the next window are loaded ahead into
   // registers, overlapping their latency with the current window's work.
-  // With them, e^{2 pi i lane / len} of the next window.
+  // With them, e^{2 pi i lane / len} of the next window, or its first (w, d).
   double ypre[kYPre];
   double2 tpre = make_double2(1.0, 0.0);
   auto prefetch = [&](int w) {
@@ -1516,7 +1555,8 @@ __global__ __launch_bounds__(64 * kRefineWaves, 4) void k_refine_waves(
     const int bs = window_basic(w), len = 2 * hw + 1;
 #pragma unroll
     for (int q = 0; q < kYPre; ++q) ypre[q] = lane + 64 * q < len ? y_at(bs + lane + 64 * q - 1) : 0.0;
-    tpre = wrot[65 * hw + lane];
+    if (kPairs) tpre = lane < len ? winpair[hw * hw + lane] : make_double2(0.0, 0.0);
+    else tpre = wrot[65 * hw + lane];
   };
   prefetch(0);
   wave_sync();
@@ -1603,8 +1643,10 @@ __global__ __launch_bounds__(64 * kRefineWaves, 4) void k_refine_waves(
     // Blackman argument (harvest.cpp:446-456) in units of pi:
     // 2 tmp / wl with tmp = (basic + m - 1) / fs - t and wl = len / fs.
     int basic;
-    double ac, as, gc, gs;
-    if (w < 64) {
+    double ac = 1.0, as = 0.0, gc = 1.0, gs = 0.0;
+    if (kPairs) {
+      basic = window_basic(w);
+    } else if (w < 64) {
       basic = __builtin_amdgcn_readlane(pw_basic, w);
       ac = lane_value(pw_ac, w);
       as = lane_value(pw_as, w);
@@ -1619,10 +1661,18 @@ __global__ __launch_bounds__(64 * kRefineWaves, 4) void k_refine_waves(
       gs = g.y;
     }
     // Sample m = lane: e^{2 pi i (a + m) / len} = e^{2 pi i a / len} e^{2 pi i m / len}.
-    const double wc0 = fma(ac, tpre.x, -(as * tpre.y)), ws0 = fma(as, tpre.x, ac * tpre.y);
+    const double wc0 = kPairs ? 0.0 : fma(ac, tpre.x, -(as * tpre.y));
+    const double ws0 = kPairs ? 0.0 : fma(as, tpre.x, ac * tpre.y);
     // (cos, sin) delta and sin 2 delta, delta = 2 pi / len, for the derivative window.
-    const double2 e1 = wrot[65 * hw + 1], e2 = wrot[65 * hw + 2];
-    const double dc = e1.x, ds = e1.y, dA = 0.5 * e1.y, dB = 0.16 * e2.y;
+    double dc = 0.0, ds = 0.0, dA = 0.0, dB = 0.0;
+    if (!kPairs) {
+      const double2 e1 = wrot[65 * hw + 1], e2 = wrot[65 * hw + 2];
+      dc = e1.x;
+      ds = e1.y;
+      dA = 0.5 * e1.y;
+      dB = 0.16 * e2.y;
+    }
+    const double2* wpair = winpair + hw * hw;
     const int nstage = (len + kStage - 1) / kStage;
     bool staged = false;
 
@@ -1650,10 +1700,22 @@ __global__ __launch_bounds__(64 * kRefineWaves, 4) void k_refine_waves(
         for (int c0 = 0; c0 < len; c0 += kStage) {
           const int cend = min(c0 + kStage, len);
           if (fill) {
-            // Samples s0 + lane of one 64-sample chunk; xv is y there when
-            // `have` (a prefetched chunk), else it is loaded.
-            auto chunk = [&](int s0, double xv, bool have) {
+            // Samples s0 + lane of one 64-sample chunk; xv is y there and
+            // pv its (w, d) when `have` (a prefetched chunk), else they are
+            // loaded.
+            auto chunk = [&](int s0, double xv, double2 pv, bool have) {
               const int m = s0 + lane;
+              if (kPairs) {
+                if (m < len) {
+                  if (!have) {
+                    xv = y_at(basic + m - 1);
+                    pv = wpair[m];
+                  }
+                  L.xwd[2 * (m - c0)] = xv * pv.x;
+                  L.xwd[2 * (m - c0) + 1] = xv * pv.y;
+                }
+                return;
+              }
               if (m < len) {
                 // GetMainWindow: 0.42 + 0.5 cos phi + 0.08 cos 2 phi.
                 const double w = fma(wc, fma(0.16, wc, 0.5), 0.34);
@@ -1676,10 +1738,10 @@ __global__ __launch_bounds__(64 * kRefineWaves, 4) void k_refine_waves(
               wc = nc;
               ws = ns;
             };
-#pragma unroll
-            for (int q = 0; q < kYPre; ++q)
-              if (64 * q >= c0 && 64 * q < cend) chunk(64 * q, ypre[q], true);
-            for (int s0 = max(c0, 64 * kYPre); s0 < cend; s0 += 64) chunk(s0, 0.0, false);
+            static_assert(kYPre == 1, "one prefetched chunk of (w, d) or rotations");
+            if (c0 == 0) chunk(0, ypre[0], tpre, true);
+            for (int s0 = max(c0, 64 * kYPre); s0 < cend; s0 += 64)
+              chunk(s0, 0.0, make_double2(0.0, 0.0), false);
             wsamp += (unsigned long long)(cend - c0);  // uniform: kept in SGPRs
             wave_sync();
           }
@@ -2248,6 +2310,11 @@ void launch_phase_table(hipStream_t s, double* tab, long long P, int fine_bits) 
   check_launch("phase_table");
 }
 
+void launch_window_pairs(hipStream_t s, double2* tab, int hmax) {
+  k_window_pairs<<<grid_for((long long)(hmax + 1) * (hmax + 1)), kThreads, 0, s>>>(tab, hmax);
+  check_launch("window_pairs");
+}
+
 void launch_window_table(hipStream_t s, double* tab, int hmax) {
   k_window_table<<<grid_for(65LL * (hmax + 1)), kThreads, 0, s>>>(tab, hmax);
   check_launch("window_table");
@@ -2358,8 +2425,8 @@ void launch_official(hipStream_t s, const Config& c, const long long* f_off,
 void launch_refine(hipStream_t s, const Config& c, const Utt* utt,
                    const long long* f_off, long long total_frames,
                    const double* y, const double* official, const int* count,
-                   const double* twiddle, const double* wintab, double* rf0, double* rscore,
-                   unsigned long long* work) {
+                   const double* twiddle, const double* wintab, const double2* winpair,
+                   double* rf0, double* rscore, unsigned long long* work) {
   int HS = 64;  // window hash: a power of two above the slot count (never full;
                 // reused for the nwin + 1 list starts)
   while (HS < c.slots + 1) HS <<= 1;
@@ -2371,12 +2438,17 @@ void launch_refine(hipStream_t s, const Config& c, const Utt* utt,
                              " bytes of LDS per workgroup; the device allows " +
                              std::to_string(c.lds_limit));
   const unsigned grid = (unsigned)((total_frames + kRefineWaves - 1) / kRefineWaves);
-  if (c.slots == 105 && HS == 128 && W == 9)  // 71-800 Hz at a working rate of 7350-8000 Hz
-    k_refine_waves<105, 128, 9><<<grid, 64 * kRefineWaves, bytes, s>>>(
-        c, utt, f_off, total_frames, HS, W, y, official, count, twiddle, wintab, rf0, rscore, work);
-  else
-    k_refine_waves<0, 0, 0><<<grid, 64 * kRefineWaves, bytes, s>>>(
-        c, utt, f_off, total_frames, HS, W, y, official, count, twiddle, wintab, rf0, rscore, work);
+  if (c.win_pairs && !winpair) throw std::runtime_error("refine: no window table");
+  const bool fixed = c.slots == 105 && HS == 128 && W == 9;  // 71-800 Hz at 7350-8000 Hz
+#define REFINE(S_, HS_, W_, P_)                                                          \
+  k_refine_waves<S_, HS_, W_, P_><<<grid, 64 * kRefineWaves, bytes, s>>>(               \
+      c, utt, f_off, total_frames, HS, W, y, official, count, twiddle, wintab, winpair, rf0, \
+      rscore, work)
+  if (fixed && c.win_pairs) REFINE(105, 128, 9, true);
+  else if (fixed) REFINE(105, 128, 9, false);
+  else if (c.win_pairs) REFINE(0, 0, 0, true);
+  else REFINE(0, 0, 0, false);
+#undef REFINE
   check_launch("refine");
 }
 
