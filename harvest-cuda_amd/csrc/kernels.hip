@@ -1305,8 +1305,8 @@ constexpr int kPer = 1;          // bins per lane group per pass
 constexpr int kFrameBins = 128;  // bin results of one frame kept in LDS (< 256)
 constexpr int kStage = 128;      // windowed samples staged per pass (multiple of 64)
 constexpr int kMaxWords = 128;   // bitmap words (bins 0..8191)
-constexpr int kYPre = 1;         // 64-sample chunks of the next window loaded ahead (more
-                                 // cost registers: 6 chunks ran 8 % slower)
+// The next window's first 128 samples (one stage) are loaded ahead; more
+// cost registers (6 chunks ran 8 % slower).
 
 
 // One wave's part of k_refine_waves' dynamic LDS, in carve order:
@@ -1378,6 +1378,29 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// v of the lane that DPP control kCtrl selects (a permutation within rows),
+// as two 32-bit moves: no LDS round trip.
+template <int kCtrl>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, kCtrl, 0xF, 0xF, false);
+  const unsigned hi =
+      (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), kCtrl, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+constexpr int kDppXor1 = 0xB1;        // quad_perm [1, 0, 3, 2]
+constexpr int kDppXor2 = 0x4E;        // quad_perm [2, 3, 0, 1]
+constexpr int kDppHalfMirror = 0x141; // lane 7 - i within each 8 lanes
+
+// Sum of v over each aligned group of 8 lanes, in every lane of the group:
+// pairs, then quads, then the two quads (the same sum in all 8 lanes, as
+// IEEE addition commutes).
+__device__ __forceinline__ double sum8(double v) {
+  v += dpp_f64<kDppXor1>(v);
+  v += dpp_f64<kDppXor2>(v);
+  return v + dpp_f64<kDppHalfMirror>(v);
 }
 
 // Value of v in lane `src` (uniform), without LDS traffic.
@@ -1544,19 +1567,25 @@ __global__ __launch_bounds__(64 * kRefineWaves, 4) void k_refine_waves(
     if (w < 64) return __builtin_amdgcn_readlane(pw_basic, w);
     return mround((t + (-__builtin_amdgcn_readfirstlane(L.win_h[w])) / fs) * fs + 0.001);
   };
-  // The first 64 kYPre samples of the next window are loaded ahead into
-  // registers, overlapping their latency with the current window's work.
-  // With them, e^{2 pi i lane / len} of the next window, or its first (w, d).
-  double ypre[kYPre];
-  double2 tpre = make_double2(1.0, 0.0);
+  // The next window's samples lane and lane + 64 are loaded ahead into
+  // registers (y0, y1), overlapping their latency with the current window's
+  // work, and with them their (w, d) (t0, t1), or for the rotation form
+  // e^{2 pi i lane / len} (t0).
+  static_assert(kStage == 128, "one stage is two 64-sample chunks");
+  double y0 = 0.0, y1 = 0.0;
+  double2 t0 = make_double2(1.0, 0.0), t1 = make_double2(0.0, 0.0);
   auto prefetch = [&](int w) {
     if (w >= nwin) return;
     const int hw = __builtin_amdgcn_readfirstlane(L.win_h[w]);
     const int bs = window_basic(w), len = 2 * hw + 1;
-#pragma unroll
-    for (int q = 0; q < kYPre; ++q) ypre[q] = lane + 64 * q < len ? y_at(bs + lane + 64 * q - 1) : 0.0;
-    if (kPairs) tpre = lane < len ? winpair[hw * hw + lane] : make_double2(0.0, 0.0);
-    else tpre = wrot[65 * hw + lane];
+    y0 = lane < len ? y_at(bs + lane - 1) : 0.0;
+    y1 = lane + 64 < len ? y_at(bs + lane + 63) : 0.0;
+    if (kPairs) {
+      t0 = lane < len ? winpair[hw * hw + lane] : make_double2(0.0, 0.0);
+      t1 = lane + 64 < len ? winpair[hw * hw + lane + 64] : make_double2(0.0, 0.0);
+    } else {
+      t0 = wrot[65 * hw + lane];
+    }
   };
   prefetch(0);
   wave_sync();
@@ -1661,8 +1690,8 @@ __global__ __launch_bounds__(64 * kRefineWaves, 4) void k_refine_waves(
       gs = g.y;
     }
     // Sample m = lane: e^{2 pi i (a + m) / len} = e^{2 pi i a / len} e^{2 pi i m / len}.
-    const double wc0 = kPairs ? 0.0 : fma(ac, tpre.x, -(as * tpre.y));
-    const double ws0 = kPairs ? 0.0 : fma(as, tpre.x, ac * tpre.y);
+    const double wc0 = kPairs ? 0.0 : fma(ac, t0.x, -(as * t0.y));
+    const double ws0 = kPairs ? 0.0 : fma(as, t0.x, ac * t0.y);
     // (cos, sin) delta and sin 2 delta, delta = 2 pi / len, for the derivative window.
     double dc = 0.0, ds = 0.0, dA = 0.0, dB = 0.0;
     if (!kPairs) {
@@ -1699,23 +1728,33 @@ __global__ __launch_bounds__(64 * kRefineWaves, 4) void k_refine_waves(
         double wc = wc0, ws = ws0;  // (cos, sin) phi_m of this lane's sample m
         for (int c0 = 0; c0 < len; c0 += kStage) {
           const int cend = min(c0 + kStage, len);
-          if (fill) {
-            // Samples s0 + lane of one 64-sample chunk; xv is y there and
-            // pv its (w, d) when `have` (a prefetched chunk), else they are
-            // loaded.
-            auto chunk = [&](int s0, double xv, double2 pv, bool have) {
+          if (kPairs && fill) {
+            // The stage's samples m0 = c0 + lane and m1 = m0 + 64: prefetched
+            // for the first stage, else both loaded before either is stored.
+            const int m0 = c0 + lane, m1 = m0 + 64;
+            double x0 = y0, x1 = y1;
+            double2 v0 = t0, v1 = t1;
+            if (c0 > 0) {
+              x0 = m0 < len ? y_at(basic + m0 - 1) : 0.0;
+              x1 = m1 < len ? y_at(basic + m1 - 1) : 0.0;
+              v0 = m0 < len ? wpair[m0] : make_double2(0.0, 0.0);
+              v1 = m1 < len ? wpair[m1] : make_double2(0.0, 0.0);
+            }
+            if (m0 < len) {
+              L.xwd[2 * lane] = x0 * v0.x;
+              L.xwd[2 * lane + 1] = x0 * v0.y;
+            }
+            if (m1 < len) {
+              L.xwd[2 * lane + 128] = x1 * v1.x;
+              L.xwd[2 * lane + 129] = x1 * v1.y;
+            }
+            wsamp += (unsigned long long)(cend - c0);  // uniform: kept in SGPRs
+            wave_sync();
+          } else if (fill) {
+            // Samples s0 + lane of one 64-sample chunk; xv is y there when
+            // `have` (a prefetched chunk), else it is loaded.
+            auto chunk = [&](int s0, double xv, bool have) {
               const int m = s0 + lane;
-              if (kPairs) {
-                if (m < len) {
-                  if (!have) {
-                    xv = y_at(basic + m - 1);
-                    pv = wpair[m];
-                  }
-                  L.xwd[2 * (m - c0)] = xv * pv.x;
-                  L.xwd[2 * (m - c0) + 1] = xv * pv.y;
-                }
-                return;
-              }
               if (m < len) {
                 // GetMainWindow: 0.42 + 0.5 cos phi + 0.08 cos 2 phi.
                 const double w = fma(wc, fma(0.16, wc, 0.5), 0.34);
@@ -1738,10 +1777,8 @@ __global__ __launch_bounds__(64 * kRefineWaves, 4) void k_refine_waves(
               wc = nc;
               ws = ns;
             };
-            static_assert(kYPre == 1, "one prefetched chunk of (w, d) or rotations");
-            if (c0 == 0) chunk(0, ypre[0], tpre, true);
-            for (int s0 = max(c0, 64 * kYPre); s0 < cend; s0 += 64)
-              chunk(s0, 0.0, make_double2(0.0, 0.0), false);
+            if (c0 == 0) chunk(0, y0, true);
+            for (int s0 = max(c0, 64); s0 < cend; s0 += 64) chunk(s0, 0.0, false);
             wsamp += (unsigned long long)(cend - c0);  // uniform: kept in SGPRs
             wave_sync();
           }
@@ -1796,13 +1833,11 @@ __global__ __launch_bounds__(64 * kRefineWaves, 4) void k_refine_waves(
           const double zr = q1[qq] - gc8[qq] * q2[qq], zi = gs8[qq] * q2[qq];
           double mre = pc[qq] * yr + ps[qq] * yi, mim = pc[qq] * yi - ps[qq] * yr;
           double dre = pc[qq] * zr + ps[qq] * zi, dim = pc[qq] * zi - ps[qq] * zr;
-#pragma unroll
-          for (int dd = kSplit / 2; dd > 0; dd >>= 1) {
-            mre += __shfl_xor(mre, dd, kSplit);
-            mim += __shfl_xor(mim, dd, kSplit);
-            dre += __shfl_xor(dre, dd, kSplit);
-            dim += __shfl_xor(dim, dd, kSplit);
-          }
+          static_assert(kSplit == 8, "sum8 reduces groups of 8 lanes");
+          mre = sum8(mre);
+          mim = sum8(mim);
+          dre = sum8(dre);
+          dim = sum8(dim);
           const int j = jb + grp + 8 * qq;
           if (sub == 0 && act[qq]) {
             L.res[2 * (j0 + j)] = mre * mre + mim * mim;
