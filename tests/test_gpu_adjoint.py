@@ -2,9 +2,9 @@
 
 Config 4 (32 x 4 s at 24 kHz, config-2 content, seed 1004): loss = mean over
 forward-voiced frames of (ln f0 - ln target)^2, target = f0 exp(N(0, 0.02^2));
-central finite differences with h = 1e-6 at 64 seeded sample indices of the
-frozen-decision forward must match the hand-written adjoint within 1e-6
-relative.  A torch float64 CPU autograd re-expression of the same frozen map
+Richardson-extrapolated central finite differences of the frozen-decision
+forward at 96 sample indices must match the hand-written adjoint within 1e-6
+relative wherever the difference resolves it.  A torch float64 CPU autograd re-expression of the same frozen map
 (tests/reexpress.py) is the second check."""
 import numpy as np
 import pytest
@@ -53,15 +53,21 @@ def test_frozen_forward_reproduces_forward(gpu):
 
 
 def test_config4_finite_differences_match_adjoint(gpu):
-    """64 seeded sample indices, h = 1e-6: |fd - adj| <= 1e-6 |adj| + 1e-7 max|grad|.
+    """Per sample index, against Richardson-extrapolated central differences
+    D = (4 D(h) - D(2h)) / 3 with h = 2e-4 (truncation O(h^4), about 1e-15
+    relative at these amplitudes): within 1e-6 relative at every index whose
+    |adj| is at least 1e-3 max|grad|, and within 1e-9 max|grad| at the
+    others, over 64 seeded indices, the 16 of largest |adj| and 16 seeded
+indices among those at or above 1e-3 max|grad|.
 
-    The absolute term is the floor of FP64 central differences here: moving
-    one sample by 1e-6 changes the rounding of every decimated sample (four
-    IIR passes amplify ulp-level differences), which moves the loss by about
-    5e-19, i.e. ~5e-13 = 6e-8 max|grad| in the quotient; a whole-utterance DC
-    shift, whose exact effect is zero, shows the same floor.  Directional
-    differences along random vectors stand far above it and are held to
-    1e-6 relative alone."""
+    The floor is that of FP64 differences of this loss: moving one sample
+    changes the rounding of every decimated sample (four IIR passes amplify
+    ulp-level differences), which moves the loss by about 5e-19, i.e. about
+    2e-15 = 3e-10 max|grad| in D at this h.  A smaller h raises it as 1 / h
+    (h = 1e-6 gives 6e-8 max|grad|); a larger one lets the O(h^4) term grow.
+    Directional differences along random unit-variance vectors (plain central
+    differences, h = 1e-6: the vector moves every sample) stand far above
+    the floor and are held to 1e-6 relative alone."""
     xs, fs = config4()
     res, rec = gpu._harvest_record(xs, fs)
     f0s = [r[0] for r in res]
@@ -70,27 +76,40 @@ def test_config4_finite_differences_match_adjoint(gpu):
     nv = sum(int((t > 0).sum()) for t in targets)
     gx = gpu._harvest_adjoint(rec, xs, g)
     rng = np.random.default_rng(np.random.SeedSequence(1004).spawn(34)[33])
-    h = 1e-6
+    h = 2e-4
     scale = max(np.abs(a).max() for a in gx)
 
-    def fd_along(dirs):
-        fp = gpu._harvest_frozen(rec, [np.ascontiguousarray(x + h * d) for x, d in zip(xs, dirs)])
-        fm = gpu._harvest_frozen(rec, [np.ascontiguousarray(x - h * d) for x, d in zip(xs, dirs)])
-        return sum(loss_difference(a, b, t, nv) for a, b, t in zip(fp, fm, targets)) / (2 * h)
+    def central(dirs, step):
+        fp = gpu._harvest_frozen(rec, [np.ascontiguousarray(x + step * d) for x, d in zip(xs, dirs)])
+        fm = gpu._harvest_frozen(rec, [np.ascontiguousarray(x - step * d) for x, d in zip(xs, dirs)])
+        return sum(loss_difference(a, b, t, nv) for a, b, t in zip(fp, fm, targets)) / (2 * step)
 
-    checked = 0
-    for _ in range(64):
-        u = int(rng.integers(len(xs)))
-        k = int(rng.integers(len(xs[u])))
+    def fd_along(dirs):
+        return (4.0 * central(dirs, h) - central(dirs, 2 * h)) / 3.0
+
+    picks = [(int(u), int(rng.integers(len(xs[u])))) for u in rng.integers(len(xs), size=64)]
+    flat = np.concatenate([np.abs(a) for a in gx])
+    ends = np.cumsum([len(a) for a in gx])
+    big = np.flatnonzero(flat >= 1e-3 * scale)
+    for i in [*np.argsort(flat)[-16:], *rng.choice(big, size=min(16, len(big)), replace=False)]:
+        u = int(np.searchsorted(ends, i, side="right"))
+        picks.append((u, int(i - (ends[u - 1] if u else 0))))
+    resolved, worst = 0, 0.0
+    for u, k in picks:
         dirs = [np.zeros_like(x) for x in xs]
         dirs[u][k] = 1.0
         fd, adj = fd_along(dirs), float(gx[u][k])
-        assert abs(fd - adj) <= 1e-6 * abs(adj) + 1e-7 * scale, (u, k, fd, adj)
-        checked += abs(adj) > 1e-5 * scale
-    assert checked >= 16  # indices far above the floor, not only near-zero ones
-    for _ in range(4):
+        if abs(adj) >= 1e-3 * scale:
+            resolved += 1
+            worst = max(worst, abs(fd - adj) / abs(adj))
+            assert abs(fd - adj) <= 1e-6 * abs(adj), (u, k, fd, adj)
+        else:
+            assert abs(fd - adj) <= 1e-9 * scale, (u, k, fd, adj, scale)
+    assert resolved >= 32, resolved  # the relative bar covers many indices, not only the top 16
+    print(f"per-index: {resolved} of {len(picks)} held to 1e-6 relative, worst {worst:.2e}")
+    for _ in range(4):  # a unit-variance direction over every sample: a far smaller step
         dirs = [rng.normal(size=len(x)) for x in xs]
-        fd = fd_along(dirs)
+        fd = central(dirs, 1e-6)
         adj = sum(float(np.dot(a, d)) for a, d in zip(gx, dirs))
         assert abs(fd - adj) <= 1e-6 * abs(adj), (fd, adj)
 
