@@ -40,7 +40,7 @@ def _run(cmd):
 def build_backend() -> Path:
     out = PKG / "libharvest_hip.so"
     objs = []
-    srcs = ["kernels.hip", "capi.hip"]
+    srcs = ["kernels.hip", "refine_table.hip", "capi.hip"]
     tmp = ROOT / "_build"
     tmp.mkdir(exist_ok=True)
 

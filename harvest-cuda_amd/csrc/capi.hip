@@ -235,6 +235,12 @@ struct hv_workspace {
   int wintab_h = -1;  // largest half window the window table covers
   DevBuf winpair;     // (w, d) of integral-offset windows (k_window_pairs)
   int winpair_h = -1; // largest half window it covers
+  // Bin tables of the refinement (refine_table.hip) for the key below, and
+  // the per-analysis buffers of that path.
+  DevBuf bt_info, bt_harm, bt_kq, bt_win, bt_bins;
+  int bt_hlo = 0, bt_hr = 0, bt_nbmax = 0;
+  double bt_key_fs = -1.0, bt_key_floor = -1.0, bt_key_ceil = -1.0;
+  DevBuf bt_needs, bt_mask, bt_cnt, bt_off, bt_scan, bt_list, bt_cinfo, bt_sched, bt_res;
   long long last_nnz = -1;
   long long last_frames = 0;
   bool submitted = false;       // an analysis is enqueued and not yet collected
@@ -250,7 +256,7 @@ struct hv_workspace {
   // copies them at collect).
   double cand_per_frame = 20.0;
   long long cand_capacity = 0;  // of the last analysis
-  hv_tuning tune{kFftMinHalf, 0, (int)(kSlabBudget >> 20)};
+  hv_tuning tune{kFftMinHalf, 0, (int)(kSlabBudget >> 20), 1};
   // hv_order_after: an event on another workspace's stream that the next
   // submitted analysis' device work waits for (after its uploads).
   hipEvent_t after_ev = nullptr;
@@ -482,6 +488,116 @@ void prepare_twiddles(hv_workspace* ws, double fs_work, double f0_floor) {
   ws->twiddle_n = need;
 }
 
+// Half window of a positive candidate (harvest.cpp:596), as the kernels
+// evaluate it.
+int half_window_of(double fs_work, double f0) { return (int)(1.5 * fs_work / f0 + 1.0); }
+
+// Smallest double in [lo, hi] (both positive) where `pred` holds, for a
+// predicate false below some point and true from it on; hi when it holds
+// nowhere below hi.  Positive doubles order like their bit patterns.
+template <class P> double first_double(double lo, double hi, P pred) {
+  uint64_t a, b;
+  std::memcpy(&a, &lo, sizeof a);
+  std::memcpy(&b, &hi, sizeof b);
+  while (a < b) {
+    const uint64_t mid = a + (b - a) / 2;
+    double m;
+    std::memcpy(&m, &mid, sizeof m);
+    if (pred(m)) b = mid;
+    else a = mid + 1;
+  }
+  double r;
+  std::memcpy(&r, &a, sizeof r);
+  return r;
+}
+
+// Bin tables of the refinement for one (fs_work, floor, ceil), cached in
+// the workspace (ws->bt_hr = 0: the per-frame refinement serves the key).
+// Candidates lie in [f0_floor, f0_ceil] up to rounding (means of values the
+// band gate keeps, harvest.cpp:250-252), so their half windows span
+// [half(ceil), refine_hmax].  The candidates of one half window h form the
+// doubles interval [flo, fhi] where half_window_of is h (it is monotone in
+// f0), and FixF0's bin of harmonic j, round(f0 n / fs j) (harvest.cpp:515),
+// is monotone in f0 too: the bins of h and j are exactly
+// [round(flo n / fs j), round(fhi n / fs j)], for j up to
+// min(fs / 2 / flo, 6) (harvest.cpp:577-578).  The kernels check every
+// slot against these ranges all the same.
+void prepare_bintable(hv_workspace* ws, double fs_work, double f0_floor, double f0_ceil) {
+  if (ws->bt_key_fs == fs_work && ws->bt_key_floor == f0_floor && ws->bt_key_ceil == f0_ceil)
+    return;
+  ws->bt_hr = 0;
+  ws->bt_key_fs = fs_work;
+  ws->bt_key_floor = f0_floor;
+  ws->bt_key_ceil = f0_ceil;
+  const int hhi = refine_hmax(fs_work, f0_floor);
+  if (!use_window_pairs(fs_work, hhi)) return;
+  const int hlo = half_window_of(fs_work, f0_ceil * (1.0 + 1e-9));
+  const int hr = hhi - hlo + 1;
+  if (hlo < 1 || hr > kBtMaxHr) return;
+  const double lo = f0_floor * 0.5, hi = f0_ceil * 2.0;  // beyond every candidate
+  std::vector<int4> info(hr), harm(6 * (size_t)hr);
+  std::vector<std::vector<int>> bins(hr);
+  long long rows = 0, records = 0;
+  int nbmax = 1;
+  for (int hb = 0; hb < hr; ++hb) {
+    const int h = hlo + hb, n = window_fft(h);
+    const double flo = first_double(lo, hi, [&](double f) { return half_window_of(fs_work, f) <= h; });
+    const double fnext = first_double(lo, hi, [&](double f) { return half_window_of(fs_work, f) < h; });
+    const double fhi = half_window_of(fs_work, hi) < h ? std::nextafter(fnext, 0.0) : hi;
+    const bool empty = half_window_of(fs_work, flo) != h || fhi < flo;
+    const int H = empty ? 0 : std::min((int)(fs_work / 2.0 / flo), 6);
+    for (int j = 0; j < 6; ++j) {
+      int4 e{0, 0, (int)bins[hb].size(), 0};
+      if (j < H) {
+        const int klo = mround(flo * n / fs_work * (j + 1));
+        const int khi = mround(fhi * n / fs_work * (j + 1));
+        e.x = klo;
+        e.y = khi - klo + 1;
+        for (int k = klo; k <= khi; ++k) bins[hb].push_back(k);
+      }
+      harm[6 * hb + j] = e;
+    }
+    const int nb = (int)bins[hb].size();
+    info[hb] = int4{n, nb, (int)rows, (int)records};
+    rows += 2 * h + 1;
+    records += nb;
+    nbmax = std::max(nbmax, nb);
+  }
+  std::vector<int> kq((size_t)hr * nbmax, 0);
+  for (int hb = 0; hb < hr; ++hb)
+    for (size_t q = 0; q < bins[hb].size(); ++q) kq[(size_t)hb * nbmax + q] = bins[hb][q];
+  int4* dinfo = ws->bt_info.ensure<int4>(hr);
+  int4* dharm = ws->bt_harm.ensure<int4>(6 * (size_t)hr);
+  int* dkq = ws->bt_kq.ensure<int>(kq.size());
+  double2* dwin = ws->bt_win.ensure<double2>((size_t)rows);
+  double* dbins = ws->bt_bins.ensure<double>((size_t)std::max(1LL, records) * kBtRecord);
+  check(hipMemcpyAsync(dinfo, info.data(), info.size() * sizeof(int4), hipMemcpyHostToDevice,
+                       ws->stream), "upload bin tables");
+  check(hipMemcpyAsync(dharm, harm.data(), harm.size() * sizeof(int4), hipMemcpyHostToDevice,
+                       ws->stream), "upload bin tables");
+  check(hipMemcpyAsync(dkq, kq.data(), kq.size() * sizeof(int), hipMemcpyHostToDevice,
+                       ws->stream), "upload bin tables");
+  Config c{};
+  c.bt_hlo = hlo;
+  c.bt_hr = hr;
+  c.bt_nbmax = nbmax;
+  launch_bt_tables(ws->stream, c, BinTable{dinfo, dharm, dwin, dbins}, dkq, dwin, dbins);
+  check(hipStreamSynchronize(ws->stream), "bin tables");
+  ws->bt_hlo = hlo;
+  ws->bt_hr = hr;
+  ws->bt_nbmax = nbmax;
+}
+
+// The refinement path of an analysis: the bin tables when the workspace's
+// tuning allows them and they exist for the analysis' key.
+void apply_bintable(hv_workspace* ws, Config& c) {
+  const bool on = ws->tune.refine_table && ws->bt_hr > 0 && ws->bt_key_fs == c.fs_work &&
+                  ws->bt_key_floor == c.f0_floor && ws->bt_key_ceil == c.f0_ceil;
+  c.bt_hlo = on ? ws->bt_hlo : 0;
+  c.bt_hr = on ? ws->bt_hr : 0;
+  c.bt_nbmax = on ? ws->bt_nbmax : 0;
+}
+
 void fill_config(Config& c, int count, int ratio, int fs, double fs_work, double f0_floor,
                  double f0_ceil, int bands, int capacity, int twiddle_n) {
   c.count = count;
@@ -498,6 +614,7 @@ void fill_config(Config& c, int count, int ratio, int fs, double fs_work, double
   c.lds_limit = 64 * 1024;
   c.hmax = refine_hmax(fs_work, f0_floor);
   c.win_pairs = use_window_pairs(fs_work, c.hmax) ? 1 : 0;
+  c.bt_hlo = c.bt_hr = c.bt_nbmax = 0;  // apply_bintable turns the bin tables on
   const double* co = kDecimation[ratio >= 2 && ratio <= 12 ? ratio : 0];
   c.a0 = co[0];
   c.a1 = co[1];
@@ -608,12 +725,15 @@ struct Bufs {
   unsigned long long* hwork;
   long long capacity;  // candidates the sequence copies into h_cand (0: none)
   double* hcand;       // h_cand when capacity > 0
+  BtBufs bt;           // bin-table refinement (when cfg.bt_hr > 0)
 };
 
 // HV_STAT_SLOTS work counters, then a flag the refinement sets if a
 // candidate's window exceeds the bound its LDS was sized for; each spread
 // over kWorkSpread addresses.
 constexpr int kWorkWords = (HV_STAT_SLOTS + 1) * kWorkSpread;
+static_assert(kStatFault == HV_STAT_SLOTS && kStatTablePairs < HV_STAT_SLOTS,
+              "device counter layout must match HV_STAT_SLOTS");
 
 Bufs size_buffers(hv_workspace* ws, const Plan& p, const double* dx) {
   const Config& c = p.cfg;
@@ -660,6 +780,20 @@ Bufs size_buffers(hv_workspace* ws, const Plan& p, const double* dx) {
   b.hwork = ws->h_work.ensure<unsigned long long>(kWorkWords);
   b.capacity = 0;
   b.hcand = nullptr;
+  b.bt = BtBufs{};
+  if (c.bt_hr > 0) {
+    const long long nblk = bt_blocks(p.total_frames), cells = (long long)c.bt_hr * nblk;
+    b.bt.needs = ws->bt_needs.ensure<unsigned long long>((size_t)p.total_frames * kBtWords);
+    b.bt.mask = ws->bt_mask.ensure<unsigned long long>(cells);
+    b.bt.cnt = ws->bt_cnt.ensure<int>(cells);
+    b.bt.off = ws->bt_off.ensure<int>(cells + 1);
+    b.bt.scan_len = scan_block_count(cells) + 1;
+    b.bt.scan_tmp = ws->bt_scan.ensure<int>(b.bt.scan_len);
+    b.bt.list = ws->bt_list.ensure<int2>((size_t)p.total_frames * kBtPairs);
+    b.bt.cinfo = ws->bt_cinfo.ensure<int>(3 * kBtMaxHr + 1);
+    b.bt.sched = ws->bt_sched.ensure<int>(1);
+    b.bt.res = ws->bt_res.ensure<double2>((size_t)p.total_frames * kBtPairs * c.bt_nbmax);
+  }
   return b;
 }
 
@@ -731,9 +865,15 @@ void enqueue_analysis(hv_workspace* ws, const Plan& p, const Bufs& b, bool event
   if (events) check(hipEventRecord(ws->ev[2], s), "hipEventRecord");
   launch_fill_int(s, reinterpret_cast<int*>(b.work), 2 * kWorkWords, 0);
   timed(ws, 6, [&] {
-    launch_refine(s, c, dutt, ws->d_f_off, p.total_frames, b.y, b.official, b.count,
-                  ws->twiddle.get<double>(), ws->wintab.get<double>(),
-                  c.win_pairs ? ws->winpair.get<double2>() : nullptr, b.rf0, b.rsc, b.work);
+    if (c.bt_hr > 0)
+      launch_refine_table(s, c, dutt, ws->d_f_off, p.total_frames, b.y, b.official, b.count,
+                          BinTable{ws->bt_info.get<int4>(), ws->bt_harm.get<int4>(),
+                                   ws->bt_win.get<double2>(), ws->bt_bins.get<double>()},
+                          b.bt, b.rf0, b.rsc, b.work);
+    else
+      launch_refine(s, c, dutt, ws->d_f_off, p.total_frames, b.y, b.official, b.count,
+                    ws->twiddle.get<double>(), ws->wintab.get<double>(),
+                    c.win_pairs ? ws->winpair.get<double2>() : nullptr, b.rf0, b.rsc, b.work);
   });
   timed(ws, 7, [&] {
     launch_prune_best(s, c, dutt, ws->d_f_off, p.total_frames, b.count, b.rf0, b.rsc, b.pf0,
@@ -769,6 +909,7 @@ std::vector<long long> graph_key(hv_workspace* ws, const Plan& p, const Bufs& b)
   const Config& c = p.cfg;
   std::vector<long long> k = {c.count, c.ratio, c.fs, c.bands, c.capacity, c.twiddle_n,
                               c.hmax, (long long)c.phase_n, ws->fft_bands, ws->fft_D, b.group,
+                              c.bt_hlo, c.bt_hr, c.bt_nbmax,
                               p.total_x, p.total_m, p.total_y, p.total_frames, p.total_tiles,
                               p.total_chunks, p.total_pblocks, p.max_chunks};
   long long bits;
@@ -782,7 +923,10 @@ std::vector<long long> graph_key(hv_workspace* ws, const Plan& p, const Bufs& b)
                         b.partial3, b.sums3, b.slab, b.counts, b.offs, b.totals, b.raw, b.tspec,
                         b.count, b.official, b.rf0, b.rsc, b.pf0, b.psc, b.dbest, b.cinit, b.work,
                         b.row_nnz, b.doffsets, b.scan_tmp, b.hoff, b.hbest, b.hwork,
-                        b.hcand};
+                        b.hcand, ws->bt_info.p, ws->bt_harm.p, ws->bt_win.p, ws->bt_bins.p,
+                        b.bt.needs,
+                        b.bt.mask, b.bt.cnt, b.bt.off, b.bt.scan_tmp, b.bt.list, b.bt.cinfo,
+                        b.bt.sched, b.bt.res};
   for (const void* q : ptrs) k.push_back((long long)(uintptr_t)q);
   k.push_back(b.capacity);
   return k;
@@ -983,10 +1127,12 @@ Plan stage_batch(hv_workspace* ws, int count, const double* const* xs, const int
   }
   prepare_filters(ws, fs_work, f0_floor, f0_ceil, nullptr, 0);
   prepare_twiddles(ws, fs_work, f0_floor);
+  prepare_bintable(ws, fs_work, f0_floor, f0_ceil);
   Plan p;
   const int B = (int)ws->boundaries.size();
   fill_config(p.cfg, count, ratio, fs, fs_work, f0_floor, f0_ceil, B, host_round(B / 10.0),
               ws->twiddle_n);
+  apply_bintable(ws, p.cfg);
   p.cfg.lds_limit = ws->lds_limit;
   layout(p, lens, ns, ffts, frames);
   prepare_phase(ws, p);
@@ -1303,9 +1449,11 @@ hv_status hv_analyze(hv_workspace* ws, const double* y, int64_t y_length, int ff
     DeviceGuard guard(ws->device);
     prepare_filters(ws, fs_work, f0_floor, f0_ceil, boundaries, bands);
     prepare_twiddles(ws, fs_work, f0_floor);
+    prepare_bintable(ws, fs_work, f0_floor, f0_ceil);
     Plan p;
     fill_config(p.cfg, 1, den, (int)(fs_work * den), fs_work, f0_floor, f0_ceil, bands, capacity,
                 ws->twiddle_n);
+    apply_bintable(ws, p.cfg);
     p.cfg.lds_limit = ws->lds_limit;
     p.cfg.lag = 0;
     layout(p, {y_length}, {(int)y_length}, {fft_size}, {frames});
@@ -1449,6 +1597,7 @@ hv_status hv_set_tuning(hv_workspace* ws, const hv_tuning* in) {
     if (in->fft_min_half < 1) invalid("fft_min_half must be positive");
     if (in->graphs != 0 && in->graphs != 1) invalid("graphs must be 0 or 1");
     if (in->slab_budget_mib < 64) invalid("slab_budget_mib must be at least 64");
+    if (in->refine_table != 0 && in->refine_table != 1) invalid("refine_table must be 0 or 1");
     ws->tune = *in;  // a new band split takes effect at the next call (filter cache key)
   });
 }

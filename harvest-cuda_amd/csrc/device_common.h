@@ -25,7 +25,12 @@ constexpr int kWindowMargin = 4;     // crossings kept either side of a tile
 constexpr int kInterpFramesPerThread = 2;
 // Work counters are spread over this many addresses (by workgroup) so that
 // the per-wave atomics of a launch do not queue on one address; the host sums.
+// Counter k (the host's HV_STAT_SLOTS counters) occupies words
+// [k kWorkSpread, (k + 1) kWorkSpread); after them, the refinement's flag for
+// a window or bin beyond its precomputed bounds.
 constexpr int kWorkSpread = 64;
+constexpr int kStatWindowSamples = 0, kStatBinSamples = 1, kStatBins = 2, kStatTablePairs = 3;
+constexpr int kStatFault = 4;
 
 // Per-utterance layout of a batch.  All offsets index the concatenated
 // device arrays of the batch.
@@ -64,13 +69,49 @@ struct Config {
   int lds_limit;     // dynamic LDS bytes a workgroup may use on this device
   int hmax;          // largest refinement half window (candidates >= f0_floor)
   int win_pairs;     // 1: the refinement windows depend on h alone (see k_window_pairs)
+  // Bin-table refinement (refine_table.hip), on when bt_hr > 0: half windows
+  // [bt_hlo, bt_hlo + bt_hr), at most bt_nbmax bins per half window.
+  int bt_hlo, bt_hr, bt_nbmax;
   long long phase_n; // size P of the filterbank's two-level phase table (>= every fft)
   int phase_bits;    // its fine part holds 2^phase_bits entries
   double a0, a1, a2, b0, b1;  // decimation IIR
   double chunk_pow[9];        // A^kChunk of the IIR state transition
 };
 
-__device__ __forceinline__ int mround(double x) {
+// FFT size of GetMeanF0 for half window h: 2^(2 + floor(log2(2h + 1)))
+// (harvest.cpp:598-599; 2h + 1 is odd, so log / kLog2 there is never within
+// rounding of an integer).  window_fft_log2 is its exponent.
+__host__ __device__ inline int window_fft_log2(int h) {
+  return 2 + (31 - __builtin_clz((unsigned)(2 * h + 1)));
+}
+__host__ __device__ inline int window_fft(int h) { return 1 << window_fft_log2(h); }
+
+// GetMainWindow (harvest.cpp:446-456) evaluates 0.42 + 0.5 cos phi +
+// 0.08 cos 2 phi at phi = 2 pi ((basic + m - 1) / fs - t) / (len / fs) with
+// basic = round(t fs - h), so phi = 2 pi (m - h - 1) / len whenever t fs is an
+// integer, which holds at every 1 ms frame when fs_work is a multiple of
+// 1000 Hz.  Sample m of such a window, from sincospi of the exact rational
+// argument.
+__device__ __forceinline__ double main_window_at(int m, int h) {
+  double c1, c2, sn;
+  sincospi(2.0 * (double)(m - h - 1) / (2 * h + 1), &sn, &c1);
+  sincospi(4.0 * (double)(m - h - 1) / (2 * h + 1), &sn, &c2);
+  return 0.42 + 0.5 * c1 + 0.08 * c2;
+}
+
+// (w[m], d[m]) of the window of half length h > 0 above: d by GetDiffWindow's
+// differences (harvest.cpp:459-468): d[0] = -w[1] / 2,
+// d[m] = -(w[m + 1] - w[m - 1]) / 2, d[len - 1] = w[len - 2] / 2.
+__device__ __forceinline__ double2 window_pair(int m, int h) {
+  const int last = 2 * h;
+  double d;
+  if (m == 0) d = -main_window_at(1, h) / 2.0;
+  else if (m == last) d = main_window_at(last - 1, h) / 2.0;
+  else d = -(main_window_at(m + 1, h) - main_window_at(m - 1, h)) / 2.0;
+  return make_double2(main_window_at(m, h), d);
+}
+
+__host__ __device__ inline int mround(double x) {
   return x > 0 ? (int)(x + 0.5) : (int)(x - 0.5);
 }
 

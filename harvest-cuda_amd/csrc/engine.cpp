@@ -770,6 +770,7 @@ void record_stats(hv_workspace* ws, Timings& t) {
   t["work_window_samples"] = (double)st[0];
   t["work_bin_samples"] = (double)st[1];
   t["work_bins"] = (double)st[2];
+  t["work_table_pairs"] = (double)st[3];
 }
 
 double ms_since(Clock::time_point& mark) {

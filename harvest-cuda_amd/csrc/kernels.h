@@ -85,6 +85,51 @@ void launch_refine(hipStream_t s, const Config& c, const Utt* utt,
                    const double* y, const double* official, const int* count,
                    const double* twiddle, const double* wintab, const double2* winpair,
                    double* rf0, double* rscore, unsigned long long* work);
+// Bin-table refinement (refine_table.hip): analyses whose frames start at
+// whole samples (Config::win_pairs) and whose half windows span at most
+// kBtMaxHr values.
+constexpr int kBtPassBins = 8;  // bins per pass over a window (at most)
+constexpr int kBtWords = 4;     // 64-bit words of a frame's half-window bitmask
+constexpr int kBtMaxHr = kBtWords * 64;
+constexpr int kBtPairs = 32;    // (frame, h) pairs per frame with stored bin results
+constexpr int kBtBlock = 8;     // window samples staged per lane at a time
+constexpr int kBtRecord = 8;    // doubles per bin record
+constexpr long long kBtDftGroups = 1536;  // persistent grid of k_bt_dft (at most)
+// Tables of half windows h = bt_hlo + hb, hb < bt_hr:
+//   info[hb]     (FFT size n, bins NB, first window row, first bin record)
+//   harm[6 hb + j] (first bin, bin count (0: harmonic unused), first result
+//                index) of harmonic j + 1
+//   win          per h, 2h + 1 rows (w[m], d[m])
+//   bins         per bin k of h, a record (2 cos w, cos w, sin w, cos w(2h),
+//                sin w(2h), 0, 0, 0) of w = 2 pi k / n
+struct BinTable {
+  const int4* info;
+  const int4* harm;
+  const double2* win;
+  const double* bins;
+};
+// Per-analysis buffers: needs[frames][kBtWords], mask and cnt [bt_hr][nblk]
+// (nblk = bt_blocks(frames)), off[bt_hr nblk + 1] with its scan scratch,
+// list[frames kBtPairs], cinfo[3 bt_hr + 1], sched[1],
+// res[frames kBtPairs bt_nbmax].
+struct BtBufs {
+  unsigned long long *needs, *mask;
+  int *cnt, *off, *scan_tmp;
+  long long scan_len;
+  int2* list;
+  int *cinfo, *sched;
+  double2* res;
+};
+long long bt_blocks(long long total_frames);
+// Fills the window rows and bin records from the bins kq[hb bt_nbmax + q]
+// (t.info already on the device).
+void launch_bt_tables(hipStream_t s, const Config& c, const BinTable& t, const int* kq,
+                      double2* win, double* bins);
+void launch_refine_table(hipStream_t s, const Config& c, const Utt* utt, const long long* f_off,
+                         long long total_frames, const double* y, const double* official,
+                         const int* count, const BinTable& t, const BtBufs& b, double* rf0,
+                         double* rscore, unsigned long long* work);
+
 void launch_prune_best(hipStream_t s, const Config& c, const Utt* utt,
                        const long long* f_off, long long total_frames,
                        const int* count, const double* rf0,

@@ -524,36 +524,15 @@ __global__ __launch_bounds__(kThreads) void k_window_table(double* tab, int hmax
 }
 
 // Refinement windows of frames whose Blackman phase offset is the integer
-// -(h + 1): GetMainWindow (harvest.cpp:446-456) evaluates
-// 0.42 + 0.5 cos phi + 0.08 cos 2 phi at phi = 2 pi ((basic + m - 1) / fs - t)
-// / (len / fs) with basic = round(t fs - h), so phi = 2 pi (m - h - 1) / len
-// whenever t fs is an integer, which holds at every 1 ms frame when fs_work is
-// a multiple of 1000 Hz.  Entry h^2 + m (m <= 2h, len = 2h + 1) holds
-// (w[m], d[m]): w from sincospi of the exact rational argument, d from w by
-// GetDiffWindow's differences (:459-468): d[0] = -w[1] / 2,
-// d[m] = -(w[m + 1] - w[m - 1]) / 2, d[len - 1] = w[len - 2] / 2.
-__device__ __forceinline__ double main_window_at(int m, int h) {
-  double c1, c2, sn;
-  sincospi(2.0 * (double)(m - h - 1) / (2 * h + 1), &sn, &c1);
-  sincospi(4.0 * (double)(m - h - 1) / (2 * h + 1), &sn, &c2);
-  return 0.42 + 0.5 * c1 + 0.08 * c2;
-}
-
+// -(h + 1) (window_pair): entry h^2 + m (m <= 2h) holds (w[m], d[m]).
 __global__ __launch_bounds__(kThreads) void k_window_pairs(double2* tab, int hmax) {
   const long long id = (long long)blockIdx.x * kThreads + threadIdx.x;
   if (id >= (long long)(hmax + 1) * (hmax + 1)) return;
   int h = (int)sqrt((double)id);
   while ((long long)h * h > id) --h;
   while ((long long)(h + 1) * (h + 1) <= id) ++h;
-  const int m = (int)(id - (long long)h * h), last = 2 * h;
-  double w = 0.0, d = 0.0;
-  if (h > 0) {
-    w = main_window_at(m, h);
-    if (m == 0) d = -main_window_at(1, h) / 2.0;
-    else if (m == last) d = main_window_at(last - 1, h) / 2.0;
-    else d = -(main_window_at(m + 1, h) - main_window_at(m - 1, h)) / 2.0;
-  }
-  tab[id] = make_double2(w, d);
+  const int m = (int)(id - (long long)h * h);
+  tab[id] = h > 0 ? window_pair(m, h) : make_double2(0.0, 0.0);
 }
 
 // Two-level phase table of size P = 2^bits_total: fine entries
@@ -1411,14 +1390,6 @@ __device__ __forceinline__ double lane_value(double v, int src) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// FFT size of GetMeanF0 for half window h: 2^(2 + floor(log2(2h + 1)))
-// (harvest.cpp:598-599; 2h + 1 is odd, so log / kLog2 there is never within
-// rounding of an integer).  window_fft_log2 is its exponent.
-__host__ __device__ inline int window_fft_log2(int h) {
-  return 2 + (31 - __builtin_clz((unsigned)(2 * h + 1)));
-}
-__host__ __device__ inline int window_fft(int h) { return 1 << window_fft_log2(h); }
-
 // q / d and q % d for 0 <= q < 2^16 and 1 <= d < 2^16 from m = ceil(2^32 / d):
 // floor(q m / 2^32) = floor(q / d + q e / (d 2^32)) with e = m d - 2^32 < d,
 // and q e / (d 2^32) < 2^-16 <= 1 / d stays below the gap to the next
@@ -1967,11 +1938,11 @@ __global__ __launch_bounds__(64 * kRefineWaves, 4) void k_refine_waves(
     }
   if (work) {
     const int spread = blockIdx.x % kWorkSpread;
-    if (fault) atomicOr(&work[3 * kWorkSpread + spread], 1ull);
+    if (fault) atomicOr(&work[kStatFault * kWorkSpread + spread], 1ull);
     if (lane == 0) {
-      atomicAdd(&work[0 * kWorkSpread + spread], wsamp);
-      atomicAdd(&work[1 * kWorkSpread + spread], bsamp);
-      atomicAdd(&work[2 * kWorkSpread + spread], nbins);
+      atomicAdd(&work[kStatWindowSamples * kWorkSpread + spread], wsamp);
+      atomicAdd(&work[kStatBinSamples * kWorkSpread + spread], bsamp);
+      atomicAdd(&work[kStatBins * kWorkSpread + spread], nbins);
     }
   }
 }

@@ -1,0 +1,647 @@
+// Bin-table refinement: WORLD's RefineF0Candidates (harvest.cpp:622-631;
+// GetRefinedF0 :589-617, GetMeanF0 :541-584, FixF0 :507-536) for analyses
+// whose 1 ms frames start at whole samples, i.e. fs_work a multiple of
+// 1000 Hz (every WORLD working rate of 8 kHz).
+//
+// A slot's refinement depends on its frame, on the half window h of its
+// candidate (the window and the FFT size n follow from h) and on the bins k
+// FixF0 reads.  With whole-sample frame offsets the window of half length h
+// is the same at every frame (window_pair), and the bins that candidates of
+// one h can read form a short range per harmonic: the candidates giving h lie
+// in one narrow f0 interval.  The work is therefore split as
+//   k_bt_plan     one lane per frame: the half windows its slots use (a
+//                 bitmask per frame) and, per block of 64 frames and per h,
+//                 the frames that use h;
+//   scan + k_bt_scatter  the (frame, h) pairs listed per h in frame order;
+//   k_bt_dft      one lane per pair and one wavefront per 64 pairs of one h:
+//                 the main and derivative windowed spectra at every bin of
+//                 h's table by Goertzel recurrences over the window's
+//                 samples, whose window values and recurrence coefficients
+//                 are uniform across the wavefront (scalar operands), with
+//                 the samples staged in LDS by direct loads;
+//   k_bt_finish   one wavefront per frame: FixF0 and GetRefinedF0's gate for
+//                 each slot from the bin results of its pair.
+// Each bin is one fixed sequence of operations (samples in order), so the
+// results depend neither on the batch nor on which kernel computed the bin.
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+
+#include "kernels.h"
+
+namespace hv {
+namespace {
+
+#define LDS __attribute__((address_space(3)))
+
+// Read-only tables addressed uniformly: the constant address space lets the
+// compiler load them with scalar loads and use them as scalar operands.
+#define CONSTANT_AS __attribute__((address_space(4)))
+template <class T> __device__ __forceinline__ const CONSTANT_AS T* as_constant(const T* p) {
+  return (const CONSTANT_AS T*)p;
+}
+
+void check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    throw std::runtime_error(std::string("HIP kernel launch failed (") + what +
+                             "): " + hipGetErrorString(e));
+}
+
+__device__ __forceinline__ unsigned long long readlane_u64(unsigned long long v, int l) {
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)v, l);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(v >> 32), l);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+// Source frame of overlap slot group g (OverlapF0Candidates harvest.cpp:417-429):
+// g = 0 the frame itself, 1..3 the frames before, 4..6 the frames after.
+__device__ __forceinline__ long long overlap_source(long long i, int g) {
+  return g == 0 ? i : (g <= 3 ? i - g : i + (g - 3));
+}
+
+// Half window of a positive candidate (harvest.cpp:596).
+__device__ __forceinline__ int half_window(double fs, double f0) {
+  return (int)(1.5 * fs / f0 + 1.0);
+}
+
+// Goertzel recurrences of NQ bins for the windowed sample x w (a) and x d
+// (b): s_m = x_m + 2 cos(w) s_{m-1} - s_{m-2} from s_{-1} = s_{-2} = 0, so
+// that sum_{m<N} x_m e^{-i w m} = e^{-i w (N-1)} (s_{N-1} - e^{-i w} s_{N-2}).
+// Steps come in pairs, whose two updates alternate between the two state
+// registers (after each pair a1 = s_m, a2 = s_{m-1}), then one last step for
+// the odd window length, which leaves s_{N-1} in a2 and s_{N-2} in a1.
+template <int NQ> struct Goertzel {
+  double c2[NQ];  // 2 cos(w) of each bin
+  double a1[NQ], a2[NQ], b1[NQ], b2[NQ];
+  __device__ __forceinline__ void clear() {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) a1[q] = a2[q] = b1[q] = b2[q] = 0.0;
+  }
+  __device__ __forceinline__ void step2(double xw0, double xd0, double xw1, double xd1) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      a2[q] = fma(c2[q], a1[q], xw0) - a2[q];
+      b2[q] = fma(c2[q], b1[q], xd0) - b2[q];
+      a1[q] = fma(c2[q], a2[q], xw1) - a1[q];
+      b1[q] = fma(c2[q], b2[q], xd1) - b1[q];
+    }
+  }
+  __device__ __forceinline__ void step_last(double xw, double xd) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      a2[q] = fma(c2[q], a1[q], xw) - a2[q];
+      b2[q] = fma(c2[q], b1[q], xd) - b2[q];
+    }
+  }
+};
+
+// (power, IF numerator) of GetMeanF0 (harvest.cpp:567-572) from the final
+// states (s_{N-1}, s_{N-2} of each signal) of one bin and its record (2 cos w, cos w, sin w, cos w(N-1),
+// sin w(N-1)): main spectrum M and derivative spectrum D by the closing
+// formula above (WORLD's forward FFT takes e^{-i theta}).
+__device__ __forceinline__ double2 goertzel_result(double a1, double a2, double b1, double b2,
+                                                   double cw, double sw, double cl, double sl) {
+  const double yr = a1 - cw * a2, yi = sw * a2;
+  const double zr = b1 - cw * b2, zi = sw * b2;
+  const double mre = cl * yr + sl * yi, mim = cl * yi - sl * yr;
+  const double dre = cl * zr + sl * zi, dim = cl * zi - sl * zr;
+  return make_double2(mre * mre + mim * mim, mre * dim - mim * dre);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Table construction, one workgroup per half window h: the window rows
+// (w[m], d[m]) (window_pair) and, per bin k of h, the record (2 cos w,
+// cos w, sin w, cos w(2h), sin w(2h)) of w = 2 pi k / n, each from sincospi
+// of the exact rational argument.
+__global__ __launch_bounds__(kThreads) void k_bt_tables(Config c, BinTable t, const int* kq,
+                                                        double2* win, double* bins) {
+  const int hb = blockIdx.x;
+  const int4 info = t.info[hb];
+  const int h = c.bt_hlo + hb, len = 2 * h + 1, n = info.x, nb = info.y;
+  for (int m = threadIdx.x; m < len; m += kThreads) win[info.z + m] = window_pair(m, h);
+  for (int q = threadIdx.x; q < nb; q += kThreads) {
+    const long long k = kq[hb * c.bt_nbmax + q];
+    double sn, cs, sl, cl;
+    sincospi(2.0 * (double)(k & (n - 1)) / n, &sn, &cs);
+    sincospi(2.0 * (double)((k * (len - 1)) & (n - 1)) / n, &sl, &cl);
+    double* r = bins + (long long)(info.w + q) * kBtRecord;
+    r[0] = 2.0 * cs;
+    r[1] = cs;
+    r[2] = sn;
+    r[3] = cl;
+    r[4] = sl;
+    for (int e = 5; e < kBtRecord; ++e) r[e] = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Plan: lane = frame of a block of 64 frames (one wavefront per block).
+// needs[f] holds bit (h - bt_hlo) for every half window h of frame f's slots;
+// mask[hb][blk] the block's frames whose pairs are listed for h (frames with
+// more than kBtPairs windows are left to k_bt_finish), cnt its popcount.
+__global__ __launch_bounds__(kThreads) void k_bt_plan(
+    Config c, const Utt* utt, const long long* f_off, long long total_frames, long long nblk,
+    const double* official, const int* count, unsigned long long* needs,
+    unsigned long long* mask, int* cnt, unsigned long long* work) {
+  const int lane = threadIdx.x & 63;
+  const long long blk = (long long)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  if (blk >= nblk) return;
+  const long long gf = blk * 64 + lane;
+  unsigned long long bits[kBtWords] = {};
+  bool fault = false;
+  if (gf < total_frames) {
+    const int ui = find_segment(f_off, c.count, gf);
+    const int cnt_u = count[ui];
+    const long long uf = utt[ui].f_off, frames = utt[ui].frames;
+    const long long i = gf - uf;
+    const double fs = c.fs_work;
+    for (int g = 0; g < 7; ++g) {
+      const long long src = overlap_source(i, g);
+      if (src < 0 || src >= frames) continue;
+      const double* row = official + (uf + src) * c.capacity;
+      for (int jj = 0; jj < cnt_u; ++jj) {
+        const double f0 = row[jj];
+        if (!(f0 > 0.0)) continue;
+        const int hb = half_window(fs, f0) - c.bt_hlo;
+        if (hb < 0 || hb >= c.bt_hr) {
+          fault = true;
+          continue;
+        }
+#pragma unroll
+        for (int w = 0; w < kBtWords; ++w)
+          if ((hb >> 6) == w) bits[w] |= 1ull << (hb & 63);
+      }
+    }
+#pragma unroll
+    for (int w = 0; w < kBtWords; ++w) needs[gf * kBtWords + w] = bits[w];
+  }
+  int pairs = 0;
+#pragma unroll
+  for (int w = 0; w < kBtWords; ++w) pairs += __popcll(bits[w]);
+  if (pairs > kBtPairs)
+#pragma unroll
+    for (int w = 0; w < kBtWords; ++w) bits[w] = 0;
+#pragma unroll
+  for (int w = 0; w < kBtWords; ++w) {
+    if (w * 64 >= c.bt_hr) break;
+    unsigned long long mine = 0;
+    for (int b = 0; b < 64; ++b) {
+      const unsigned long long bal = __ballot((bits[w] >> b) & 1ull);
+      if (lane == b) mine = bal;
+    }
+    const int hb = w * 64 + lane;
+    if (hb < c.bt_hr) {
+      mask[hb * nblk + blk] = mine;
+      cnt[hb * nblk + blk] = __popcll(mine);
+    }
+  }
+  if (fault && work) atomicOr(&work[kStatFault * kWorkSpread + (blockIdx.x % kWorkSpread)], 1ull);
+}
+
+// Pair lists: the pairs of h (hb = h - bt_hlo) occupy list positions
+// [off[hb nblk], off[(hb + 1) nblk]) in frame order.  Entry (frame,
+// utterance << 5 | rank of h among the frame's half windows).
+__global__ __launch_bounds__(kThreads) void k_bt_scatter(
+    Config c, const long long* f_off, long long total_frames, long long nblk,
+    const unsigned long long* mask, const int* off, int2* list) {
+  const int lane = threadIdx.x & 63;
+  const long long blk = (long long)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  if (blk >= nblk) return;
+  const long long gf = blk * 64 + lane;
+  const int ui = gf < total_frames ? find_segment(f_off, c.count, gf) : 0;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  int r = 0;
+  for (int base = 0; base < c.bt_hr; base += 64) {
+    const int hl = base + lane;
+    unsigned long long mk = 0;
+    int of = 0;
+    if (hl < c.bt_hr) {
+      mk = mask[hl * nblk + blk];
+      of = off[hl * nblk + blk];
+    }
+    const int nh = min(64, c.bt_hr - base);
+    for (int k = 0; k < nh; ++k) {
+      const unsigned long long m = readlane_u64(mk, k);
+      if (!m) continue;
+      const int o = __builtin_amdgcn_readlane(of, k);
+      if ((m >> lane) & 1ull) {
+        list[o + __popcll(m & below)] = make_int2((int)gf, (ui << 5) | r);
+        ++r;
+      }
+    }
+  }
+}
+
+// Chunks of 64 pairs, longest windows first (their chunks cost the most):
+// order index k takes h = bt_hlo + bt_hr - 1 - k.  cinfo holds cpre[0..HR]
+// (first chunk of each order index, cpre[HR] the total), then the list start
+// and end of each order index.  Also clears the chunk counter of k_bt_dft.
+__global__ __launch_bounds__(kBtMaxHr) void k_bt_chunks(Config c, long long nblk, const int* off,
+                                                        int* cinfo, int* sched) {
+  __shared__ int wave_sums[kBtMaxHr / 64];
+  const int HR = c.bt_hr, k = threadIdx.x, lane = k & 63, wave = k >> 6;
+  int nch = 0, st = 0, en = 0;
+  if (k < HR) {
+    const int hb = HR - 1 - k;
+    st = off[hb * nblk];
+    en = off[(hb + 1) * nblk];
+    nch = (en - st + 63) >> 6;
+  }
+  int incl = nch;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int o = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += o;
+  }
+  if (lane == 63) wave_sums[wave] = incl;
+  __syncthreads();
+  int before = 0, total = 0;
+#pragma unroll
+  for (int w = 0; w < kBtMaxHr / 64; ++w) {
+    if (w < wave) before += wave_sums[w];
+    total += wave_sums[w];
+  }
+  if (k < HR) {
+    cinfo[k] = before + incl - nch;
+    cinfo[HR + 1 + k] = st;
+    cinfo[2 * HR + 1 + k] = en;
+  }
+  if (k == 0) {
+    cinfo[HR] = total;
+    sched[0] = 0;
+  }
+}
+
+namespace {
+
+// Steps m, m + 1 (pair) or the last step m of a pass: window rows (uniform,
+// (w, d)) applied to the samples.
+template <int NQ>
+__device__ __forceinline__ void bt_step2(const CONSTANT_AS double* win, int m, double y0,
+                                         double y1, Goertzel<NQ>& g) {
+  g.step2(y0 * win[2 * m], y0 * win[2 * m + 1], y1 * win[2 * m + 2], y1 * win[2 * m + 3]);
+}
+template <int NQ>
+__device__ __forceinline__ void bt_last(const CONSTANT_AS double* win, int m, double yv,
+                                        Goertzel<NQ>& g) {
+  g.step_last(yv * win[2 * m], yv * win[2 * m + 1]);
+}
+
+// Direct load of 16 bytes per lane from p into LDS at byte address
+// lds + 16 lane (global_load_lds_dwordx4; M0 holds the LDS address).  It is
+// written as inline assembly so that the compiler's wait-count model, which
+// would wait for every such load before any LDS read, leaves the waits to
+// bt_wait; it has no register outputs, and the memory clobber keeps the
+// compiler's own memory accesses on their side of it.
+__device__ __forceinline__ void lds_dma16(const double* p, unsigned lds) {
+  asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(p), "s"(lds)
+               : "memory", "m0");
+}
+// Waits until at most one block's direct loads (kBtBlock / 2) or no vector
+// memory operations of this wave are outstanding; loads complete in issue
+// order.
+template <bool kOneBlock> __device__ __forceinline__ void bt_wait() {
+  if (kOneBlock) asm volatile("s_waitcnt vmcnt(%0)" ::"n"(kBtBlock / 2) : "memory");
+  else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
+// A pass over the window's len (odd) samples yu[clamp(first + m, 0, last)]:
+// blocks of kBtBlock samples are staged in LDS (lane l's samples 2j, 2j + 1
+// of a block at buf[(64 j + l) 2]), the next block's loads in flight while
+// the current one is summed.  When every lane's blocks lie inside its
+// utterance (!clamp) they go to LDS by direct loads, 16 bytes per lane;
+// otherwise through registers with the clamped indices.  The last block's
+// samples past the window are staged, not used.
+template <int NQ>
+__device__ __forceinline__ void bt_pass(const double* yu, long long first, long long last,
+                                        bool clamp, int len, const CONSTANT_AS double* win,
+                                        LDS double* buf0, LDS double* buf1, int lane,
+                                        Goertzel<NQ>& g) {
+  constexpr int kLoads = kBtBlock / 2;
+  const unsigned lds0 = __builtin_amdgcn_readfirstlane((unsigned)(uintptr_t)buf0);
+  const unsigned lds1 = __builtin_amdgcn_readfirstlane((unsigned)(uintptr_t)buf1);
+  // Loads block [m0, m0 + kBtBlock) into buf (LDS address lds).
+  auto issue = [&](int m0, unsigned lds, LDS double* buf) {
+    if (!clamp) {
+#pragma unroll
+      for (int j = 0; j < kLoads; ++j) lds_dma16(yu + first + m0 + 2 * j, lds + 1024 * j);
+    } else {
+#pragma unroll
+      for (int j = 0; j < kLoads; ++j) {
+        long long i0 = first + m0 + 2 * j, i1 = i0 + 1;
+        i0 = i0 < 0 ? 0 : (i0 > last ? last : i0);
+        i1 = i1 < 0 ? 0 : (i1 > last ? last : i1);
+        const double v0 = yu[i0], v1 = yu[i1];
+        buf[128 * j + 2 * lane] = v0;
+        buf[128 * j + 2 * lane + 1] = v1;
+      }
+    }
+  };
+  // One loop over the sample pairs (the Goertzel state stays in the same
+  // registers throughout); at each block's first pair the next block's loads
+  // are issued into the other buffer.  Block b is in buf0 when b is even.
+  constexpr int kPairs = kBtBlock / 2;
+  issue(0, lds0, buf0);
+  const int npairs = (len - 1) / 2;
+  for (int p = 0; p < npairs; ++p) {
+    const int blk = p / kPairs, jj = p % kPairs;
+    if (jj == 0) {
+      // The previous block's reads are done before its buffer is refilled.
+      asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+      if ((blk + 1) * kBtBlock < len) {
+        if (blk & 1) issue((blk + 1) * kBtBlock, lds0, buf0);
+        else issue((blk + 1) * kBtBlock, lds1, buf1);
+        bt_wait<true>();
+      } else {
+        bt_wait<false>();
+      }
+    }
+    const LDS double* v = ((blk & 1) ? buf1 : buf0) + 128 * jj + 2 * lane;
+    const double v0 = v[0], v1 = v[1];
+    bt_step2<NQ>(win, 2 * p, v0, v1, g);
+  }
+  // The last sample, m = len - 1 (its block's loads may still be in flight).
+  const int m = len - 1, blk = m / kBtBlock, jj = (m % kBtBlock) / 2;
+  bt_wait<false>();
+  bt_last<NQ>(win, m, ((blk & 1) ? buf1 : buf0)[128 * jj + 2 * lane], g);
+}
+
+template <int NQ>
+__device__ __forceinline__ void bt_chunk(bool clamp, const double* yu, long long first,
+                                         long long last, int len,
+                                         const CONSTANT_AS double* win,
+                                         const CONSTANT_AS double* rec, LDS double* buf0,
+                                         LDS double* buf1, int lane, double2* out, bool act) {
+  Goertzel<NQ> g;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) g.c2[q] = rec[q * kBtRecord];
+  g.clear();
+  bt_pass<NQ>(yu, first, last, clamp, len, win, buf0, buf1, lane, g);
+  if (act)
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const CONSTANT_AS double* r = rec + q * kBtRecord;
+      out[q] = goertzel_result(g.a2[q], g.a1[q], g.b2[q], g.b1[q], r[1], r[2], r[3], r[4]);
+    }
+}
+
+}  // namespace
+
+// One wavefront per chunk of 64 pairs of one h.  A workgroup takes four
+// consecutive chunks at a time from a counter until none is left (a
+// persistent grid): its waves then mostly share one h, so that they stream
+// the same table rows through the scalar cache together.
+// res[(frame kBtPairs + rank) bt_nbmax + q] receives (power, IF numerator)
+// of the pair's bin q.
+__global__ __launch_bounds__(kThreads) void k_bt_dft(
+    Config c, const Utt* utt, const double* y, const int2* list, const int* cinfo, int* sched,
+    BinTable t, double2* res, unsigned long long* work) {
+  __shared__ int s_info[3 * kBtMaxHr + 1];
+  __shared__ int s_next;
+  // Sample staging: two buffers per wave of kBtBlock samples per lane.
+  __shared__ double stage0[kThreads / 64][kBtBlock * 64], stage1[kThreads / 64][kBtBlock * 64];
+  const int HR = c.bt_hr;
+  for (int k = threadIdx.x; k < 3 * HR + 1; k += kThreads) s_info[k] = cinfo[k];
+  __syncthreads();
+  const int* cpre = s_info;
+  const int* hstart = s_info + HR + 1;
+  const int* hend = s_info + 2 * HR + 1;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  LDS double* buf0 = (LDS double*)stage0[wave];
+  LDS double* buf1 = (LDS double*)stage1[wave];
+  const int total = cpre[HR];
+  const int spf = (int)(c.fs_work / 1000.0);  // samples per 1 ms frame (whole)
+  unsigned long long wsamp = 0, bsamp = 0, nbins = 0, npairs = 0;
+  const CONSTANT_AS double* wins = as_constant(reinterpret_cast<const double*>(t.win));
+  const CONSTANT_AS double* bins = as_constant(t.bins);
+  while (true) {
+    if (threadIdx.x == 0) s_next = atomicAdd(sched, 1) * (kThreads / 64);
+    __syncthreads();
+    const int first_ch = s_next;
+    __syncthreads();
+    if (first_ch >= total) break;  // uniform across the workgroup
+    const int ch = first_ch + wave;
+    if (ch >= total) continue;
+    int lo = 0, hi = HR - 1;  // order index: largest k with cpre[k] <= ch
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (cpre[mid] <= ch) lo = mid;
+      else hi = mid - 1;
+    }
+    const int k = __builtin_amdgcn_readfirstlane(lo);
+    const int hb = HR - 1 - k, h = c.bt_hlo + hb, len = 2 * h + 1;
+    const int pos0 = hstart[k] + ((ch - cpre[k]) << 6);
+    const int pos = pos0 + lane;
+    const bool act = pos < hend[k];
+    // Inactive lanes repeat lane 0's pair (always active) and store nothing.
+    const int2 e = list[act ? pos : pos0];
+    const int ui = e.y >> 5, r = e.y & 31;
+    const long long gf = e.x;
+    const long long yoff = utt[ui].y_off, uf = utt[ui].f_off;
+    const long long last = (long long)utt[ui].n - 1;
+    const long long first = (long long)spf * (gf - uf) - h - 1;  // GetBaseIndex - 1
+    const int rows = (len + kBtBlock - 1) / kBtBlock * kBtBlock;  // samples staged
+    const bool clamp = __ballot(first < 0 || first + rows - 1 > last) != 0ull;
+    const double* yu = y + yoff;
+    const int4 info = t.info[hb];
+    const int nb = __builtin_amdgcn_readfirstlane(info.y);
+    const CONSTANT_AS double* win = wins + 2LL * __builtin_amdgcn_readfirstlane(info.z);
+    const CONSTANT_AS double* rec0 = bins + (long long)kBtRecord * __builtin_amdgcn_readfirstlane(info.w);
+    double2* out = res + (gf * kBtPairs + r) * c.bt_nbmax;
+    // Passes of at most kBtPassBins bins, split evenly.
+    const int npass = (nb + kBtPassBins - 1) / kBtPassBins;
+    for (int p = 0; p < npass; ++p) {
+      const int q0 = p * nb / npass, nq = (p + 1) * nb / npass - q0;
+      const CONSTANT_AS double* rec = rec0 + (long long)q0 * kBtRecord;
+#define BT_CHUNK(NQ)                                                                        \
+  case NQ:                                                                                 \
+    bt_chunk<NQ>(clamp, yu, first, last, len, win, rec, buf0, buf1, lane, out + q0, act); \
+    break;
+      switch (nq) {
+        BT_CHUNK(1) BT_CHUNK(2) BT_CHUNK(3) BT_CHUNK(4)
+        BT_CHUNK(5) BT_CHUNK(6) BT_CHUNK(7) BT_CHUNK(8)
+        default: break;
+      }
+#undef BT_CHUNK
+    }
+    const unsigned long long na = (unsigned long long)__popcll(__ballot(act));
+    npairs += na;
+    wsamp += na * len;
+    bsamp += na * len * nb;
+    nbins += na * nb;
+  }
+  if (work && lane == 0) {
+    const int spread = blockIdx.x % kWorkSpread;
+    atomicAdd(&work[kStatWindowSamples * kWorkSpread + spread], wsamp);
+    atomicAdd(&work[kStatBinSamples * kWorkSpread + spread], bsamp);
+    atomicAdd(&work[kStatBins * kWorkSpread + spread], nbins);
+    atomicAdd(&work[kStatTablePairs * kWorkSpread + spread], npairs);
+  }
+}
+
+namespace {
+
+// Bin q of half window h at frame `first` computed alone, by the same
+// operations as k_bt_dft (frames with more than kBtPairs windows).
+__device__ double2 bt_bin_direct(const double* yu, long long first, long long last, int h,
+                                 const double2* win, const double* rec) {
+  const int len = 2 * h + 1;
+  Goertzel<1> g;
+  g.c2[0] = rec[0];
+  g.clear();
+  auto at = [&](int m) -> double {
+    long long i = first + m;
+    i = i < 0 ? 0 : (i > last ? last : i);
+    return yu[i];
+  };
+  int m = 0;
+  for (; m + 1 < len; m += 2) {
+    const double y0 = at(m), y1 = at(m + 1);
+    g.step2(y0 * win[m].x, y0 * win[m].y, y1 * win[m + 1].x, y1 * win[m + 1].y);
+  }
+  const double yv = at(m);
+  g.step_last(yv * win[m].x, yv * win[m].y);
+  return goertzel_result(g.a2[0], g.a1[0], g.b2[0], g.b1[0], rec[1], rec[2], rec[3], rec[4]);
+}
+
+}  // namespace
+
+// FixF0 (harvest.cpp:507-536) and GetRefinedF0's gate (:610-614) of every
+// slot, one wavefront per frame (lanes over slots), from the bin results of
+// the slot's (frame, h) pair.  The divisions that feed no integer decision
+// are products with reciprocals (1 / (2 pi), 1 / (h + 1), 1 / f0): within two
+// ulps of WORLD's quotients, which the spectra's own rounding far exceeds.
+__global__ __launch_bounds__(kThreads) void k_bt_finish(
+    Config c, const Utt* utt, const long long* f_off, long long total_frames, const double* y,
+    const double* official, const int* count, const unsigned long long* needs, BinTable t,
+    const double2* res, double* rf0, double* rscore, unsigned long long* work) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const long long gf = (long long)blockIdx.x * (kThreads / 64) + wave;
+  if (gf >= total_frames) return;
+  const int ui = find_segment(f_off, c.count, gf);
+  const int cnt_u = count[ui];
+  const int ncand = 7 * cnt_u;
+  if (ncand == 0) return;
+  const long long uf = utt[ui].f_off, frames = utt[ui].frames;
+  const long long i = gf - uf;
+  const double fs = c.fs_work;
+  unsigned long long nw[kBtWords];
+  int pairs = 0;
+#pragma unroll
+  for (int w = 0; w < kBtWords; ++w) {
+    nw[w] = needs[gf * kBtWords + w];
+    pairs += __popcll(nw[w]);
+  }
+  const bool direct = pairs > kBtPairs;
+  double* out_f0 = rf0 + gf * c.slots;
+  double* out_sc = rscore + gf * c.slots;
+  const double2* row = res + gf * kBtPairs * c.bt_nbmax;
+  const int spf = (int)(fs / 1000.0);
+  bool fault = false;
+  constexpr double kInvTwoPi = 0.5 / kPi;
+  for (int s = lane; s < ncand; s += 64) {
+    const int g = s / cnt_u, jj = s - g * cnt_u;
+    const long long src = overlap_source(i, g);
+    const double f0 = (src >= 0 && src < frames) ? official[(uf + src) * c.capacity + jj] : 0.0;
+    double refined = 0.0, score = 0.0;
+    const int h = f0 > 0.0 ? half_window(fs, f0) : 0;
+    const int hb = h - c.bt_hlo;
+    if (f0 > 0.0 && (hb < 0 || hb >= c.bt_hr)) fault = true;
+    if (f0 > 0.0 && hb >= 0 && hb < c.bt_hr) {
+      int r = 0;
+#pragma unroll
+      for (int w = 0; w < kBtWords; ++w) {
+        const int lo = w * 64;
+        const unsigned long long m =
+            hb >= lo + 64 ? ~0ull : (hb <= lo ? 0ull : ((1ull << (hb - lo)) - 1ull));
+        r += __popcll(nw[w] & m);
+      }
+      const int4 info = t.info[hb];
+      const int n = info.x, H = min((int)(fs / 2.0 / f0), 6);
+      const double qv = f0 * n / fs;
+      const double inv_n = ldexp(1.0, -window_fft_log2(h));  // 1 / n, exact
+      const double inv_f0 = 1.0 / f0;
+      const long long first = (long long)spf * i - h - 1;
+      const double* yu = y + utt[ui].y_off;
+      const long long last = (long long)utt[ui].n - 1;
+      const double2* pr = row + (long long)r * c.bt_nbmax;
+      double numer = 0.0, den = 0.0, dev = 0.0;
+      for (int hh = 0; hh < H; ++hh) {
+        const int k = mround(qv * (hh + 1));
+        const int4 hr = t.harm[hb * 6 + hh];  // (first bin, bins, first result index)
+        const int kk = k - hr.x;
+        double2 pn = make_double2(0.0, 0.0);
+        if (kk < 0 || kk >= hr.y) fault = true;  // beyond the tables: reported, not read
+        else if (direct)
+          pn = bt_bin_direct(yu, first, last, h, t.win + info.z,
+                             t.bins + (long long)(info.w + hr.z + kk) * kBtRecord);
+        else pn = pr[hr.z + kk];
+        const double pw = pn.x, nu = pn.y;
+        const double inv_h = hh == 0 ? 1.0 : hh == 1 ? 0.5 : hh == 2 ? 1.0 / 3.0
+                           : hh == 3 ? 0.25 : hh == 4 ? 0.2 : 1.0 / 6.0;
+        const double ifr = pw == 0.0 ? 0.0 : (double)k * fs * inv_n + nu / pw * fs * kInvTwoPi;
+        const double amp = sqrt(pw);
+        numer += amp * ifr;
+        den += amp * (hh + 1.0);
+        dev += fabs((ifr * inv_h - f0) * inv_f0);
+      }
+      refined = numer / (den + kGuard);
+      score = 1.0 / (dev / H + kGuard);
+      if (refined < c.f0_floor || refined > c.f0_ceil || score < 2.5) {
+        refined = 0.0;
+        score = 0.0;
+      }
+    }
+    out_f0[s] = refined;
+    out_sc[s] = score;
+  }
+  if (fault && work) atomicOr(&work[kStatFault * kWorkSpread + (blockIdx.x % kWorkSpread)], 1ull);
+}
+
+// ---------------------------------------------------------------------------
+// Launchers.
+
+void launch_bt_tables(hipStream_t s, const Config& c, const BinTable& t, const int* kq,
+                      double2* win, double* bins) {
+  if (c.bt_hr <= 0) return;
+  k_bt_tables<<<c.bt_hr, kThreads, 0, s>>>(c, t, kq, win, bins);
+  check_launch("bt_tables");
+}
+
+long long bt_blocks(long long total_frames) { return (total_frames + 63) / 64; }
+
+void launch_refine_table(hipStream_t s, const Config& c, const Utt* utt, const long long* f_off,
+                         long long total_frames, const double* y, const double* official,
+                         const int* count, const BinTable& t, const BtBufs& b, double* rf0,
+                         double* rscore, unsigned long long* work) {
+  if (c.bt_hr <= 0 || c.bt_hr > kBtMaxHr) throw std::runtime_error("refine_table: no bin table");
+  if (total_frames <= 0) return;
+  const long long nblk = bt_blocks(total_frames);
+  const unsigned grid_blk = (unsigned)((nblk + kThreads / 64 - 1) / (kThreads / 64));
+  k_bt_plan<<<grid_blk, kThreads, 0, s>>>(c, utt, f_off, total_frames, nblk, official, count,
+                                          b.needs, b.mask, b.cnt, work);
+  check_launch("bt_plan");
+  launch_exclusive_scan(s, b.cnt, b.off, (long long)c.bt_hr * nblk, b.scan_tmp, b.scan_len);
+  k_bt_scatter<<<grid_blk, kThreads, 0, s>>>(c, f_off, total_frames, nblk, b.mask, b.off, b.list);
+  check_launch("bt_scatter");
+  k_bt_chunks<<<1, kBtMaxHr, 0, s>>>(c, nblk, b.off, b.cinfo, b.sched);
+  check_launch("bt_chunks");
+  // A persistent grid: enough workgroups to fill the device, fewer for short
+  // analyses (at most one per four possible chunks: kBtPairs pairs per
+  // frame, 64 per chunk, a partial chunk per h).
+  long long groups = (total_frames * kBtPairs / 64 + c.bt_hr + 3) / 4;
+  groups = groups < 64 ? 64 : (groups > kBtDftGroups ? kBtDftGroups : groups);
+  k_bt_dft<<<(unsigned)groups, kThreads, 0, s>>>(c, utt, y, b.list, b.cinfo, b.sched, t, b.res,
+                                                 work);
+  check_launch("bt_dft");
+  k_bt_finish<<<(unsigned)((total_frames + 3) / 4), kThreads, 0, s>>>(
+      c, utt, f_off, total_frames, y, official, count, b.needs, t, b.res, rf0, rscore, work);
+  check_launch("bt_finish");
+}
+
+}  // namespace hv

@@ -2,7 +2,8 @@
 
 ``tuning.json`` holds what ``harvest_cuda.autotune`` chose on an MI355X:
   backend: fft_min_half (FFT/direct filterbank split), graphs (HIP graph
-           capture of the launch sequence), slab_budget_mib;
+           capture of the launch sequence), slab_budget_mib, refine_table
+           (bin-table refinement where frames start at whole samples);
   batch:   how large batches are cut into length buckets (max_groups,
            min_group_frames, split_frames, first_weight, last_weight) and
            whether the buckets' device work runs one after another
@@ -18,7 +19,7 @@ from pathlib import Path
 from . import _core
 
 TABLE = Path(__file__).resolve().parent / "tuning.json"
-BACKEND_KEYS = ("fft_min_half", "graphs", "slab_budget_mib")
+BACKEND_KEYS = ("fft_min_half", "graphs", "slab_budget_mib", "refine_table")
 BATCH_KEYS = ("max_groups", "min_group_frames", "split_frames", "first_weight", "last_weight",
               "serialize", "depth")
 

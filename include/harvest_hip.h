@@ -51,8 +51,10 @@ typedef enum hv_status {
  *   [0] refinement window samples computed (distinct (frame, h) windows)
  *   [1] refinement bin-samples: one term of the main and of the derivative
  *       spectrum DFT at one bin (distinct (frame, h, k) bins)
- *   [2] distinct refinement bins */
-#define HV_STAT_SLOTS 3
+ *   [2] refinement bins computed
+ *   [3] (frame, h) windows computed by the bin-table refinement (0 when the
+ *       per-frame refinement ran) */
+#define HV_STAT_SLOTS 4
 
 /* Tuning of a workspace.  No reference counterpart: the package's
  * autotuner (harvest_cuda/autotune.py) picks the values on the GPU and
@@ -67,13 +69,19 @@ typedef enum hv_status {
  *                    replayed after that; analyses that collect timings run
  *                    launch by launch (default 0);
  *   slab_budget_mib  device memory of the crossing slabs of one band group
- *                    (default 4096; at least 64).
+ *                    (default 4096; at least 64);
+ *   refine_table     1: analyses whose 1 ms frames start at whole samples
+ *                    (working rate a multiple of 1000 Hz) refine candidates
+ *                    through per-half-window bin tables, one lane per (frame,
+ *                    half window) pair; 0: one wavefront per frame for every
+ *                    analysis (default 1).
  * Results do not depend on the tuning beyond the rounding of the two
  * filterbank forms (fft_min_half). */
 typedef struct hv_tuning {
   int fft_min_half;
   int graphs;
   int slab_budget_mib;
+  int refine_table;
 } hv_tuning;
 hv_status hv_get_tuning(hv_workspace* ws, hv_tuning* out);
 hv_status hv_set_tuning(hv_workspace* ws, const hv_tuning* in);
