@@ -51,7 +51,7 @@ def build_backend() -> Path:
 
     def compile_host(src):  # host-only code: the host compiler, no device pass
         obj = tmp / (src + ".o")
-        _run(["g++", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-c", str(CSRC / src),
+        _run(["g++", "-O3", "-std=c++17", "-fPIC", "-pthread", "-ffp-contract=off", "-c", str(CSRC / src),
               "-o", str(obj)])
         return obj
 

@@ -240,7 +240,8 @@ struct hv_workspace {
   DevBuf bt_info, bt_harm, bt_kq, bt_win, bt_bins;
   int bt_hlo = 0, bt_hr = 0, bt_nbmax = 0;
   double bt_key_fs = -1.0, bt_key_floor = -1.0, bt_key_ceil = -1.0;
-  DevBuf bt_needs, bt_mask, bt_cnt, bt_off, bt_scan, bt_list, bt_cinfo, bt_sched, bt_res;
+  DevBuf bt_codes, bt_inv, bt_needs, bt_mask, bt_cnt, bt_off, bt_scan, bt_list, bt_cinfo,
+      bt_sched, bt_res;
   long long last_nnz = -1;
   long long last_frames = 0;
   bool submitted = false;       // an analysis is enqueued and not yet collected
@@ -558,6 +559,7 @@ void prepare_bintable(hv_workspace* ws, double fs_work, double f0_floor, double 
       harm[6 * hb + j] = e;
     }
     const int nb = (int)bins[hb].size();
+    if (nb > 0 && bins[hb].back() > 127) return;  // candidate codes hold 7-bit bins
     info[hb] = int4{n, nb, (int)rows, (int)records};
     rows += 2 * h + 1;
     records += nb;
@@ -783,6 +785,8 @@ Bufs size_buffers(hv_workspace* ws, const Plan& p, const double* dx) {
   b.bt = BtBufs{};
   if (c.bt_hr > 0) {
     const long long nblk = bt_blocks(p.total_frames), cells = (long long)c.bt_hr * nblk;
+    b.bt.codes = ws->bt_codes.ensure<unsigned long long>((size_t)p.total_frames * c.capacity);
+    b.bt.inv = ws->bt_inv.ensure<double>((size_t)p.total_frames * c.capacity);
     b.bt.needs = ws->bt_needs.ensure<unsigned long long>((size_t)p.total_frames * kBtWords);
     b.bt.mask = ws->bt_mask.ensure<unsigned long long>(cells);
     b.bt.cnt = ws->bt_cnt.ensure<int>(cells);
@@ -924,7 +928,7 @@ std::vector<long long> graph_key(hv_workspace* ws, const Plan& p, const Bufs& b)
                         b.count, b.official, b.rf0, b.rsc, b.pf0, b.psc, b.dbest, b.cinit, b.work,
                         b.row_nnz, b.doffsets, b.scan_tmp, b.hoff, b.hbest, b.hwork,
                         b.hcand, ws->bt_info.p, ws->bt_harm.p, ws->bt_win.p, ws->bt_bins.p,
-                        b.bt.needs,
+                        b.bt.codes, b.bt.inv, b.bt.needs,
                         b.bt.mask, b.bt.cnt, b.bt.off, b.bt.scan_tmp, b.bt.list, b.bt.cinfo,
                         b.bt.sched, b.bt.res};
   for (const void* q : ptrs) k.push_back((long long)(uintptr_t)q);
@@ -1096,8 +1100,8 @@ void world_sizes(long long length, int fs, double f0_floor, int ratio, double fs
   *frames = (int)(1000.0 * (double)length / fs / 1.0) + 1;
 }
 
-// Host input is staged and uploaded in slices of this many samples (4 MiB).
-constexpr long long kUploadSlice = 1LL << 19;
+// Host input is staged and uploaded in slices of this many samples (8 MiB).
+constexpr long long kUploadSlice = 1LL << 20;
 
 void check_common(int fs, double f0_floor, double f0_ceil) {
   if (fs < 8000 || fs > 192000) invalid("fs must be between 8000 and 192000 Hz");
@@ -1145,21 +1149,24 @@ Plan stage_batch(hv_workspace* ws, int count, const double* const* xs, const int
     dx = xs[0];
   } else {
     // The batch goes through pinned staging in slices: the copy of slice k+1
-    // (which also checks that every sample is finite) overlaps the upload of
-    // slice k.  Slices split utterances freely; each utterance is contiguous
-    // at its x_off in both buffers.
+    // (which also checks that every sample is finite; parts of it on several
+    // host threads) overlaps the upload of slice k.  Slices split utterances
+    // freely; each utterance is contiguous at its x_off in both buffers.
     double* hx = ws->h_x.ensure<double>(p.total_x);
     double* d = ws->x.ensure<double>(p.total_x);
     bool finite = true;
     int u = 0;
+    std::vector<CopyPiece> pieces;
     for (long long at = 0; at < p.total_x; at += kUploadSlice) {
       const long long end = std::min(at + kUploadSlice, p.total_x);
+      pieces.clear();
       for (; u < count && p.utts[u].x_off < end; ++u) {
         const long long a = std::max(at, p.utts[u].x_off);
         const long long b = std::min(end, p.utts[u].x_off + p.utts[u].length);
-        finite &= copy_finite(hx + a, xs[u] + (a - p.utts[u].x_off), b - a);
+        pieces.push_back(CopyPiece{hx + a, xs[u] + (a - p.utts[u].x_off), b - a});
         if (b < p.utts[u].x_off + p.utts[u].length) break;  // continues in the next slice
       }
+      finite &= copy_finite_parallel(pieces.data(), (int)pieces.size());
       check(hipMemcpyAsync(d + at, hx + at, (end - at) * sizeof(double), hipMemcpyHostToDevice, s),
             "upload x");
     }

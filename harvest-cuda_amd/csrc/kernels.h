@@ -108,11 +108,14 @@ struct BinTable {
   const double2* win;
   const double* bins;
 };
-// Per-analysis buffers: needs[frames][kBtWords], mask and cnt [bt_hr][nblk]
-// (nblk = bt_blocks(frames)), off[bt_hr nblk + 1] with its scan scratch,
+// Per-analysis buffers: codes and inv [frames][capacity],
+// needs[frames][kBtWords], mask and cnt [bt_hr][nblk] (nblk =
+// bt_blocks(frames)), off[bt_hr nblk + 1] with its scan scratch,
 // list[frames kBtPairs], cinfo[3 bt_hr + 1], sched[1],
 // res[frames kBtPairs bt_nbmax].
 struct BtBufs {
+  unsigned long long* codes;
+  double* inv;
   unsigned long long *needs, *mask;
   int *cnt, *off, *scan_tmp;
   long long scan_len;

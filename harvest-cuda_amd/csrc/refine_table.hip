@@ -9,6 +9,8 @@
 // is the same at every frame (window_pair), and the bins that candidates of
 // one h can read form a short range per harmonic: the candidates giving h lie
 // in one narrow f0 interval.  The work is therefore split as
+//   k_bt_codes    one lane per official candidate: its half window h,
+//                 harmonic count and FixF0 bins, packed in one word, and 1/f0;
 //   k_bt_plan     one lane per frame: the half windows its slots use (a
 //                 bitmask per frame) and, per block of 64 frames and per h,
 //                 the frames that use h;
@@ -139,38 +141,77 @@ __global__ __launch_bounds__(kThreads) void k_bt_tables(Config c, BinTable t, co
 }
 
 // ---------------------------------------------------------------------------
+// Candidate codes: for official candidate jj of frame f (official[f][jj]),
+// code = 1 | hb << 1 | H << 9 | k_1 << 12 | ... | k_6 << 42 with hb = h -
+// bt_hlo for the half window h (harvest.cpp:596), H the harmonics FixF0 uses
+// (:577-578) and k_j the bin of harmonic j (:515, 7 bits each); 0 for a
+// candidate that is not positive or not one of the utterance's count.
+// inv[f][jj] = 1 / f0.  A half window or bin outside the tables sets the
+// fault flag.
+__global__ __launch_bounds__(kThreads) void k_bt_codes(
+    Config c, const long long* f_off, long long total_frames, const double* official,
+    const int* count, const int4* harm, unsigned long long* codes, double* inv,
+    unsigned long long* work) {
+  const long long e = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (e >= total_frames * c.capacity) return;
+  const long long gf = e / c.capacity;
+  const int jj = (int)(e - gf * c.capacity);
+  const int ui = find_segment(f_off, c.count, gf);
+  const double f0 = official[e];
+  unsigned long long code = 0;
+  double r = 0.0;
+  bool fault = false;
+  if (jj < count[ui] && f0 > 0.0) {
+    const double fs = c.fs_work;
+    const int h = half_window(fs, f0), hb = h - c.bt_hlo;
+    if (hb < 0 || hb >= c.bt_hr) {
+      fault = true;
+    } else {
+      const int n = window_fft(h), H = min((int)(fs / 2.0 / f0), 6);
+      const double qv = f0 * n / fs;
+      code = 1ull | (unsigned long long)hb << 1 | (unsigned long long)H << 9;
+      for (int j = 0; j < H; ++j) {
+        const int k = mround(qv * (j + 1));
+        const int4 hr = harm[hb * 6 + j];  // (first bin, bins, first result index)
+        if (k - hr.x < 0 || k - hr.x >= hr.y || k > 127) fault = true;
+        code |= (unsigned long long)(k & 127) << (12 + 7 * j);
+      }
+      r = 1.0 / f0;
+      if (fault) code = 0;
+    }
+  }
+  codes[e] = code;
+  inv[e] = r;
+  if (fault && work) atomicOr(&work[kStatFault * kWorkSpread + (blockIdx.x % kWorkSpread)], 1ull);
+}
+
+// ---------------------------------------------------------------------------
 // Plan: lane = frame of a block of 64 frames (one wavefront per block).
 // needs[f] holds bit (h - bt_hlo) for every half window h of frame f's slots;
 // mask[hb][blk] the block's frames whose pairs are listed for h (frames with
 // more than kBtPairs windows are left to k_bt_finish), cnt its popcount.
 __global__ __launch_bounds__(kThreads) void k_bt_plan(
     Config c, const Utt* utt, const long long* f_off, long long total_frames, long long nblk,
-    const double* official, const int* count, unsigned long long* needs,
-    unsigned long long* mask, int* cnt, unsigned long long* work) {
+    const unsigned long long* codes, const int* count, unsigned long long* needs,
+    unsigned long long* mask, int* cnt) {
   const int lane = threadIdx.x & 63;
   const long long blk = (long long)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
   if (blk >= nblk) return;
   const long long gf = blk * 64 + lane;
   unsigned long long bits[kBtWords] = {};
-  bool fault = false;
   if (gf < total_frames) {
     const int ui = find_segment(f_off, c.count, gf);
     const int cnt_u = count[ui];
     const long long uf = utt[ui].f_off, frames = utt[ui].frames;
     const long long i = gf - uf;
-    const double fs = c.fs_work;
     for (int g = 0; g < 7; ++g) {
       const long long src = overlap_source(i, g);
       if (src < 0 || src >= frames) continue;
-      const double* row = official + (uf + src) * c.capacity;
+      const unsigned long long* row = codes + (uf + src) * c.capacity;
       for (int jj = 0; jj < cnt_u; ++jj) {
-        const double f0 = row[jj];
-        if (!(f0 > 0.0)) continue;
-        const int hb = half_window(fs, f0) - c.bt_hlo;
-        if (hb < 0 || hb >= c.bt_hr) {
-          fault = true;
-          continue;
-        }
+        const unsigned long long code = row[jj];
+        if (!(code & 1ull)) continue;
+        const int hb = (int)(code >> 1) & 0xFF;
 #pragma unroll
         for (int w = 0; w < kBtWords; ++w)
           if ((hb >> 6) == w) bits[w] |= 1ull << (hb & 63);
@@ -199,7 +240,6 @@ __global__ __launch_bounds__(kThreads) void k_bt_plan(
       cnt[hb * nblk + blk] = __popcll(mine);
     }
   }
-  if (fault && work) atomicOr(&work[kStatFault * kWorkSpread + (blockIdx.x % kWorkSpread)], 1ull);
 }
 
 // Pair lists: the pairs of h (hb = h - bt_hlo) occupy list positions
@@ -512,14 +552,23 @@ __device__ double2 bt_bin_direct(const double* yu, long long first, long long la
 }  // namespace
 
 // FixF0 (harvest.cpp:507-536) and GetRefinedF0's gate (:610-614) of every
-// slot, one wavefront per frame (lanes over slots), from the bin results of
-// the slot's (frame, h) pair.  The divisions that feed no integer decision
-// are products with reciprocals (1 / (2 pi), 1 / (h + 1), 1 / f0): within two
-// ulps of WORLD's quotients, which the spectra's own rounding far exceeds.
+// slot, one wavefront per frame (lanes over slots), from the slot's
+// candidate code and the bin results of its (frame, h) pair.  The divisions
+// that feed no integer decision are products with reciprocals (1 / (2 pi),
+// 1 / (h + 1), 1 / f0): within two ulps of WORLD's quotients, which the
+// spectra's own rounding far exceeds.
 __global__ __launch_bounds__(kThreads) void k_bt_finish(
     Config c, const Utt* utt, const long long* f_off, long long total_frames, const double* y,
-    const double* official, const int* count, const unsigned long long* needs, BinTable t,
-    const double2* res, double* rf0, double* rscore, unsigned long long* work) {
+    const double* official, const unsigned long long* codes, const double* inv,
+    const int* count, const unsigned long long* needs, BinTable t, const double2* res,
+    double* rf0, double* rscore) {
+  // (first bin, first result index) of every (half window, harmonic).
+  __shared__ int2 s_harm[6 * kBtMaxHr];
+  for (int k = threadIdx.x; k < 6 * c.bt_hr; k += kThreads) {
+    const int4 hr = t.harm[k];
+    s_harm[k] = make_int2(hr.x, hr.z);
+  }
+  __syncthreads();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const long long gf = (long long)blockIdx.x * (kThreads / 64) + wave;
   if (gf >= total_frames) return;
@@ -542,18 +591,17 @@ __global__ __launch_bounds__(kThreads) void k_bt_finish(
   double* out_sc = rscore + gf * c.slots;
   const double2* row = res + gf * kBtPairs * c.bt_nbmax;
   const int spf = (int)(fs / 1000.0);
-  bool fault = false;
   constexpr double kInvTwoPi = 0.5 / kPi;
   for (int s = lane; s < ncand; s += 64) {
     const int g = s / cnt_u, jj = s - g * cnt_u;
     const long long src = overlap_source(i, g);
-    const double f0 = (src >= 0 && src < frames) ? official[(uf + src) * c.capacity + jj] : 0.0;
+    const long long e = (uf + src) * c.capacity + jj;
+    const unsigned long long code = (src >= 0 && src < frames) ? codes[e] : 0ull;
     double refined = 0.0, score = 0.0;
-    const int h = f0 > 0.0 ? half_window(fs, f0) : 0;
-    const int hb = h - c.bt_hlo;
-    if (f0 > 0.0 && (hb < 0 || hb >= c.bt_hr)) fault = true;
-    if (f0 > 0.0 && hb >= 0 && hb < c.bt_hr) {
-      int r = 0;
+    if (code & 1ull) {
+      const double f0 = official[e], inv_f0 = inv[e];
+      const int hb = (int)(code >> 1) & 0xFF, H = (int)(code >> 9) & 7, h = c.bt_hlo + hb;
+      int r = 0;  // rank of h among the frame's half windows
 #pragma unroll
       for (int w = 0; w < kBtWords; ++w) {
         const int lo = w * 64;
@@ -561,26 +609,22 @@ __global__ __launch_bounds__(kThreads) void k_bt_finish(
             hb >= lo + 64 ? ~0ull : (hb <= lo ? 0ull : ((1ull << (hb - lo)) - 1ull));
         r += __popcll(nw[w] & m);
       }
-      const int4 info = t.info[hb];
-      const int n = info.x, H = min((int)(fs / 2.0 / f0), 6);
-      const double qv = f0 * n / fs;
       const double inv_n = ldexp(1.0, -window_fft_log2(h));  // 1 / n, exact
-      const double inv_f0 = 1.0 / f0;
-      const long long first = (long long)spf * i - h - 1;
-      const double* yu = y + utt[ui].y_off;
-      const long long last = (long long)utt[ui].n - 1;
       const double2* pr = row + (long long)r * c.bt_nbmax;
       double numer = 0.0, den = 0.0, dev = 0.0;
       for (int hh = 0; hh < H; ++hh) {
-        const int k = mround(qv * (hh + 1));
-        const int4 hr = t.harm[hb * 6 + hh];  // (first bin, bins, first result index)
-        const int kk = k - hr.x;
-        double2 pn = make_double2(0.0, 0.0);
-        if (kk < 0 || kk >= hr.y) fault = true;  // beyond the tables: reported, not read
-        else if (direct)
-          pn = bt_bin_direct(yu, first, last, h, t.win + info.z,
-                             t.bins + (long long)(info.w + hr.z + kk) * kBtRecord);
-        else pn = pr[hr.z + kk];
+        const int k = (int)(code >> (12 + 7 * hh)) & 127;
+        const int2 hr = s_harm[hb * 6 + hh];
+        const int q = hr.y + (k - hr.x);
+        double2 pn;
+        if (direct) {
+          const int4 info = t.info[hb];
+          pn = bt_bin_direct(y + utt[ui].y_off, (long long)spf * i - h - 1,
+                             (long long)utt[ui].n - 1, h, t.win + info.z,
+                             t.bins + (long long)(info.w + q) * kBtRecord);
+        } else {
+          pn = pr[q];
+        }
         const double pw = pn.x, nu = pn.y;
         const double inv_h = hh == 0 ? 1.0 : hh == 1 ? 0.5 : hh == 2 ? 1.0 / 3.0
                            : hh == 3 ? 0.25 : hh == 4 ? 0.2 : 1.0 / 6.0;
@@ -600,7 +644,6 @@ __global__ __launch_bounds__(kThreads) void k_bt_finish(
     out_f0[s] = refined;
     out_sc[s] = score;
   }
-  if (fault && work) atomicOr(&work[kStatFault * kWorkSpread + (blockIdx.x % kWorkSpread)], 1ull);
 }
 
 // ---------------------------------------------------------------------------
@@ -623,8 +666,12 @@ void launch_refine_table(hipStream_t s, const Config& c, const Utt* utt, const l
   if (total_frames <= 0) return;
   const long long nblk = bt_blocks(total_frames);
   const unsigned grid_blk = (unsigned)((nblk + kThreads / 64 - 1) / (kThreads / 64));
-  k_bt_plan<<<grid_blk, kThreads, 0, s>>>(c, utt, f_off, total_frames, nblk, official, count,
-                                          b.needs, b.mask, b.cnt, work);
+  const long long entries = total_frames * c.capacity;
+  k_bt_codes<<<(unsigned)((entries + kThreads - 1) / kThreads), kThreads, 0, s>>>(
+      c, f_off, total_frames, official, count, t.harm, b.codes, b.inv, work);
+  check_launch("bt_codes");
+  k_bt_plan<<<grid_blk, kThreads, 0, s>>>(c, utt, f_off, total_frames, nblk, b.codes, count,
+                                          b.needs, b.mask, b.cnt);
   check_launch("bt_plan");
   launch_exclusive_scan(s, b.cnt, b.off, (long long)c.bt_hr * nblk, b.scan_tmp, b.scan_len);
   k_bt_scatter<<<grid_blk, kThreads, 0, s>>>(c, f_off, total_frames, nblk, b.mask, b.off, b.list);
@@ -640,7 +687,8 @@ void launch_refine_table(hipStream_t s, const Config& c, const Utt* utt, const l
                                                  work);
   check_launch("bt_dft");
   k_bt_finish<<<(unsigned)((total_frames + 3) / 4), kThreads, 0, s>>>(
-      c, utt, f_off, total_frames, y, official, count, b.needs, t, b.res, rf0, rscore, work);
+      c, utt, f_off, total_frames, y, official, b.codes, b.inv, count, b.needs, t, b.res, rf0,
+      rscore);
   check_launch("bt_finish");
 }
 

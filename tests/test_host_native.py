@@ -8,7 +8,9 @@ under AddressSanitizer with UndefinedBehaviorSanitizer.
     mapping against a fake HIP runtime (tests/cpp/test_cuda_device.cpp:9-101);
   * test_engine.cpp           the host engine (thread pool, contour stages,
     two-workspace pipeline, failure handling) against a CPU stand-in of the
-    backend's C ABI (fake_backend.cpp).
+    backend's C ABI (fake_backend.cpp);
+  * test_host_copy.cpp        the backend's parallel staging copy (its thread
+    pool, concurrent callers, finiteness flags).
 """
 import shutil
 import subprocess
@@ -24,6 +26,7 @@ SOURCES = {
     "test_workspace_cache": [CPP / "test_workspace_cache.cpp"],
     "test_device": [CPP / "test_device.cpp"],
     "test_engine": [CPP / "test_engine.cpp", CPP / "fake_backend.cpp", CSRC / "engine.cpp"],
+    "test_host_copy": [CPP / "test_host_copy.cpp", CSRC / "host_copy.cpp"],
 }
 MODES = {
     "plain": ["-O2"],
