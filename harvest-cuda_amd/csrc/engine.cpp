@@ -891,7 +891,7 @@ BatchTuning g_tuning;
 void set_batch_tuning(const BatchTuning& t) {
   if (t.max_groups < 1 || t.min_group_frames < 1 || t.split_frames < 0 || !(t.first_weight > 0) ||
       !(t.last_weight > 0) || !std::isfinite(t.first_weight) || !std::isfinite(t.last_weight) ||
-      (t.serialize != 0 && t.serialize != 1) || t.depth < 2 || t.depth > 4)
+      (t.serialize != 0 && t.serialize != 1) || t.depth < 2 || t.depth > 8)
     throw std::invalid_argument("invalid batch tuning");
   std::lock_guard<std::mutex> lock(g_tuning_mutex);
   g_tuning = t;

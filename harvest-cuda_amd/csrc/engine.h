@@ -87,7 +87,7 @@ struct BatchTuning {
   // workspaces' kernels do not run concurrently; uploads, downloads and host
   // work still overlap the device).
   int serialize = 0;
-  // Workspaces a pipelined batch keeps in flight (2-4): groups g, g + 1, ...,
+  // Workspaces a pipelined batch keeps in flight (2-8): groups g, g + 1, ...,
   // g + depth - 1 are enqueued while group g's contours run.
   int depth = 2;
 };

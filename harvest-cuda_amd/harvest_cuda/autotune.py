@@ -3,18 +3,21 @@
     python -m harvest_cuda.autotune [--out harvest_cuda/tuning.json]
 
 It measures, on a calibration workload drawn like the batched benchmark
-(256 utterances at 24 kHz, lengths log-uniform over 0.5-30 s) but from
+(512 utterances at 24 kHz, lengths log-uniform over 0.5-30 s) but from
 another seed (2003), so that the benchmark's own utterances are not tuned
 on:
   1. the FFT/direct filterbank split (fft_min_half): the filterbank kernels'
      time on one 60 s utterance at 24 kHz for each candidate split, keeping
      only splits whose F0 equals the default's to 1e-9 with identical
      voicing;
-  2. the length buckets (max_groups, first_weight, last_weight) and whether
-     the buckets' device work runs one after another (serialize): the wall
-     time of one batched call, NumPy in and out, median of 3, over a grid;
-  3. how many workspaces keep buckets in flight (depth 2-4);
-  4. HIP graph capture on or off at the chosen buckets.
+  2. the pipeline: how many length buckets (max_groups) and how many
+     workspaces keep buckets in flight (depth 2-4), jointly, from the wall
+     time of one batched call, NumPy in and out, median of 3; more buckets
+     in flight overlap one bucket's host staging, copies and contours with
+     the others' kernels, at the price of more, smaller launches;
+  3. the bucket boundaries (first_weight, last_weight) at that pipeline;
+  4. whether the buckets' device work runs one after another (serialize);
+  5. HIP graph capture on or off at the chosen pipeline.
 The winners and every measurement go to the table, which the package
 applies at import (harvest_cuda/tuning.py).
 """
@@ -34,7 +37,8 @@ from . import _core, tuning
 from .signals import _voiced_unvoiced, config3
 
 FFT_SPLITS = (32, 48, 64, 80, 96, 128, 160, 1001)
-GROUPS = (2, 3, 4, 6, 8, 12)
+DEPTHS = (2, 3, 4)
+GROUPS = (4, 8, 12, 16, 24, 32)
 WEIGHTS = (0.4, 0.7, 1.0)
 
 
@@ -86,26 +90,39 @@ def batch_wall(xs, fs, repeats: int) -> float:
     return statistics.median(ms)
 
 
-def tune_buckets(xs, fs, repeats: int = 3) -> tuple[dict, list]:
+def tune_pipeline(xs, fs, repeats: int = 3) -> tuple[dict, list]:
+    base = {"min_group_frames": 20000, "split_frames": 200000}
     rows = []
-    for serialize, g, first, last in itertools.product((0, 1), GROUPS, WEIGHTS, WEIGHTS):
-        tuning.set_batch(max_groups=g, min_group_frames=20000, split_frames=200000,
-                         first_weight=first, last_weight=last, serialize=serialize)
+
+    def measure(**kw):
+        tuning.set_batch(**base, **kw)
         ms = batch_wall(xs, fs, repeats)
-        rows.append({"serialize": serialize, "max_groups": g, "first_weight": first,
-                     "last_weight": last, "ms": ms})
-        log(f"serialize {serialize} groups {g} first {first} last {last}: {ms:.1f} ms")
-    best = min(rows, key=lambda r: r["ms"])
-    chosen = {"max_groups": best["max_groups"], "min_group_frames": 20000, "split_frames": 200000,
-              "first_weight": best["first_weight"], "last_weight": best["last_weight"],
-              "serialize": best["serialize"]}
-    return chosen, rows
+        rows.append({**kw, "ms": ms})
+        log(" ".join(f"{k} {v}" for k, v in kw.items()) + f": {ms:.1f} ms")
+        return ms
+
+    chosen = {"first_weight": 0.4, "last_weight": 0.4, "serialize": 0}
+    best = None
+    for depth, g in itertools.product(DEPTHS, GROUPS):
+        ms = measure(max_groups=g, depth=depth, **chosen)
+        if best is None or ms < best[0]:
+            best = (ms, depth, g)
+    chosen.update(depth=best[1], max_groups=best[2])
+    for first, last in itertools.product(WEIGHTS, WEIGHTS):
+        ms = measure(**{**chosen, "first_weight": first, "last_weight": last})
+        if ms < best[0]:
+            best = (ms,) + best[1:]
+            chosen.update(first_weight=first, last_weight=last)
+    ms = measure(**{**chosen, "serialize": 1})
+    if ms < best[0]:
+        chosen["serialize"] = 1
+    return {**base, **chosen}, rows
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--out", default=str(tuning.TABLE))
-    ap.add_argument("--count", type=int, default=256)
+    ap.add_argument("--count", type=int, default=512)
     args = ap.parse_args()
     if not _core.cuda_available(0):
         raise SystemExit("no supported gfx950 GPU visible")
@@ -116,15 +133,7 @@ def main():
     xs, fs = config3(seed=2003, count=args.count)
     audio = sum(len(x) for x in xs) / fs
     log(f"calibration batch: {len(xs)} utterances, {audio:.0f} audio-s")
-    batch, bucket_rows = tune_buckets(xs, fs)
-    tuning.set_batch(**batch)
-    depth_rows = []
-    for depth in (2, 3, 4):
-        tuning.set_batch(depth=depth)
-        ms = batch_wall(xs, fs, 5)
-        depth_rows.append({"depth": depth, "ms": ms})
-        log(f"depth {depth}: {ms:.1f} ms")
-    batch["depth"] = min(depth_rows, key=lambda r: r["ms"])["depth"]
+    batch, pipeline_rows = tune_pipeline(xs, fs)
     tuning.set_batch(**batch)
     graph_rows = []
     for graphs in (0, 1):
@@ -138,8 +147,7 @@ def main():
         "batch": batch,
         "calibration": {"utterances": len(xs), "audio_seconds": audio, "fs": fs, "seed": 2003,
                         "seconds_to_tune": time.perf_counter() - t0},
-        "measured": {"fft_split": fft_rows, "buckets": bucket_rows, "depth": depth_rows,
-                     "graphs": graph_rows},
+        "measured": {"fft_split": fft_rows, "pipeline": pipeline_rows, "graphs": graph_rows},
     }
     Path(args.out).write_text(json.dumps(table, indent=1) + "\n")
     log(f"wrote {args.out}: backend {table['backend']}, batch {batch}")
