@@ -47,7 +47,9 @@ METRIC = "audio-sec/sec on 1 MI355X, batched vs plain, same V/UV, ≤1e-6 rel F0
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 KERNEL_KEYS = ["kernel_decimate", "kernel_filterbank", "kernel_interp", "kernel_refine",
-               "kernel_prune_compact"]
+               "kernel_prune_compact", "kernel_refine_dft"]
+WORK_KEYS = {"window_samples": "work_window_samples", "bin_samples": "work_bin_samples",
+             "table_pairs": "work_table_pairs"}
 KERNEL_NAMES = {"kernel_refine": "k_refine_waves", "kernel_filterbank": "k_filter_fft+k_filter_events",
                 "kernel_interp": "k_tile_scan+k_interp", "kernel_decimate": "k_iir_*+k_carry_*",
                 "kernel_prune_compact": "k_prune_best+scan+k_compact"}
@@ -119,29 +121,42 @@ def filterbank_flops(lengths, fs, fft_min_half, floor=71.0, ceil=800.0):
 
 
 def roofline(kernels_ms, work, launches, workload, lengths, fs):
-    """Roofline objects of the refinement and filterbank kernels, and of
-    the one with the larger time per step.  Both are bound by FP64 vector
-    FMAs (no MFMA: neither is a dense contraction), against the 78.6 TFLOP/s
-    FP64 vector peak (MI355X's FP64 matrix peak is the same).  The
-    refinement's algorithmic FLOPs are 8 per bin-sample (a Goertzel step, an
-    FMA and a subtraction, for the main and the derivative window) plus 17
-    per window sample, counted on the device; the filterbank's are
-    filterbank_flops().  `launches` is the analyses per step (one per bucket):
-    a launch here is one analysis' refinement kernel, or its filterbank
-    stage (tile spectra, FFT bands, direct bands)."""
+    """Roofline objects of the refinement's DFT kernel and of the filterbank,
+    and of the one with the larger time per step.  Both are bound by FP64
+    vector FMAs (no MFMA: neither is a dense contraction), against the
+    78.6 TFLOP/s FP64 vector peak (MI355X's FP64 matrix peak is the same).
+    At working rates that are multiples of 1000 Hz the refinement runs the
+    bin-table path, whose DFT kernel k_bt_dft computes, per (frame, half
+    window) pair, every bin of that half window's table by a Goertzel
+    recurrence over the main- and derivative-windowed samples: 6 FLOP per
+    bin-sample (an FMA and a subtraction per signal, an FMA counted as 2)
+    plus 2 per window sample (the two window products), counted on the
+    device; it is timed alone.  Other rates run k_refine_waves: 8 FLOP per
+    bin-sample plus 17 per window sample, timed with its small set-up
+    kernels.  The filterbank's FLOPs are filterbank_flops().  `launches` is
+    the analyses per step (one per bucket): a launch here is one analysis'
+    refinement DFT kernel, or its filterbank stage (tile spectra, FFT bands,
+    direct bands)."""
     from harvest_cuda import tuning
     fft_min = tuning.current()["backend"]["fft_min_half"]
     out = {}
-    per = kernels_ms["kernel_refine"] / launches
-    flops = (8.0 * work["bin_samples"] + 17.0 * work["window_samples"]) / launches
+    if work["table_pairs"] > 0:
+        per = kernels_ms["kernel_refine_dft"] / launches
+        flops = (6.0 * work["bin_samples"] + 2.0 * work["window_samples"]) / launches
+        kernel, model = "k_bt_dft", "6 FLOP per bin-sample + 2 FLOP per window sample"
+    else:
+        per = kernels_ms["kernel_refine"] / launches
+        flops = (8.0 * work["bin_samples"] + 17.0 * work["window_samples"]) / launches
+        kernel, model = "k_refine_waves", "8 FLOP per bin-sample + 17 FLOP per window sample"
     ach = flops / (per / 1000.0) / 1e12
     out["kernel_refine"] = {
         "bound": "fp64_valu", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-        "frac": ach / FP64_PEAK_TFLOPS, "traffic": measured_traffic("k_refine_waves", workload),
-        "kernel": "k_refine_waves", "avg_launch_ms": per, "launches_per_step": launches,
+        "frac": ach / FP64_PEAK_TFLOPS, "traffic": measured_traffic(kernel, workload),
+        "kernel": kernel, "avg_launch_ms": per, "launches_per_step": launches,
         "flop_per_launch": flops, "bin_samples_per_launch": work["bin_samples"] / launches,
         "window_samples_per_launch": work["window_samples"] / launches,
-        "model": "8 FLOP per bin-sample + 17 FLOP per window sample"}
+        "refinement_stage_ms_per_launch": kernels_ms["kernel_refine"] / launches,
+        "model": model}
     per = kernels_ms["kernel_filterbank"] / launches
     flops = filterbank_flops(lengths, fs, fft_min) / launches
     ach = flops / (per / 1000.0) / 1e12
@@ -153,7 +168,8 @@ def roofline(kernels_ms, work, launches, workload, lengths, fs):
         "launches_per_step": launches, "flop_per_launch": flops,
         "model": f"FFT bands (h >= {fft_min}): 2.5 n log2 n per 4096-point transform + 6 FLOP "
                  "per bin; direct bands: 2 FLOP per tap per output sample"}
-    name = max(("kernel_refine", "kernel_filterbank"), key=lambda k: kernels_ms[k])
+    name = max(("kernel_refine", "kernel_filterbank"),
+               key=lambda k: out[k]["avg_launch_ms"])
     return out[name], out
 
 
@@ -233,7 +249,7 @@ def batched_leg(args, result):
     gc.collect()
     gc.freeze()
     kernels = {k: 0.0 for k in KERNEL_KEYS}
-    work = {"window_samples": 0.0, "bin_samples": 0.0}
+    work = {k: 0.0 for k in WORK_KEYS}
     device_ms, launches = 0.0, 0
     steps_ms = []
     t0 = time.perf_counter()
@@ -248,8 +264,8 @@ def batched_leg(args, result):
         last = now
         for k in KERNEL_KEYS:
             kernels[k] += tm[k]
-        work["window_samples"] += tm["work_window_samples"]
-        work["bin_samples"] += tm["work_bin_samples"]
+        for k, key in WORK_KEYS.items():
+            work[k] += tm[key]
         device_ms += tm["preprocess"] + tm["filter_events"] + tm["refine"]
         launches += int(tm.get("groups", 1))
     elapsed = time.perf_counter() - t0
@@ -332,15 +348,15 @@ def config2_leg(args):
         _core.harvest_device(ptr, [len(x)], fs, timings=True)
     steps = max(args.steps, 10)
     kernels = {k: 0.0 for k in KERNEL_KEYS}
-    work = {"window_samples": 0.0, "bin_samples": 0.0}
+    work = {k: 0.0 for k in WORK_KEYS}
     device_ms = 0.0
     t0 = time.perf_counter()
     for _ in range(steps):
         out, tm = _core.harvest_device(ptr, [len(x)], fs, timings=True)
         for k in KERNEL_KEYS:
             kernels[k] += tm[k]
-        work["window_samples"] += tm["work_window_samples"]
-        work["bin_samples"] += tm["work_bin_samples"]
+        for k, key in WORK_KEYS.items():
+            work[k] += tm[key]
         device_ms += tm["preprocess"] + tm["filter_events"] + tm["refine"]
     hbm_ms = (time.perf_counter() - t0) / steps * 1000.0
     for _ in range(2):

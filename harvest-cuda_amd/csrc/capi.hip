@@ -312,12 +312,10 @@ struct Plan {
 
 hipStream_t s_of(hv_workspace* ws) { return ws->stream; }
 
-// Records `launch` between two events attributed to timing slot `slot`.
-template <class F> void timed(hv_workspace* ws, int slot, F&& launch) {
-  if (!ws->timing) {
-    launch();
-    return;
-  }
+// Two events attributed to timing slot `slot`, for the caller to record
+// around a launch on the workspace's stream; a null pair when not timing.
+std::pair<hipEvent_t, hipEvent_t> timer_pair(hv_workspace* ws, int slot) {
+  if (!ws->timing) return {nullptr, nullptr};
   if (ws->timers_used + 2 > ws->timer_events.size()) {
     for (int k = 0; k < 16; ++k) {
       hipEvent_t e;
@@ -328,9 +326,15 @@ template <class F> void timed(hv_workspace* ws, int slot, F&& launch) {
   hipEvent_t a = ws->timer_events[ws->timers_used], b = ws->timer_events[ws->timers_used + 1];
   ws->timers_used += 2;
   ws->timer_slots.push_back(slot);
-  check(hipEventRecord(a, ws->stream), "hipEventRecord");
+  return {a, b};
+}
+
+// Records `launch` between two events attributed to timing slot `slot`.
+template <class F> void timed(hv_workspace* ws, int slot, F&& launch) {
+  auto ev = timer_pair(ws, slot);
+  if (ev.first) check(hipEventRecord(ev.first, ws->stream), "hipEventRecord");
   launch();
-  check(hipEventRecord(b, ws->stream), "hipEventRecord");
+  if (ev.second) check(hipEventRecord(ev.second, ws->stream), "hipEventRecord");
 }
 
 void start_timers(hv_workspace* ws, bool on) {
@@ -869,12 +873,13 @@ void enqueue_analysis(hv_workspace* ws, const Plan& p, const Bufs& b, bool event
   if (events) check(hipEventRecord(ws->ev[2], s), "hipEventRecord");
   launch_fill_int(s, reinterpret_cast<int*>(b.work), 2 * kWorkWords, 0);
   timed(ws, 6, [&] {
-    if (c.bt_hr > 0)
+    if (c.bt_hr > 0) {
+      auto dft = timer_pair(ws, 8);
       launch_refine_table(s, c, dutt, ws->d_f_off, p.total_frames, b.y, b.official, b.count,
                           BinTable{ws->bt_info.get<int4>(), ws->bt_harm.get<int4>(),
                                    ws->bt_win.get<double2>(), ws->bt_bins.get<double>()},
-                          b.bt, b.rf0, b.rsc, b.work);
-    else
+                          b.bt, b.rf0, b.rsc, b.work, dft.first, dft.second);
+    } else
       launch_refine(s, c, dutt, ws->d_f_off, p.total_frames, b.y, b.official, b.count,
                     ws->twiddle.get<double>(), ws->wintab.get<double>(),
                     c.win_pairs ? ws->winpair.get<double2>() : nullptr, b.rf0, b.rsc, b.work);

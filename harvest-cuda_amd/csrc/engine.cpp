@@ -760,7 +760,8 @@ using Clock = std::chrono::steady_clock;
 void record_device(const double* ms, Timings& t) {
   static const char* names[HV_TIMING_SLOTS] = {
       "preprocess",          "filter_events",  "refine",        "kernel_decimate",
-      "kernel_filterbank",   "kernel_interp",  "kernel_refine", "kernel_prune_compact"};
+      "kernel_filterbank",   "kernel_interp",  "kernel_refine", "kernel_prune_compact",
+      "kernel_refine_dft"};
   for (int k = 0; k < HV_TIMING_SLOTS; ++k) t[names[k]] = ms[k];
 }
 

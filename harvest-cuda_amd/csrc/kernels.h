@@ -131,7 +131,8 @@ void launch_bt_tables(hipStream_t s, const Config& c, const BinTable& t, const i
 void launch_refine_table(hipStream_t s, const Config& c, const Utt* utt, const long long* f_off,
                          long long total_frames, const double* y, const double* official,
                          const int* count, const BinTable& t, const BtBufs& b, double* rf0,
-                         double* rscore, unsigned long long* work);
+                         double* rscore, unsigned long long* work, hipEvent_t dft_begin = nullptr,
+                         hipEvent_t dft_end = nullptr);
 
 void launch_prune_best(hipStream_t s, const Config& c, const Utt* utt,
                        const long long* f_off, long long total_frames,

@@ -51,6 +51,12 @@ void check_launch(const char* what) {
                              "): " + hipGetErrorString(e));
 }
 
+void record(hipEvent_t ev, hipStream_t s) {
+  hipError_t e = hipEventRecord(ev, s);
+  if (e != hipSuccess)
+    throw std::runtime_error(std::string("hipEventRecord: ") + hipGetErrorString(e));
+}
+
 __device__ __forceinline__ unsigned long long readlane_u64(unsigned long long v, int l) {
   const unsigned lo = __builtin_amdgcn_readlane((unsigned)v, l);
   const unsigned hi = __builtin_amdgcn_readlane((unsigned)(v >> 32), l);
@@ -661,7 +667,8 @@ long long bt_blocks(long long total_frames) { return (total_frames + 63) / 64; }
 void launch_refine_table(hipStream_t s, const Config& c, const Utt* utt, const long long* f_off,
                          long long total_frames, const double* y, const double* official,
                          const int* count, const BinTable& t, const BtBufs& b, double* rf0,
-                         double* rscore, unsigned long long* work) {
+                         double* rscore, unsigned long long* work, hipEvent_t dft_begin,
+                         hipEvent_t dft_end) {
   if (c.bt_hr <= 0 || c.bt_hr > kBtMaxHr) throw std::runtime_error("refine_table: no bin table");
   if (total_frames <= 0) return;
   const long long nblk = bt_blocks(total_frames);
@@ -683,9 +690,11 @@ void launch_refine_table(hipStream_t s, const Config& c, const Utt* utt, const l
   // frame, 64 per chunk, a partial chunk per h).
   long long groups = (total_frames * kBtPairs / 64 + c.bt_hr + 3) / 4;
   groups = groups < 64 ? 64 : (groups > kBtDftGroups ? kBtDftGroups : groups);
+  if (dft_begin) record(dft_begin, s);
   k_bt_dft<<<(unsigned)groups, kThreads, 0, s>>>(c, utt, y, b.list, b.cinfo, b.sched, t, b.res,
                                                  work);
   check_launch("bt_dft");
+  if (dft_end) record(dft_end, s);
   k_bt_finish<<<(unsigned)((total_frames + 3) / 4), kThreads, 0, s>>>(
       c, utt, f_off, total_frames, y, official, b.codes, b.inv, count, b.needs, t, b.res, rf0,
       rscore);

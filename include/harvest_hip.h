@@ -43,9 +43,11 @@ typedef enum hv_status {
  *   [2] refine         refinement, pruning, compaction, download
  * Kernel sums inside those spans:
  *   [3] decimation scan kernels      [4] filterbank + crossing kernel
- *   [5] tile scan + interpolation    [6] refinement kernel
- *   [7] prune, best, scan, compaction */
-#define HV_TIMING_SLOTS 8
+ *   [5] tile scan + interpolation    [6] refinement kernels
+ *   [7] prune, best, scan, compaction
+ *   [8] bin-table DFT kernel alone (part of [6]; 0 when the per-frame
+ *       refinement ran) */
+#define HV_TIMING_SLOTS 9
 
 /* Work counters of the last analysis call (hv_last_stats):
  *   [0] refinement window samples computed (distinct (frame, h) windows)
