@@ -163,7 +163,7 @@ def roofline(kernels_ms, work, launches, workload, lengths, fs):
     out["kernel_filterbank"] = {
         "bound": "fp64_valu", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
         "frac": ach / FP64_PEAK_TFLOPS,
-        "traffic": measured_traffic("k_filter_fft+k_filter_events", workload),
+        "traffic": measured_traffic("k_spectrum+k_filter_fft+k_filter_events", workload),
         "kernel": "k_spectrum+k_filter_fft+k_filter_events", "avg_launch_ms": per,
         "launches_per_step": launches, "flop_per_launch": flops,
         "model": f"FFT bands (h >= {fft_min}): 2.5 n log2 n per 4096-point transform + 6 FLOP "
