@@ -55,9 +55,20 @@ void check(hipError_t e, const char* what) {
   Device::check(e == hipErrorMemoryAllocation ? hipErrorOutOfMemory : e, what);
 }
 
+// While set, ensure() records the size it is asked for instead of
+// allocating (reserve_batches); a buffer whose recorded size exceeds its
+// allocation grows to that size, once, at its next ensure().
+thread_local bool g_recording = false;
+
+struct Recording {
+  Recording() { g_recording = true; }
+  ~Recording() { g_recording = false; }
+};
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  size_t want = 0;  // largest size recorded by a reservation
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
@@ -69,7 +80,12 @@ struct DevBuf {
   }
   template <class T> T* ensure(size_t n) {
     size_t need = std::max<size_t>(n, 1) * sizeof(T);
-    if (need > bytes) {
+    if (g_recording) {
+      want = std::max(want, need);
+      return static_cast<T*>(p);
+    }
+    if (need > bytes || want > bytes) {
+      need = std::max(need, want);
       release();
       check(hipMalloc(&p, need), "hipMalloc");
       bytes = need;
@@ -82,6 +98,7 @@ struct DevBuf {
 struct PinnedBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  size_t want = 0;  // largest size recorded by a reservation
   PinnedBuf() = default;
   PinnedBuf(const PinnedBuf&) = delete;
   PinnedBuf& operator=(const PinnedBuf&) = delete;
@@ -93,7 +110,12 @@ struct PinnedBuf {
   }
   template <class T> T* ensure(size_t n) {
     size_t need = std::max<size_t>(n, 1) * sizeof(T);
-    if (need > bytes) {
+    if (g_recording) {
+      want = std::max(want, need);
+      return static_cast<T*>(p);
+    }
+    if (need > bytes || want > bytes) {
+      need = std::max(need, want);
       release();
       check(hipHostMalloc(&p, need, hipHostMallocDefault), "hipHostMalloc");
       bytes = need;
@@ -689,10 +711,13 @@ void layout(Plan& p, const std::vector<long long>& lengths, const std::vector<in
 }
 
 // Upload the plan's descriptor arrays through one pinned staging buffer.
+constexpr int kPlanArrays = 6;
+size_t plan_bytes(int U) { return U * sizeof(Utt) + kPlanArrays * (U + 1) * sizeof(long long); }
+
 void upload_plan(hv_workspace* ws, const Plan& p) {
   const int U = p.cfg.count;
-  constexpr int kArrays = 6;
-  const size_t bytes = U * sizeof(Utt) + kArrays * (U + 1) * sizeof(long long);
+  constexpr int kArrays = kPlanArrays;
+  const size_t bytes = plan_bytes(U);
   char* h = ws->h_meta.ensure<char>(bytes);
   std::memcpy(h, p.utts.data(), U * sizeof(Utt));
   size_t at = U * sizeof(Utt);
@@ -1118,9 +1143,9 @@ void check_common(int fs, double f0_floor, double f0_ceil) {
 // already points to the concatenated batch in device memory.
 // Plan, upload and preprocessing shared by the analysis and the frozen-decision
 // entry point: on return the decimated, mean-removed y is on the device.
-Plan stage_batch(hv_workspace* ws, int count, const double* const* xs, const int64_t* lengths,
-                 int fs, double f0_floor, double f0_ceil, double* timings, bool device_x,
-                 const double** staged) {
+// Validation, the per-rate tables and the layout of a batch.
+Plan plan_batch(hv_workspace* ws, int count, const int64_t* lengths, int fs, double f0_floor,
+                double f0_ceil) {
   if (count < 1) invalid("count must be positive");
   check_common(fs, f0_floor, f0_ceil);
   const int ratio = std::max(std::min(host_round(fs / 8000.0), 12), 1);
@@ -1145,6 +1170,13 @@ Plan stage_batch(hv_workspace* ws, int count, const double* const* xs, const int
   p.cfg.lds_limit = ws->lds_limit;
   layout(p, lens, ns, ffts, frames);
   prepare_phase(ws, p);
+  return p;
+}
+
+Plan stage_batch(hv_workspace* ws, int count, const double* const* xs, const int64_t* lengths,
+                 int fs, double f0_floor, double f0_ceil, double* timings, bool device_x,
+                 const double** staged) {
+  Plan p = plan_batch(ws, count, lengths, fs, f0_floor, f0_ceil);
   hipStream_t s = ws->stream;
   start_timers(ws, timings != nullptr);
   check(hipEventRecord(ws->ev[0], s), "hipEventRecord");
@@ -1223,6 +1255,30 @@ void submit_batch(hv_workspace* ws, int count, const double* const* xs, const in
     ws->after_pending = false;
   }
   analyze_submit(ws, p, dx, true);
+}
+
+// Records the buffer sizes of each group's analysis (two-step form), so that
+// the next submission grows every buffer once to the largest of them.
+void reserve_batches(hv_workspace* ws, int groups, const int* counts, const int64_t* lengths,
+                     int fs, double f0_floor, double f0_ceil, bool host_input) {
+  if (!ws) invalid("null workspace");
+  if (groups < 1) invalid("groups must be positive");
+  DeviceGuard guard(ws->device);
+  long long at = 0;
+  for (int g = 0; g < groups; ++g) {
+    if (counts[g] < 1) invalid("count must be positive");
+    const Plan p = plan_batch(ws, counts[g], lengths + at, fs, f0_floor, f0_ceil);
+    at += counts[g];
+    Recording rec;
+    ws->h_meta.ensure<char>(plan_bytes(p.cfg.count));
+    ws->plan.ensure<char>(plan_bytes(p.cfg.count));
+    if (host_input) {
+      ws->h_x.ensure<double>(p.total_x);
+      ws->x.ensure<double>(p.total_x);
+    }
+    Bufs b = size_buffers(ws, p, nullptr);
+    size_candidate_copy(ws, p, b);
+  }
 }
 
 // Frozen-decision refinement of entries (value and/or adjoint), then the
@@ -1548,6 +1604,16 @@ hv_status hv_analyze_batch_device_submit(hv_workspace* ws, int count, const doub
     if (!x_device || !lengths) invalid("null pointer");
     const double* xs[1] = {x_device};
     submit_batch(ws, count, xs, lengths, fs, f0_floor, f0_ceil, timings != 0, true);
+  });
+}
+
+hv_status hv_reserve_batches(hv_workspace* ws, int groups, const int* counts,
+                             const int64_t* lengths, int fs, double f0_floor, double f0_ceil,
+                             int host_input) {
+  return guarded([&] {
+    if (!counts || !lengths) invalid("null pointer");
+    if (ws && ws->submitted) invalid("the workspace has a submitted analysis to collect first");
+    reserve_batches(ws, groups, counts, lengths, fs, f0_floor, f0_ceil, host_input != 0);
   });
 }
 

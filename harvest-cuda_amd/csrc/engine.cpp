@@ -954,6 +954,25 @@ Buckets make_buckets(const std::vector<int64_t>& frames, int groups, bool by_len
   return b;
 }
 
+// Sizes each workspace of a pipelined batch for the largest of the groups it
+// will run (group g runs on ws[g % depth]) before any is submitted, so that
+// the submissions neither allocate nor free.
+void reserve_groups(const std::vector<hv_workspace*>& ws, const Buckets& bk,
+                    const std::vector<int64_t>& lengths, int fs, double f0_floor,
+                    double f0_ceil, bool host_input) {
+  const int groups = (int)bk.bounds.size() - 1, depth = (int)ws.size();
+  for (int w = 0; w < depth && w < groups; ++w) {
+    std::vector<int> counts;
+    std::vector<int64_t> lens;
+    for (int g = w; g < groups; g += depth) {
+      counts.push_back(bk.bounds[g + 1] - bk.bounds[g]);
+      for (int k = bk.bounds[g]; k < bk.bounds[g + 1]; ++k) lens.push_back(lengths[bk.order[k]]);
+    }
+    throw_status(hv_reserve_batches(ws[w], (int)counts.size(), counts.data(), lens.data(), fs,
+                                    f0_floor, f0_ceil, host_input ? 1 : 0));
+  }
+}
+
 // Adds one group's device spans, kernel sums and work counters to `t`.
 void accumulate_device(hv_workspace* ws, const double* ms, Timings& t) {
   Timings one;
@@ -1084,6 +1103,7 @@ std::vector<Result> analyze_batch(hv_workspace* ws, const std::vector<const doub
     };
     std::vector<hv_workspace*> all{ws};
     all.insert(all.end(), more.begin(), more.end());
+    reserve_groups(all, bk, lengths, fs, f0_floor, f0_ceil, true);
     return batch_pipelined(all, submit, lengths, bk, fs, f0_floor, f0_ceil, frame_period,
                            host_threads, timings);
   }
@@ -1114,6 +1134,7 @@ std::vector<Result> analyze_batch_device(hv_workspace* ws, const double* x_devic
     };
     std::vector<hv_workspace*> all{ws};
     all.insert(all.end(), more.begin(), more.end());
+    reserve_groups(all, bk, lengths, fs, f0_floor, f0_ceil, false);
     return batch_pipelined(all, submit, lengths, bk, fs, f0_floor, f0_ceil, frame_period,
                            host_threads, timings);
   }

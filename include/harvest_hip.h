@@ -216,6 +216,19 @@ hv_status hv_analyze_batch_device_submit(hv_workspace* ws, int count, const doub
 hv_status hv_analyze_batch_collect(hv_workspace* ws, int32_t* offsets, double* best,
                                    int64_t* nnz, double* timings_ms);
 
+/* Sizes every buffer of `ws` for the largest of `groups` batches at once:
+ * group k holds counts[k] utterances, whose lengths follow those of group
+ * k - 1 in `lengths`; host_input = 1 when they will come from host memory
+ * (the _submit form), 0 for device input.  The buffers grow on the next
+ * submission, once, to the largest size any group needs, so that submitting
+ * the groups afterwards in any order neither allocates nor frees device or
+ * pinned memory (a freed device buffer is cleared by the driver in the
+ * background, which slows the device work that follows for a while).
+ * Validates like hv_analyze_batch.  No reference counterpart. */
+hv_status hv_reserve_batches(hv_workspace* ws, int groups, const int* counts,
+                             const int64_t* lengths, int fs, double f0_floor, double f0_ceil,
+                             int host_input);
+
 /* The device work of the next analysis submitted on `ws` (after its input
  * uploads) starts only when everything enqueued on `before` so far has
  * finished: the host engine uses it to run the groups of a pipelined batch

@@ -10,6 +10,7 @@
 // utterance's samples: voiced sections with a smooth track, near-duplicate
 // and octave candidates, and sparse unvoiced candidates, which exercise
 // every contour stage.  A workspace can be told to fail its next submit.
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -40,6 +41,9 @@ struct hv_workspace {
   Batch pending;
   bool submitted = false;
   int fail_next_submit = 0;
+  // Length lists of the groups of the last hv_reserve_batches call: every
+  // later submission must be one of them.
+  std::vector<std::vector<int64_t>> reserved;
   std::thread download;
   std::mutex mutex;
   std::condition_variable cv;
@@ -132,6 +136,14 @@ hv_status run(hv_workspace* ws, const Batch& b, int32_t* offsets, double* best, 
   return HV_OK;
 }
 
+// True when no reservation was made or (lengths, count) is one of its groups.
+bool reserved_group(const hv_workspace* ws, int count, const int64_t* lengths) {
+  if (ws->reserved.empty()) return true;
+  for (const auto& g : ws->reserved)
+    if ((int)g.size() == count && std::equal(g.begin(), g.end(), lengths)) return true;
+  return false;
+}
+
 Batch batch_of(int count, const double* const* xs, const int64_t* lengths, int fs) {
   Batch b;
   b.xs.assign(xs, xs + count);
@@ -190,6 +202,7 @@ hv_status hv_analyze_batch_submit(hv_workspace* ws, int count, const double* con
     --ws->fail_next_submit;
     return fail(HV_EDEVICE, "injected device failure");
   }
+  if (!reserved_group(ws, count, lengths)) return fail(HV_EINVAL, "submitted group was not reserved");
   ws->pending = batch_of(count, xs, lengths, fs);
   ws->submitted = true;
   return HV_OK;
@@ -202,8 +215,21 @@ hv_status hv_analyze_batch_device_submit(hv_workspace* ws, int count, const doub
     --ws->fail_next_submit;
     return fail(HV_EDEVICE, "injected device failure");
   }
+  if (!reserved_group(ws, count, lengths)) return fail(HV_EINVAL, "submitted group was not reserved");
   ws->pending = device_batch(count, x_device, lengths, fs);
   ws->submitted = true;
+  return HV_OK;
+}
+
+hv_status hv_reserve_batches(hv_workspace* ws, int groups, const int* counts,
+                             const int64_t* lengths, int, double, double, int) {
+  if (!ws || ws->submitted) return fail(HV_EINVAL, "bad workspace state");
+  if (groups < 1 || !counts || !lengths) return fail(HV_EINVAL, "bad groups");
+  ws->reserved.clear();
+  for (int g = 0; g < groups; ++g) {
+    ws->reserved.emplace_back(lengths, lengths + counts[g]);
+    lengths += counts[g];
+  }
   return HV_OK;
 }
 
