@@ -1255,7 +1255,9 @@ __global__ __launch_bounds__(kThreads) void k_official(Config c, const long long
 
 // ---------------------------------------------------------------------------
 // Refinement (GetRefinedF0 harvest.cpp:589-617, GetMeanF0 :541-584, FixF0
-// :507-536), one wavefront per 1 ms frame, four frames per workgroup.
+// :507-536), one wavefront per 1 ms frame, kRefineWaves frames per
+// workgroup.  Analyses whose frames start at whole samples take the
+// bin-table path (refine_table.hip) instead.
 //
 // Slot s < 7 count of a frame holds the overlapped candidate of
 // OverlapF0Candidates (harvest.cpp:417-429).  A slot's windows depend only on
