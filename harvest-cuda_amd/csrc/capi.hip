@@ -606,6 +606,7 @@ void prepare_bintable(hv_workspace* ws, double fs_work, double f0_floor, double 
   check(hipMemcpyAsync(dkq, kq.data(), kq.size() * sizeof(int), hipMemcpyHostToDevice,
                        ws->stream), "upload bin tables");
   Config c{};
+  c.fs_work = fs_work;
   c.bt_hlo = hlo;
   c.bt_hr = hr;
   c.bt_nbmax = nbmax;

@@ -118,13 +118,23 @@ __device__ __forceinline__ double2 goertzel_result(double a1, double a2, double 
   return make_double2(mre * mre + mim * mim, mre * dim - mim * dre);
 }
 
+// FixF0's amplitude sqrt(P) and instantaneous frequency k fs / n + Nu / P
+// fs / (2 pi) (harvest.cpp:515-522; 0 where P is 0) of one bin from its
+// (power, IF numerator) and k fs / n (the record's entry 5).  They depend
+// on the (frame, h, k) alone, so every slot that reads the bin shares them.
+__device__ __forceinline__ double2 bin_amp_if(double2 pn, double kfn, double fs) {
+  constexpr double kInvTwoPi = 0.5 / kPi;
+  const double ifr = pn.x == 0.0 ? 0.0 : kfn + pn.y / pn.x * fs * kInvTwoPi;
+  return make_double2(sqrt(pn.x), ifr);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
 // Table construction, one workgroup per half window h: the window rows
 // (w[m], d[m]) (window_pair) and, per bin k of h, the record (2 cos w,
-// cos w, sin w, cos w(2h), sin w(2h)) of w = 2 pi k / n, each from sincospi
-// of the exact rational argument.
+// cos w, sin w, cos w(2h), sin w(2h), k fs / n) of w = 2 pi k / n, the
+// trigonometric entries from sincospi of the exact rational argument.
 __global__ __launch_bounds__(kThreads) void k_bt_tables(Config c, BinTable t, const int* kq,
                                                         double2* win, double* bins) {
   const int hb = blockIdx.x;
@@ -142,7 +152,8 @@ __global__ __launch_bounds__(kThreads) void k_bt_tables(Config c, BinTable t, co
     r[2] = sn;
     r[3] = cl;
     r[4] = sl;
-    for (int e = 5; e < kBtRecord; ++e) r[e] = 0.0;
+    r[5] = (double)k * c.fs_work * ldexp(1.0, -window_fft_log2(h));  // 1 / n exact
+    for (int e = 6; e < kBtRecord; ++e) r[e] = 0.0;
   }
 }
 
@@ -421,7 +432,8 @@ __device__ __forceinline__ void bt_chunk(bool clamp, const double* yu, long long
                                          long long last, int len,
                                          const CONSTANT_AS double* win,
                                          const CONSTANT_AS double* rec, LDS double* buf0,
-                                         LDS double* buf1, int lane, double2* out, bool act) {
+                                         LDS double* buf1, int lane, double2* out, bool act,
+                                         double fs) {
   Goertzel<NQ> g;
 #pragma unroll
   for (int q = 0; q < NQ; ++q) g.c2[q] = rec[q * kBtRecord];
@@ -431,7 +443,9 @@ __device__ __forceinline__ void bt_chunk(bool clamp, const double* yu, long long
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const CONSTANT_AS double* r = rec + q * kBtRecord;
-      out[q] = goertzel_result(g.a2[q], g.a1[q], g.b2[q], g.b1[q], r[1], r[2], r[3], r[4]);
+      out[q] = bin_amp_if(goertzel_result(g.a2[q], g.a1[q], g.b2[q], g.b1[q], r[1], r[2], r[3],
+                                          r[4]),
+                          r[5], fs);
     }
 }
 
@@ -441,8 +455,8 @@ __device__ __forceinline__ void bt_chunk(bool clamp, const double* yu, long long
 // consecutive chunks at a time from a counter until none is left (a
 // persistent grid): its waves then mostly share one h, so that they stream
 // the same table rows through the scalar cache together.
-// res[(frame kBtPairs + rank) bt_nbmax + q] receives (power, IF numerator)
-// of the pair's bin q.
+// res[(frame kBtPairs + rank) bt_nbmax + q] receives (amplitude,
+// instantaneous frequency) of the pair's bin q (bin_amp_if).
 __global__ __launch_bounds__(kThreads) void k_bt_dft(
     Config c, const Utt* utt, const double* y, const int2* list, const int* cinfo, int* sched,
     BinTable t, double2* res, unsigned long long* work) {
@@ -506,7 +520,8 @@ __global__ __launch_bounds__(kThreads) void k_bt_dft(
       const CONSTANT_AS double* rec = rec0 + (long long)q0 * kBtRecord;
 #define BT_CHUNK(NQ)                                                                        \
   case NQ:                                                                                 \
-    bt_chunk<NQ>(clamp, yu, first, last, len, win, rec, buf0, buf1, lane, out + q0, act); \
+    bt_chunk<NQ>(clamp, yu, first, last, len, win, rec, buf0, buf1, lane, out + q0, act, \
+                 c.fs_work);                                                                \
     break;
       switch (nq) {
         BT_CHUNK(1) BT_CHUNK(2) BT_CHUNK(3) BT_CHUNK(4)
@@ -535,7 +550,7 @@ namespace {
 // Bin q of half window h at frame `first` computed alone, by the same
 // operations as k_bt_dft (frames with more than kBtPairs windows).
 __device__ double2 bt_bin_direct(const double* yu, long long first, long long last, int h,
-                                 const double2* win, const double* rec) {
+                                 const double2* win, const double* rec, double fs) {
   const int len = 2 * h + 1;
   Goertzel<1> g;
   g.c2[0] = rec[0];
@@ -552,7 +567,9 @@ __device__ double2 bt_bin_direct(const double* yu, long long first, long long la
   }
   const double yv = at(m);
   g.step_last(yv * win[m].x, yv * win[m].y);
-  return goertzel_result(g.a2[0], g.a1[0], g.b2[0], g.b1[0], rec[1], rec[2], rec[3], rec[4]);
+  return bin_amp_if(
+      goertzel_result(g.a2[0], g.a1[0], g.b2[0], g.b1[0], rec[1], rec[2], rec[3], rec[4]), rec[5],
+      fs);
 }
 
 }  // namespace
@@ -597,7 +614,6 @@ __global__ __launch_bounds__(kThreads) void k_bt_finish(
   double* out_sc = rscore + gf * c.slots;
   const double2* row = res + gf * kBtPairs * c.bt_nbmax;
   const int spf = (int)(fs / 1000.0);
-  constexpr double kInvTwoPi = 0.5 / kPi;
   for (int s = lane; s < ncand; s += 64) {
     const int g = s / cnt_u, jj = s - g * cnt_u;
     const long long src = overlap_source(i, g);
@@ -615,7 +631,6 @@ __global__ __launch_bounds__(kThreads) void k_bt_finish(
             hb >= lo + 64 ? ~0ull : (hb <= lo ? 0ull : ((1ull << (hb - lo)) - 1ull));
         r += __popcll(nw[w] & m);
       }
-      const double inv_n = ldexp(1.0, -window_fft_log2(h));  // 1 / n, exact
       const double2* pr = row + (long long)r * c.bt_nbmax;
       double numer = 0.0, den = 0.0, dev = 0.0;
       for (int hh = 0; hh < H; ++hh) {
@@ -627,15 +642,13 @@ __global__ __launch_bounds__(kThreads) void k_bt_finish(
           const int4 info = t.info[hb];
           pn = bt_bin_direct(y + utt[ui].y_off, (long long)spf * i - h - 1,
                              (long long)utt[ui].n - 1, h, t.win + info.z,
-                             t.bins + (long long)(info.w + q) * kBtRecord);
+                             t.bins + (long long)(info.w + q) * kBtRecord, fs);
         } else {
           pn = pr[q];
         }
-        const double pw = pn.x, nu = pn.y;
+        const double amp = pn.x, ifr = pn.y;
         const double inv_h = hh == 0 ? 1.0 : hh == 1 ? 0.5 : hh == 2 ? 1.0 / 3.0
                            : hh == 3 ? 0.25 : hh == 4 ? 0.2 : 1.0 / 6.0;
-        const double ifr = pw == 0.0 ? 0.0 : (double)k * fs * inv_n + nu / pw * fs * kInvTwoPi;
-        const double amp = sqrt(pw);
         numer += amp * ifr;
         den += amp * (hh + 1.0);
         dev += fabs((ifr * inv_h - f0) * inv_f0);
