@@ -39,7 +39,7 @@ struct ExtraLeases {
 // Backend tuning applied to every workspace the cache creates (and to the
 // cached ones when it changes); set from the package's tuning table.
 std::mutex g_tuning_mutex;
-hv_tuning g_tuning{64, 0, 4096, 1};
+hv_tuning g_tuning{64, 0, 4096, 1, 0};
 
 hv_tuning current_tuning() {
   std::lock_guard<std::mutex> lock(g_tuning_mutex);
@@ -459,10 +459,12 @@ PYBIND11_MODULE(_core, m) {
     d["fs_work"] = p.fs_work;
     return d;
   });
-  m.def("_set_tuning", [](int fft_min_half, int graphs, int slab_budget_mib, int refine_table) {
-    const hv_tuning t{fft_min_half, graphs, slab_budget_mib, refine_table};
+  m.def("_set_tuning", [](int fft_min_half, int graphs, int slab_budget_mib, int refine_table,
+                          int refine_groups) {
+    const hv_tuning t{fft_min_half, graphs, slab_budget_mib, refine_table, refine_groups};
     if (fft_min_half < 1 || (graphs != 0 && graphs != 1) || slab_budget_mib < 64 ||
-        (refine_table != 0 && refine_table != 1))
+        (refine_table != 0 && refine_table != 1) ||
+        (refine_groups != 0 && (refine_groups < 64 || refine_groups > 4096)))
       throw std::invalid_argument("invalid tuning");
     {
       std::lock_guard<std::mutex> lock(g_tuning_mutex);
@@ -473,7 +475,7 @@ PYBIND11_MODULE(_core, m) {
     py::gil_scoped_release release;
     cache().clear();
   }, py::arg("fft_min_half"), py::arg("graphs"), py::arg("slab_budget_mib"),
-     py::arg("refine_table") = 1,
+     py::arg("refine_table") = 1, py::arg("refine_groups") = 0,
      "Backend tuning of every workspace (include/harvest_hip.h hv_tuning).");
   m.def("_get_tuning", [] {
     const hv_tuning t = current_tuning();
@@ -482,6 +484,7 @@ PYBIND11_MODULE(_core, m) {
     d["graphs"] = t.graphs;
     d["slab_budget_mib"] = t.slab_budget_mib;
     d["refine_table"] = t.refine_table;
+    d["refine_groups"] = t.refine_groups;
     return d;
   });
   m.def("_set_batch_tuning", [](int max_groups, int64_t min_group_frames, int64_t split_frames,

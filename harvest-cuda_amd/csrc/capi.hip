@@ -279,7 +279,7 @@ struct hv_workspace {
   // copies them at collect).
   double cand_per_frame = 20.0;
   long long cand_capacity = 0;  // of the last analysis
-  hv_tuning tune{kFftMinHalf, 0, (int)(kSlabBudget >> 20), 1};
+  hv_tuning tune{kFftMinHalf, 0, (int)(kSlabBudget >> 20), 1, 0};
   // hv_order_after: an event on another workspace's stream that the next
   // submitted analysis' device work waits for (after its uploads).
   hipEvent_t after_ev = nullptr;
@@ -625,6 +625,7 @@ void apply_bintable(hv_workspace* ws, Config& c) {
   c.bt_hlo = on ? ws->bt_hlo : 0;
   c.bt_hr = on ? ws->bt_hr : 0;
   c.bt_nbmax = on ? ws->bt_nbmax : 0;
+  c.bt_groups = ws->tune.refine_groups > 0 ? ws->tune.refine_groups : kBtDftGroups;
 }
 
 void fill_config(Config& c, int count, int ratio, int fs, double fs_work, double f0_floor,
@@ -644,6 +645,7 @@ void fill_config(Config& c, int count, int ratio, int fs, double fs_work, double
   c.hmax = refine_hmax(fs_work, f0_floor);
   c.win_pairs = use_window_pairs(fs_work, c.hmax) ? 1 : 0;
   c.bt_hlo = c.bt_hr = c.bt_nbmax = 0;  // apply_bintable turns the bin tables on
+  c.bt_groups = kBtDftGroups;
   const double* co = kDecimation[ratio >= 2 && ratio <= 12 ? ratio : 0];
   c.a0 = co[0];
   c.a1 = co[1];
@@ -944,7 +946,7 @@ std::vector<long long> graph_key(hv_workspace* ws, const Plan& p, const Bufs& b)
   const Config& c = p.cfg;
   std::vector<long long> k = {c.count, c.ratio, c.fs, c.bands, c.capacity, c.twiddle_n,
                               c.hmax, (long long)c.phase_n, ws->fft_bands, ws->fft_D, b.group,
-                              c.bt_hlo, c.bt_hr, c.bt_nbmax,
+                              c.bt_hlo, c.bt_hr, c.bt_nbmax, c.bt_groups,
                               p.total_x, p.total_m, p.total_y, p.total_frames, p.total_tiles,
                               p.total_chunks, p.total_pblocks, p.max_chunks};
   long long bits;
@@ -1677,6 +1679,8 @@ hv_status hv_set_tuning(hv_workspace* ws, const hv_tuning* in) {
     if (in->graphs != 0 && in->graphs != 1) invalid("graphs must be 0 or 1");
     if (in->slab_budget_mib < 64) invalid("slab_budget_mib must be at least 64");
     if (in->refine_table != 0 && in->refine_table != 1) invalid("refine_table must be 0 or 1");
+    if (in->refine_groups != 0 && (in->refine_groups < 64 || in->refine_groups > 4096))
+      invalid("refine_groups must be 0 or between 64 and 4096");
     ws->tune = *in;  // a new band split takes effect at the next call (filter cache key)
   });
 }

@@ -70,8 +70,9 @@ struct Config {
   int hmax;          // largest refinement half window (candidates >= f0_floor)
   int win_pairs;     // 1: the refinement windows depend on h alone (see k_window_pairs)
   // Bin-table refinement (refine_table.hip), on when bt_hr > 0: half windows
-  // [bt_hlo, bt_hlo + bt_hr), at most bt_nbmax bins per half window.
-  int bt_hlo, bt_hr, bt_nbmax;
+  // [bt_hlo, bt_hlo + bt_hr), at most bt_nbmax bins per half window; the
+  // DFT kernel's persistent grid has at most bt_groups workgroups.
+  int bt_hlo, bt_hr, bt_nbmax, bt_groups;
   long long phase_n; // size P of the filterbank's two-level phase table (>= every fft)
   int phase_bits;    // its fine part holds 2^phase_bits entries
   double a0, a1, a2, b0, b1;  // decimation IIR

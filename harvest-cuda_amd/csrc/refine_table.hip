@@ -702,7 +702,8 @@ void launch_refine_table(hipStream_t s, const Config& c, const Utt* utt, const l
   // analyses (at most one per four possible chunks: kBtPairs pairs per
   // frame, 64 per chunk, a partial chunk per h).
   long long groups = (total_frames * kBtPairs / 64 + c.bt_hr + 3) / 4;
-  groups = groups < 64 ? 64 : (groups > kBtDftGroups ? kBtDftGroups : groups);
+  const long long cap = c.bt_groups > 0 ? c.bt_groups : kBtDftGroups;
+  groups = groups < 64 ? 64 : (groups > cap ? cap : groups);
   if (dft_begin) record(dft_begin, s);
   k_bt_dft<<<(unsigned)groups, kThreads, 0, s>>>(c, utt, y, b.list, b.cinfo, b.sched, t, b.res,
                                                  work);

@@ -9,7 +9,9 @@ on:
   1. the FFT/direct filterbank split (fft_min_half): the filterbank kernels'
      time on one 60 s utterance at 24 kHz for each candidate split, keeping
      only splits whose F0 equals the default's to 1e-9 with identical
-     voicing;
+     voicing; then, on the same utterance, the persistent grid of the
+     bin-table refinement's DFT kernel (refine_groups), by that kernel's
+     time, keeping only grids whose F0 equals the default's bit for bit;
   2. the pipeline: how many length buckets (max_groups) and how many
      workspaces keep buckets in flight (depth 2-4), jointly, from the wall
      time of one batched call, NumPy in and out, median of 3; more buckets
@@ -17,7 +19,9 @@ on:
      the others' kernels, at the price of more, smaller launches;
   3. the bucket boundaries (first_weight, last_weight) at that pipeline;
   4. whether the buckets' device work runs one after another (serialize);
-  5. HIP graph capture on or off at the chosen pipeline.
+  5. the crossing slabs' memory budget per band group (slab_budget_mib),
+     which sets how many bands one filterbank launch covers;
+  6. HIP graph capture on or off at the chosen pipeline.
 The winners and every measurement go to the table, which the package
 applies at import (harvest_cuda/tuning.py).
 """
@@ -37,6 +41,8 @@ from . import _core, tuning
 from .signals import _voiced_unvoiced, config3
 
 FFT_SPLITS = (32, 48, 64, 80, 96, 128, 160, 1001)
+REFINE_GROUPS = (512, 768, 1024, 1536, 2048, 3072)
+SLAB_BUDGETS = (1024, 2048, 4096, 8192)
 DEPTHS = (2, 3, 4)
 GROUPS = (4, 8, 12, 16, 24, 32)
 WEIGHTS = (0.4, 0.7, 1.0)
@@ -46,21 +52,27 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def tune_fft_split(repeats: int = 10) -> tuple[int, list]:
+def calibration_utterance():
+    """One 60 s utterance at 24 kHz in device memory: (tensor, length, fs)."""
     import torch
     rng = np.random.default_rng(np.random.SeedSequence(2003).spawn(1)[0])
     fs = 24000
     x = np.ascontiguousarray(_voiced_unvoiced(rng, 60 * fs, fs, 25.0))
     dx = torch.from_numpy(x).to("cuda:0")
     torch.cuda.synchronize()
+    return dx, len(x), fs
+
+
+def tune_fft_split(repeats: int = 10) -> tuple[int, list]:
+    dx, n, fs = calibration_utterance()
     rows, ref = [], None
     for split in FFT_SPLITS:
         tuning.set_backend(fft_min_half=split)
         for _ in range(3):
-            _core.harvest_device(dx.data_ptr(), [len(x)], fs, timings=True)
+            _core.harvest_device(dx.data_ptr(), [n], fs, timings=True)
         ms = []
         for _ in range(repeats):
-            out, tm = _core.harvest_device(dx.data_ptr(), [len(x)], fs, timings=True)
+            out, tm = _core.harvest_device(dx.data_ptr(), [n], fs, timings=True)
             ms.append(tm["kernel_filterbank"])
         f0 = out[0][0]
         if ref is None and split == 64:
@@ -77,6 +89,28 @@ def tune_fft_split(repeats: int = 10) -> tuple[int, list]:
         if same_vuv and rel <= 1e-9:
             ok.append(r)
     best = min(ok, key=lambda r: r["filterbank_ms"])["fft_min_half"]
+    return best, rows
+
+
+def tune_refine_groups(repeats: int = 10) -> tuple[int, list]:
+    dx, n, fs = calibration_utterance()
+    rows, ref = [], None
+    for groups in (0,) + REFINE_GROUPS:  # 0: the backend's default, the reference output
+        tuning.set_backend(refine_groups=groups)
+        for _ in range(3):
+            _core.harvest_device(dx.data_ptr(), [n], fs, timings=True)
+        ms = []
+        for _ in range(repeats):
+            out, tm = _core.harvest_device(dx.data_ptr(), [n], fs, timings=True)
+            ms.append(tm["kernel_refine_dft"])
+        f0 = out[0][0]
+        if ref is None:
+            ref = f0
+        rows.append({"refine_groups": groups, "dft_ms": statistics.median(ms),
+                     "same_f0": bool(np.array_equal(f0, ref))})
+        log(f"refine_groups {groups}: k_bt_dft {rows[-1]['dft_ms']:.4f} ms")
+    ok = [r for r in rows if r["same_f0"] and r["refine_groups"] > 0]
+    best = min(ok, key=lambda r: r["dft_ms"])["refine_groups"] if ok else 0
     return best, rows
 
 
@@ -127,14 +161,24 @@ def main():
     if not _core.cuda_available(0):
         raise SystemExit("no supported gfx950 GPU visible")
     t0 = time.perf_counter()
-    tuning.set_backend(fft_min_half=64, graphs=0, slab_budget_mib=4096)
+    tuning.set_backend(fft_min_half=64, graphs=0, slab_budget_mib=4096, refine_groups=0)
     split, fft_rows = tune_fft_split()
     tuning.set_backend(fft_min_half=split)
+    groups, group_rows = tune_refine_groups()
+    tuning.set_backend(refine_groups=groups)
     xs, fs = config3(seed=2003, count=args.count)
     audio = sum(len(x) for x in xs) / fs
     log(f"calibration batch: {len(xs)} utterances, {audio:.0f} audio-s")
     batch, pipeline_rows = tune_pipeline(xs, fs)
     tuning.set_batch(**batch)
+    slab_rows = []
+    for budget in SLAB_BUDGETS:
+        tuning.set_backend(slab_budget_mib=budget)
+        ms = batch_wall(xs, fs, 5)
+        slab_rows.append({"slab_budget_mib": budget, "ms": ms})
+        log(f"slab_budget_mib {budget}: {ms:.1f} ms")
+    slab = min(slab_rows, key=lambda r: r["ms"])["slab_budget_mib"]
+    tuning.set_backend(slab_budget_mib=slab)
     graph_rows = []
     for graphs in (0, 1):
         tuning.set_backend(graphs=graphs)
@@ -143,11 +187,13 @@ def main():
         log(f"graphs {graphs}: {ms:.1f} ms")
     graphs = min(graph_rows, key=lambda r: r["ms"])["graphs"]
     table = {
-        "backend": {"fft_min_half": split, "graphs": graphs, "slab_budget_mib": 4096},
+        "backend": {"fft_min_half": split, "graphs": graphs, "slab_budget_mib": slab,
+                    "refine_table": 1, "refine_groups": groups},
         "batch": batch,
         "calibration": {"utterances": len(xs), "audio_seconds": audio, "fs": fs, "seed": 2003,
                         "seconds_to_tune": time.perf_counter() - t0},
-        "measured": {"fft_split": fft_rows, "pipeline": pipeline_rows, "graphs": graph_rows},
+        "measured": {"fft_split": fft_rows, "refine_groups": group_rows, "pipeline": pipeline_rows,
+                     "slab_budget": slab_rows, "graphs": graph_rows},
     }
     Path(args.out).write_text(json.dumps(table, indent=1) + "\n")
     log(f"wrote {args.out}: backend {table['backend']}, batch {batch}")

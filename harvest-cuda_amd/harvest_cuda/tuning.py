@@ -19,7 +19,7 @@ from pathlib import Path
 from . import _core
 
 TABLE = Path(__file__).resolve().parent / "tuning.json"
-BACKEND_KEYS = ("fft_min_half", "graphs", "slab_budget_mib", "refine_table")
+BACKEND_KEYS = ("fft_min_half", "graphs", "slab_budget_mib", "refine_table", "refine_groups")
 BATCH_KEYS = ("max_groups", "min_group_frames", "split_frames", "first_weight", "last_weight",
               "serialize", "depth")
 

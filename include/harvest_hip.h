@@ -76,7 +76,9 @@ typedef enum hv_status {
  *                    (working rate a multiple of 1000 Hz) refine candidates
  *                    through per-half-window bin tables, one lane per (frame,
  *                    half window) pair; 0: one wavefront per frame for every
- *                    analysis (default 1).
+ *                    analysis (default 1);
+ *   refine_groups    workgroups of the bin-table DFT kernel's persistent grid
+ *                    (64 to 4096; 0: the default 1536).
  * Results do not depend on the tuning beyond the rounding of the two
  * filterbank forms (fft_min_half). */
 typedef struct hv_tuning {
@@ -84,6 +86,7 @@ typedef struct hv_tuning {
   int graphs;
   int slab_budget_mib;
   int refine_table;
+  int refine_groups;
 } hv_tuning;
 hv_status hv_get_tuning(hv_workspace* ws, hv_tuning* out);
 hv_status hv_set_tuning(hv_workspace* ws, const hv_tuning* in);

@@ -132,3 +132,24 @@ def test_metrics_compare():
     assert a["cents_max"] == pytest.approx(1200)
     assert a["cents_p99"] == pytest.approx(1188)
     assert a["over_600_cents"] == 1
+
+
+@pytest.mark.parametrize("kw", [{"fft_min_half": 0}, {"graphs": 2}, {"slab_budget_mib": 32},
+                                {"refine_table": 2}, {"refine_groups": 63},
+                                {"refine_groups": 4097}])
+def test_tuning_validation(core, kw):
+    """Backend tuning values outside their ranges raise ValueError and leave
+    the current tuning in place."""
+    before = dict(core._get_tuning())
+    args = {**before, **kw}
+    with pytest.raises(ValueError):
+        core._set_tuning(**args)
+    assert dict(core._get_tuning()) == before
+
+
+def test_tuning_table_keys():
+    """The committed tuning table names only known settings."""
+    from harvest_cuda import tuning
+    table = tuning.load()
+    assert set(table["backend"]) <= set(tuning.BACKEND_KEYS)
+    assert set(table["batch"]) <= set(tuning.BATCH_KEYS)

@@ -85,3 +85,22 @@ def test_frames_beyond_the_pair_limit(gpu, oracle, kind):
     ef0, _ = oracle.harvest(x, fs, floor, ceil)
     assert_parity(ef0, table[0], RTOL)
     assert_parity(frame[0], table[0], 1e-10)
+
+
+def test_dft_grid_leaves_results_unchanged(gpu):
+    """The DFT kernel's persistent grid (refine_groups) only changes which
+    workgroup computes a pair: results are bitwise the same at any size."""
+    from harvest_cuda import tuning
+    saved = tuning.current()
+    fs = 16000
+    xs = [reference_signal(k, seconds=s, fs=fs) for k, s in (("speechlike", 7.5), ("noise", 2.0))]
+    try:
+        tuning.set_backend(refine_groups=0)
+        want = gpu.harvest_batch(xs, fs)
+        for groups in (64, 333, 4096):
+            tuning.set_backend(refine_groups=groups)
+            got = gpu.harvest_batch(xs, fs)
+            for (f0, _), (wf0, _) in zip(got, want):
+                np.testing.assert_array_equal(f0, wf0, err_msg=f"refine_groups {groups}")
+    finally:
+        tuning.apply(saved)
