@@ -854,8 +854,10 @@ __device__ __forceinline__ FftTwiddles load_fft_twiddles(const double2* tw) {
   return f;
 }
 
+// Complex product with fused multiply-adds (the FFT filterbank's results
+// equal the direct FIR's up to rounding, not bit for bit).
 __device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+  return make_double2(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x));
 }
 
 // Unscaled in-place DIF FFT of 2048 complex values (re, im in LDS, element
@@ -889,7 +891,7 @@ __device__ void fft2048(LDS double* re, LDS double* im, const FftTwiddles& tws) 
     for (int r = 1; r < 8; ++r) {
       const double2 w = wr[r];
       const double wi = SIGN * w.y;
-      const double xr = vr[r] * w.x - vi[r] * wi, xi = vr[r] * wi + vi[r] * w.x;
+      const double xr = fma(vr[r], w.x, -(vi[r] * wi)), xi = fma(vr[r], wi, vi[r] * w.x);
       vr[r] = xr;
       vi[r] = xi;
     }
@@ -963,7 +965,7 @@ __global__ __launch_bounds__(kThreads) void k_spectrum(Config c, const Utt* utt,
     const double dr = ckr - cmr, di = cki - cmi;
     const double orr = 0.5 * di, oi = -0.5 * dr;
     const double2 w = tw[k];  // e^{-2 pi i k / 4096} = (w.x, -w.y)
-    out[k] = make_double2(er + (w.x * orr + w.y * oi), ei + (w.x * oi - w.y * orr));
+    out[k] = make_double2(er + fma(w.x, orr, w.y * oi), ei + fma(w.x, oi, -(w.y * orr)));
   }
 }
 
@@ -1000,7 +1002,7 @@ __global__ __launch_bounds__(kThreads) void k_filter_fft(
     const double ar = 0.5 * (zk.x + zm.x), ai = 0.5 * (zk.y - zm.y);
     const double dr = 0.5 * (zk.x - zm.x), di = 0.5 * (zk.y + zm.y);
     const double2 w = tw[k];
-    const double br = dr * w.x - di * w.y, bi = dr * w.y + di * w.x;
+    const double br = fma(dr, w.x, -(di * w.y)), bi = fma(dr, w.y, di * w.x);
     re[sw(k)] = ar - bi;
     im[sw(k)] = ai + br;
   };
