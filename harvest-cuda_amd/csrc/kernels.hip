@@ -613,9 +613,13 @@ __device__ void tile_crossings(const Config& c, const Utt& u, const Quirk& qk, l
     const double q = 2.0 * (qk.d1re * cs + qk.d1im * sn) + qk.d2;
     return (k & 1) ? -q : q;
   };
+  // v[0..7]: this thread's outputs j0 + r (kept in registers); v[8], v[9]:
+  // the next thread's first two, read back after the barrier.
+  double v[10];
 #pragma unroll
   for (int r = 0; r < 8; ++r) {
-    f[pad8(j0 + r)] = acc[r] + correction_at(k0 + r, tcs, tsn) * inv_n;
+    v[r] = acc[r] + correction_at(k0 + r, tcs, tsn) * inv_n;
+    f[pad8(j0 + r)] = v[r];
     const double nc = fma(tcs, rcs, -(tsn * rsn)), ns = fma(tsn, rcs, tcs * rsn);
     tcs = nc;
     tsn = ns;
@@ -625,19 +629,25 @@ __device__ void tile_crossings(const Config& c, const Utt& u, const Quirk& qk, l
   // Crossing flags: bit (type * 8 + r).  Types: 0 negative-going, 1
   // positive-going, 2 peak, 3 dip; WORLD's order (harvest.cpp:218-237).
   auto F = [&](int j) -> double { return f[pad8(j)]; };
+  // The last thread owns j < kTileStride only, which needs no v[8], v[9].
+  const bool next = threadIdx.x + 1 < kThreads;
+  v[8] = next ? F(j0 + 8) : 0.0;
+  v[9] = next ? F(j0 + 9) : 0.0;
+  // Every sample pair and triple of the thread lies inside the utterance.
+  const bool inside = i0 + j0 + 7 <= N - 3;
   unsigned flags = 0;
 #pragma unroll
   for (int r = 0; r < 8; ++r) {
     const int j = j0 + r;
     const long long i = i0 + j;
     if (j >= kTileStride) break;  // owned by the next tile
-    if (i <= N - 2) {
-      const double a = F(j), bb = F(j + 1);
+    if (inside || i <= N - 2) {
+      const double a = v[r], bb = v[r + 1];
       if (0.0 < a && bb <= 0.0) flags |= 1u << (0 * 8 + r);
       if (a < 0.0 && bb >= 0.0) flags |= 1u << (1 * 8 + r);
     }
-    if (i <= N - 3) {
-      const double d0 = F(j + 1) - F(j), d1 = F(j + 2) - F(j + 1);
+    if (inside || i <= N - 3) {
+      const double d0 = v[r + 1] - v[r], d1 = v[r + 2] - v[r + 1];
       if (0.0 < d0 && d1 <= 0.0) flags |= 1u << (2 * 8 + r);
       if (d0 < 0.0 && d1 >= 0.0) flags |= 1u << (3 * 8 + r);
     }
