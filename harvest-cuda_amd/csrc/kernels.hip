@@ -72,6 +72,42 @@ __device__ unsigned long long block_scan_u64(unsigned long long v,
   return before + incl - v;
 }
 
+// Inclusive prefix sum of v over the wave by DPP: row shifts by 1, 2, 4, 8
+// within each 16-lane row, then the row totals broadcast to the rows after
+// them (row_bcast:15 into rows 1 and 3, row_bcast:31 into rows 2 and 3).
+__device__ __forceinline__ unsigned wave_scan_u32(unsigned v) {
+  v += __builtin_amdgcn_update_dpp(0u, v, 0x111, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0u, v, 0x112, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0u, v, 0x114, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0u, v, 0x118, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0u, v, 0x142, 0xa, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0u, v, 0x143, 0xc, 0xf, false);
+  return v;
+}
+
+// block_scan_u64 for values of 16-bit fields whose totals over the block
+// stay below 2^16: the two 32-bit halves never carry into each other, so
+// each is scanned on its own.
+__device__ unsigned long long block_scan_fields(unsigned long long v,
+                                                unsigned long long* wave_sums,
+                                                unsigned long long& total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned lo = wave_scan_u32((unsigned)v), hi = wave_scan_u32((unsigned)(v >> 32));
+  const unsigned long long incl = ((unsigned long long)hi << 32) | lo;
+  if (lane == 63) wave_sums[wave] = incl;
+  __syncthreads();
+  unsigned long long before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < kThreads / 64; ++w) {
+    unsigned long long s = wave_sums[w];
+    if (w < wave) before += s;
+    all += s;
+  }
+  total = all;
+  __syncthreads();
+  return before + incl - v;
+}
+
 constexpr int kXcds = 8;  // XCDs that workgroups are dispatched to in turn
 
 // Workgroup b of G is dispatched to XCD b mod 8; renumbered so that each XCD
@@ -657,7 +693,9 @@ __device__ void tile_crossings(const Config& c, const Utt& u, const Quirk& qk, l
   for (int t = 0; t < 4; ++t)
     packed |= (unsigned long long)__popc((flags >> (t * 8)) & 0xffu) << (16 * t);
   unsigned long long total;
-  unsigned long long excl = block_scan_u64(packed, wave_sums, total);
+  // Per type at most kSlabCap - 1 crossings per tile (same-type crossings
+  // are two samples apart or more): the 16-bit fields never overflow.
+  unsigned long long excl = block_scan_fields(packed, wave_sums, total);
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     unsigned bits = (flags >> (t * 8)) & 0xffu;
