@@ -89,6 +89,22 @@ def measured_traffic(kernel, workload):
     return None
 
 
+def stage_bytes(lengths, fs, floor=71.0, ceil=800.0):
+    """The per-stage terms of algorithmic_bytes(), summed over utterances:
+    the filterbank's 16 (n/2 + 1)(2B + 1) (signal spectrum and one round trip
+    of each band's product spectrum at WORLD's global n) and the
+    refinement's 24 F C (each candidate slot read, its refined f0 and score
+    written)."""
+    from harvest_cuda import _core
+    fb = rf = 0.0
+    for L in lengths:
+        p = _core._params(L, fs, floor, ceil, 5.0)
+        n, B, F, C = p["fft_size"], p["bands"], p["frames"], 7 * p["capacity"]
+        fb += 16 * (n // 2 + 1) * (2 * B + 1)
+        rf += 24 * F * C
+    return {"kernel_filterbank": fb, "kernel_refine": rf}
+
+
 def filterbank_flops(lengths, fs, fft_min_half, floor=71.0, ceil=800.0):
     """FP64 FLOPs of the filterbank as this build computes it, summed over
     utterances: bands whose half length h is at least fft_min_half run by
@@ -136,7 +152,9 @@ def roofline(kernels_ms, work, launches, workload, lengths, fs):
     kernels.  The filterbank's FLOPs are filterbank_flops().  `launches` is
     the analyses per step (one per bucket): a launch here is one analysis'
     refinement DFT kernel, or its filterbank stage (tile spectra, FFT bands,
-    direct bands)."""
+    direct bands).  Each object also carries `hbm_view`: the same launch
+    time against HBM bandwidth, by the stage's term of the byte count
+    algorithmic_bytes() uses (stage_bytes())."""
     from harvest_cuda import tuning
     fft_min = tuning.current()["backend"]["fft_min_half"]
     out = {}
@@ -168,6 +186,16 @@ def roofline(kernels_ms, work, launches, workload, lengths, fs):
         "launches_per_step": launches, "flop_per_launch": flops,
         "model": f"FFT bands (h >= {fft_min}): 2.5 n log2 n per 4096-point transform + 6 FLOP "
                  "per bin; direct bands: 2 FLOP per tap per output sample"}
+    # The same launches against HBM, by the stage's term of the
+    # implementation-independent byte count (algorithmic_bytes()).
+    terms = stage_bytes(lengths, fs)
+    for k, r in out.items():
+        per_bytes = terms[k] / launches
+        gbps = per_bytes / (r["avg_launch_ms"] / 1000.0) / 1e9
+        r["hbm_view"] = {"algorithmic_bytes_per_launch": per_bytes, "achieved_gbps": gbps,
+                         "peak_gbps": HBM_PEAK_GBS, "frac": gbps / HBM_PEAK_GBS,
+                         "model": "16(n/2+1)(2B+1) per utterance" if k == "kernel_filterbank"
+                                  else "24FC per utterance"}
     name = max(("kernel_refine", "kernel_filterbank"),
                key=lambda k: out[k]["avg_launch_ms"])
     return out[name], out
