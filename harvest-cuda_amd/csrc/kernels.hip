@@ -1080,11 +1080,18 @@ __global__ __launch_bounds__(kThreads) void k_filter_fft(
   __syncthreads();
   fft2048<1>(re, im, tws);
   // z[s] = (s even ? Re : Im) w[s / 2] / 2048; output j is z[j + D + h + 1].
+  // The thread's 8 outputs read w[k0 .. k0 + 4].  Within a group of 8
+  // consecutive k, dig_rev adds 256 per step (the low digit moves to the top),
+  // which sw turns into 272: so w[k] is at sw(dig_rev(k0)) + 272 (k - k0)
+  // before the next group's first index kw, and at sw(dig_rev(kw)) +
+  // 272 (k - kw) from it on (k < kFftM since D + h + 1 + 8 t + 7 < 2 kFftM).
+  const int s0 = 8 * t + D + h + 1, k0 = s0 >> 1, kw = (k0 | 7) + 1;
+  const int a0 = sw(dig_rev(k0)), a1 = sw(dig_rev(kw & (kFftM - 1)));
   double acc[8];
 #pragma unroll
   for (int r = 0; r < 8; ++r) {
-    const int s = 8 * t + r + D + h + 1;
-    const int p = sw(dig_rev(s >> 1));
+    const int s = s0 + r, k = s >> 1;
+    const int p = k < kw ? a0 + 272 * (k - k0) : a1 + 272 * (k - kw);
     acc[r] = ((s & 1) ? im[p] : re[p]) * (1.0 / kFftM);
   }
   __syncthreads();  // every value is read: buf takes the outputs
