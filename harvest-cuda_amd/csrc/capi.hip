@@ -585,7 +585,7 @@ void prepare_bintable(hv_workspace* ws, double fs_work, double f0_floor, double 
       harm[6 * hb + j] = e;
     }
     const int nb = (int)bins[hb].size();
-    if (nb > 0 && bins[hb].back() > 127) return;  // candidate codes hold 7-bit bins
+    if (nb > 127) return;  // candidate codes hold 7-bit indices of the table bins
     info[hb] = int4{n, nb, (int)rows, (int)records};
     rows += 2 * h + 1;
     records += nb;

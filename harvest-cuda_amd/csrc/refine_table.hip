@@ -159,10 +159,11 @@ __global__ __launch_bounds__(kThreads) void k_bt_tables(Config c, BinTable t, co
 
 // ---------------------------------------------------------------------------
 // Candidate codes: for official candidate jj of frame f (official[f][jj]),
-// code = 1 | hb << 1 | H << 9 | k_1 << 12 | ... | k_6 << 42 with hb = h -
+// code = 1 | hb << 1 | H << 9 | q_1 << 12 | ... | q_6 << 42 with hb = h -
 // bt_hlo for the half window h (harvest.cpp:596), H the harmonics FixF0 uses
-// (:577-578) and k_j the bin of harmonic j (:515, 7 bits each); 0 for a
-// candidate that is not positive or not one of the utterance's count.
+// (:577-578) and q_j the index, among h's table bins, of the bin k_j of
+// harmonic j (:515; 7 bits each); 0 for a candidate that is not positive or
+// not one of the utterance's count.
 // inv[f][jj] = 1 / f0.  A half window or bin outside the tables sets the
 // fault flag.
 __global__ __launch_bounds__(kThreads) void k_bt_codes(
@@ -190,8 +191,8 @@ __global__ __launch_bounds__(kThreads) void k_bt_codes(
       for (int j = 0; j < H; ++j) {
         const int k = mround(qv * (j + 1));
         const int4 hr = harm[hb * 6 + j];  // (first bin, bins, first result index)
-        if (k - hr.x < 0 || k - hr.x >= hr.y || k > 127) fault = true;
-        code |= (unsigned long long)(k & 127) << (12 + 7 * j);
+        if (k - hr.x < 0 || k - hr.x >= hr.y) fault = true;
+        code |= (unsigned long long)((hr.z + k - hr.x) & 127) << (12 + 7 * j);
       }
       r = 1.0 / f0;
       if (fault) code = 0;
@@ -585,13 +586,6 @@ __global__ __launch_bounds__(kThreads) void k_bt_finish(
     const double* official, const unsigned long long* codes, const double* inv,
     const int* count, const unsigned long long* needs, BinTable t, const double2* res,
     double* rf0, double* rscore) {
-  // (first bin, first result index) of every (half window, harmonic).
-  __shared__ int2 s_harm[6 * kBtMaxHr];
-  for (int k = threadIdx.x; k < 6 * c.bt_hr; k += kThreads) {
-    const int4 hr = t.harm[k];
-    s_harm[k] = make_int2(hr.x, hr.z);
-  }
-  __syncthreads();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const long long gf = (long long)blockIdx.x * (kThreads / 64) + wave;
   if (gf >= total_frames) return;
@@ -634,9 +628,7 @@ __global__ __launch_bounds__(kThreads) void k_bt_finish(
       const double2* pr = row + (long long)r * c.bt_nbmax;
       double numer = 0.0, den = 0.0, dev = 0.0;
       for (int hh = 0; hh < H; ++hh) {
-        const int k = (int)(code >> (12 + 7 * hh)) & 127;
-        const int2 hr = s_harm[hb * 6 + hh];
-        const int q = hr.y + (k - hr.x);
+        const int q = (int)(code >> (12 + 7 * hh)) & 127;
         double2 pn;
         if (direct) {
           const int4 info = t.info[hb];
