@@ -144,6 +144,14 @@ __device__ __forceinline__ long long select_base(const Config& c, const Utt& u) 
   return u.m - 9 - u.nbeg - (long long)(c.lag / c.ratio) * c.ratio;
 }
 
+// Orders this wave's LDS accesses: writes by any lane before it are visible
+// to reads by any lane after it.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Sequential loops load this many inputs ahead, so that the loads' latency
 // overlaps instead of stalling every step of the recursion.
 constexpr int kIirAhead = 8;
@@ -328,11 +336,64 @@ __global__ __launch_bounds__(kThreads) void k_carry_top(Config c, const Utt* utt
   }
 }
 
+// Full-rate outputs (passes 0, 2, 3) of one wave's chunks: each lane's
+// kIirAhead outputs go to LDS, then the wave stores them kIirAhead lanes
+// per instruction, each lane group covering one lane's kIirAhead
+// consecutive samples (whole 64-byte segments rather than one element of
+// 64 scattered lines per store).
+struct IirStage {
+  double v[64 * (kIirAhead + 1)];  // lane l's output q at l (kIirAhead + 1) + q
+  long long base[64];              // lane l's first output index
+  int len[64];                     // lane l's outputs (0 for a lane past the end)
+};
+
 template <int PASS>
 __global__ __launch_bounds__(kThreads) void k_iir_replay(
     Config c, const Utt* utt, const long long* c_off, long long total,
     const double* x, double* w, const double* carries, double* y) {
   long long cg = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (PASS != 1) {
+    __shared__ IirStage stage[kThreads / 64];
+    IirStage& st = stage[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
+    const bool live = cg < total;
+    const Utt u = utt[live ? find_segment(c_off, c.count, cg) : 0];
+    const long long start = live ? (cg - u.c_off) * kChunk : 0;
+    const long long end = live ? min(start + (long long)kChunk, u.m) : 0;
+    double* out = PASS == 3 ? y : w;
+    st.base[lane] = u.m_off + start;
+    st.len[lane] = (int)(end - start);
+    double w0 = 0.0, w1 = 0.0, w2 = 0.0;
+    if (live) {
+      w0 = carries[cg * 3 + 0];
+      w1 = carries[cg * 3 + 1];
+      w2 = carries[cg * 3 + 2];
+    }
+    // Every lane takes kChunk / kIirAhead rounds (a short chunk's extra
+    // samples are computed and not stored), so that the wave stores together.
+    for (int r0 = 0; r0 < kChunk; r0 += kIirAhead) {
+      double in[kIirAhead];
+#pragma unroll
+      for (int q = 0; q < kIirAhead; ++q)
+        in[q] = start + r0 + q < end ? iir_input<PASS>(c, u, x, w, start + r0 + q) : 0.0;
+#pragma unroll
+      for (int q = 0; q < kIirAhead; ++q) {
+        double wt = in[q] + c.a0 * w0 + c.a1 * w1 + c.a2 * w2;
+        st.v[lane * (kIirAhead + 1) + q] = c.b0 * wt + c.b1 * w0 + c.b1 * w1 + c.b0 * w2;
+        w2 = w1;
+        w1 = w0;
+        w0 = wt;
+      }
+      wave_sync();
+#pragma unroll
+      for (int g = 0; g < kIirAhead; ++g) {  // lanes g 64 / kIirAhead + lane / kIirAhead
+        const int l = g * (64 / kIirAhead) + lane / kIirAhead, q = lane % kIirAhead;
+        if (r0 + q < st.len[l]) out[st.base[l] + r0 + q] = st.v[l * (kIirAhead + 1) + q];
+      }
+      wave_sync();  // the values are stored before the next round overwrites them
+    }
+    return;
+  }
   if (cg >= total) return;
   const Utt u = utt[find_segment(c_off, c.count, cg)];
   long long start = (cg - u.c_off) * kChunk;
@@ -1420,14 +1481,6 @@ __device__ __forceinline__ int lds_add(LDS int* p, int v) {
 }
 __device__ __forceinline__ void lds_or(LDS unsigned long long* p, unsigned long long v) {
   __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
-// Orders this wave's LDS accesses: writes by any lane before it are visible
-// to reads by any lane after it.
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // v of the lane that DPP control kCtrl selects (a permutation within rows),
