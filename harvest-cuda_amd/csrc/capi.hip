@@ -713,10 +713,12 @@ void layout(Plan& p, const std::vector<long long>& lengths, const std::vector<in
     invalid("input or analysis settings exceed supported allocation sizes");
 }
 
-// Upload the plan's descriptor arrays through one pinned staging buffer.
+// Bytes of a plan's descriptor arrays: the utterances, then six offset
+// arrays of U + 1 entries.
 constexpr int kPlanArrays = 6;
 size_t plan_bytes(int U) { return U * sizeof(Utt) + kPlanArrays * (U + 1) * sizeof(long long); }
 
+// Upload the plan's descriptor arrays through one pinned staging buffer.
 void upload_plan(hv_workspace* ws, const Plan& p) {
   const int U = p.cfg.count;
   constexpr int kArrays = kPlanArrays;

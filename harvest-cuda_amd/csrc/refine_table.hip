@@ -577,10 +577,12 @@ __device__ double2 bt_bin_direct(const double* yu, long long first, long long la
 
 // FixF0 (harvest.cpp:507-536) and GetRefinedF0's gate (:610-614) of every
 // slot, one wavefront per frame (lanes over slots), from the slot's
-// candidate code and the bin results of its (frame, h) pair.  The divisions
-// that feed no integer decision are products with reciprocals (1 / (2 pi),
-// 1 / (h + 1), 1 / f0): within two ulps of WORLD's quotients, which the
-// spectra's own rounding far exceeds.
+// candidate code and the bins' amplitudes and frequencies of its (frame, h)
+// pair.  The error term's divisions by j + 1 and by f0 (and the frequency's
+// by 2 pi, in bin_amp_if) are products with reciprocals: within two ulps of
+// WORLD's quotients, far below the spectra's own rounding; the score they
+// feed is compared with 2.5, so a slot could change only if its score fell
+// within a few ulps of that threshold.
 __global__ __launch_bounds__(kThreads) void k_bt_finish(
     Config c, const Utt* utt, const long long* f_off, long long total_frames, const double* y,
     const double* official, const unsigned long long* codes, const double* inv,
