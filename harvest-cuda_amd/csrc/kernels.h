@@ -88,7 +88,7 @@ void launch_refine(hipStream_t s, const Config& c, const Utt* utt,
 // Bin-table refinement (refine_table.hip): analyses whose frames start at
 // whole samples (Config::win_pairs) and whose half windows span at most
 // kBtMaxHr values.
-constexpr int kBtPassBins = 8;  // bins per pass over a window (at most)
+constexpr int kBtPassBins = 12;  // bins per pass over a window (at most)
 constexpr int kBtWords = 4;     // 64-bit words of a frame's half-window bitmask
 constexpr int kBtMaxHr = kBtWords * 64;
 constexpr int kBtPairs = 32;    // (frame, h) pairs per frame with stored bin results

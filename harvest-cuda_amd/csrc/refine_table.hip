@@ -515,6 +515,7 @@ __global__ __launch_bounds__(kThreads) void k_bt_dft(
     const CONSTANT_AS double* rec0 = bins + (long long)kBtRecord * __builtin_amdgcn_readfirstlane(info.w);
     double2* out = res + (gf * kBtPairs + r) * c.bt_nbmax;
     // Passes of at most kBtPassBins bins, split evenly.
+    static_assert(kBtPassBins <= 12, "the pass switch below has cases for up to 12 bins");
     const int npass = (nb + kBtPassBins - 1) / kBtPassBins;
     for (int p = 0; p < npass; ++p) {
       const int q0 = p * nb / npass, nq = (p + 1) * nb / npass - q0;
@@ -527,6 +528,7 @@ __global__ __launch_bounds__(kThreads) void k_bt_dft(
       switch (nq) {
         BT_CHUNK(1) BT_CHUNK(2) BT_CHUNK(3) BT_CHUNK(4)
         BT_CHUNK(5) BT_CHUNK(6) BT_CHUNK(7) BT_CHUNK(8)
+        BT_CHUNK(9) BT_CHUNK(10) BT_CHUNK(11) BT_CHUNK(12)
         default: break;
       }
 #undef BT_CHUNK
