@@ -527,6 +527,11 @@ PYBIND11_MODULE(_core, m) {
     }
     return std::make_pair(replays, captures);
   }, py::arg("device") = 0, "(graph replays, graph captures) of the device's workspaces.");
+  m.def("_allocation_stats", [] {
+    int64_t allocations = 0, regrowths = 0;
+    hv_allocation_stats(&allocations, &regrowths);
+    return std::make_pair(allocations, regrowths);
+  }, "(buffers allocated, device buffers freed on growth) by the backend's workspaces so far.");
   m.def("_host_cache_allocated", [] { return HostCache::instance().allocated(); });
   m.def("_cached_workspaces", [] { return cache().size(); });
 }

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -55,6 +56,11 @@ void check(hipError_t e, const char* what) {
   Device::check(e == hipErrorMemoryAllocation ? hipErrorOutOfMemory : e, what);
 }
 
+// Device and pinned buffers allocated by every workspace of the process,
+// and the device buffers among them that replaced a smaller one (freed on
+// growth); hv_allocation_stats.
+std::atomic<long long> g_allocations{0}, g_device_regrowths{0};
+
 // While set, ensure() records the size it is asked for instead of
 // allocating (reserve_batches); a buffer whose recorded size exceeds its
 // allocation grows to that size, once, at its next ensure().
@@ -86,9 +92,11 @@ struct DevBuf {
     }
     if (need > bytes || want > bytes) {
       need = std::max(need, want);
+      if (p) g_device_regrowths.fetch_add(1, std::memory_order_relaxed);
       release();
       check(hipMalloc(&p, need), "hipMalloc");
       bytes = need;
+      g_allocations.fetch_add(1, std::memory_order_relaxed);
     }
     return static_cast<T*>(p);
   }
@@ -119,6 +127,7 @@ struct PinnedBuf {
       release();
       check(hipHostMalloc(&p, need, hipHostMallocDefault), "hipHostMalloc");
       bytes = need;
+      g_allocations.fetch_add(1, std::memory_order_relaxed);
     }
     return static_cast<T*>(p);
   }
@@ -1696,6 +1705,11 @@ hv_status hv_order_after(hv_workspace* ws, hv_workspace* before) {
     check(hipEventRecord(ws->after_ev, before->stream), "hipEventRecord");
     ws->after_pending = true;
   });
+}
+
+void hv_allocation_stats(int64_t* allocations, int64_t* device_regrowths) {
+  if (allocations) *allocations = g_allocations.load(std::memory_order_relaxed);
+  if (device_regrowths) *device_regrowths = g_device_regrowths.load(std::memory_order_relaxed);
 }
 
 hv_status hv_graph_stats(hv_workspace* ws, int64_t* replays, int64_t* captures) {

@@ -94,6 +94,13 @@ hv_status hv_set_tuning(hv_workspace* ws, const hv_tuning* in);
  * workspace was created.  No reference counterpart. */
 hv_status hv_graph_stats(hv_workspace* ws, int64_t* replays, int64_t* captures);
 
+/* Device and pinned buffers the workspaces of this process have allocated
+ * so far, and how many of the device ones replaced (freed) a smaller buffer
+ * of the same workspace.  Repeating a batch allocates none; a pipelined
+ * batch on new workspaces frees none (hv_reserve_batches).  Either pointer
+ * may be NULL.  No reference counterpart. */
+void hv_allocation_stats(int64_t* allocations, int64_t* device_regrowths);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char* hv_last_error(void);
 

@@ -82,3 +82,26 @@ def test_config3_timings_leave_results_unchanged(gpu, workload, batched):
     for key in ("kernel_refine", "kernel_filterbank", "kernel_interp", "work_bin_samples"):
         assert tm[key] > 0, key
     assert tm["kernel_refine"] < tm["preprocess"] + tm["filter_events"] + tm["refine"]
+
+
+def test_batch_sizes_its_buffers_once(gpu, workload, batched):
+    """The pipelined batch reserves each workspace's buffers for its largest
+    bucket before submitting: on new workspaces no device buffer is freed to
+    grow (the driver clears freed device memory in the background, which
+    slowed the calls that followed), and once the pinned candidate buffers
+    have adapted to the batch's candidate count (after the first call),
+    repeating the call allocates nothing; the F0 stay the same."""
+    xs, fs = workload
+    gpu.clear_cache()
+    _, regrowths = gpu._allocation_stats()
+    first = gpu.harvest_batch(xs, fs)
+    assert gpu._allocation_stats()[1] == regrowths
+    second = gpu.harvest_batch(xs, fs)
+    stats = gpu._allocation_stats()
+    assert stats[1] == regrowths
+    third = gpu.harvest_batch(xs, fs)
+    assert gpu._allocation_stats() == stats
+    for (f0, _), (f1, _), (f2, _), (bf0, _) in zip(first, second, third, batched):
+        np.testing.assert_array_equal(f0, bf0)
+        np.testing.assert_array_equal(f1, bf0)
+        np.testing.assert_array_equal(f2, bf0)
