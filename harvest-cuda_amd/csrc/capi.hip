@@ -271,7 +271,7 @@ struct hv_workspace {
   DevBuf bt_info, bt_harm, bt_kq, bt_win, bt_bins;
   int bt_hlo = 0, bt_hr = 0, bt_nbmax = 0;
   double bt_key_fs = -1.0, bt_key_floor = -1.0, bt_key_ceil = -1.0;
-  DevBuf bt_codes, bt_inv, bt_needs, bt_mask, bt_cnt, bt_off, bt_scan, bt_list, bt_cinfo,
+  DevBuf bt_codes, bt_inv, bt_hbs, bt_needs, bt_mask, bt_cnt, bt_off, bt_scan, bt_list, bt_cinfo,
       bt_sched, bt_res;
   long long last_nnz = -1;
   long long last_frames = 0;
@@ -569,7 +569,7 @@ void prepare_bintable(hv_workspace* ws, double fs_work, double f0_floor, double 
   if (!use_window_pairs(fs_work, hhi)) return;
   const int hlo = half_window_of(fs_work, f0_ceil * (1.0 + 1e-9));
   const int hr = hhi - hlo + 1;
-  if (hlo < 1 || hr > kBtMaxHr) return;
+  if (hlo < 1 || hr >= kBtMaxHr) return;  // hb < 0xFF: a byte per candidate, 0xFF for none
   const double lo = f0_floor * 0.5, hi = f0_ceil * 2.0;  // beyond every candidate
   std::vector<int4> info(hr), harm(6 * (size_t)hr);
   std::vector<std::vector<int>> bins(hr);
@@ -830,6 +830,7 @@ Bufs size_buffers(hv_workspace* ws, const Plan& p, const double* dx) {
     const long long nblk = bt_blocks(p.total_frames), cells = (long long)c.bt_hr * nblk;
     b.bt.codes = ws->bt_codes.ensure<unsigned long long>((size_t)p.total_frames * c.capacity);
     b.bt.inv = ws->bt_inv.ensure<double>((size_t)p.total_frames * c.capacity);
+    b.bt.hbs = ws->bt_hbs.ensure<unsigned char>((size_t)p.total_frames * bt_hb_stride(c.capacity));
     b.bt.needs = ws->bt_needs.ensure<unsigned long long>((size_t)p.total_frames * kBtWords);
     b.bt.mask = ws->bt_mask.ensure<unsigned long long>(cells);
     b.bt.cnt = ws->bt_cnt.ensure<int>(cells);
@@ -972,7 +973,7 @@ std::vector<long long> graph_key(hv_workspace* ws, const Plan& p, const Bufs& b)
                         b.count, b.official, b.rf0, b.rsc, b.pf0, b.psc, b.dbest, b.cinit, b.work,
                         b.row_nnz, b.doffsets, b.scan_tmp, b.hoff, b.hbest, b.hwork,
                         b.hcand, ws->bt_info.p, ws->bt_harm.p, ws->bt_win.p, ws->bt_bins.p,
-                        b.bt.codes, b.bt.inv, b.bt.needs,
+                        b.bt.codes, b.bt.inv, b.bt.hbs, b.bt.needs,
                         b.bt.mask, b.bt.cnt, b.bt.off, b.bt.scan_tmp, b.bt.list, b.bt.cinfo,
                         b.bt.sched, b.bt.res};
   for (const void* q : ptrs) k.push_back((long long)(uintptr_t)q);

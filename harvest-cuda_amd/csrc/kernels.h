@@ -109,13 +109,17 @@ struct BinTable {
   const double* bins;
 };
 // Per-analysis buffers: codes and inv [frames][capacity],
-// needs[frames][kBtWords], mask and cnt [bt_hr][nblk] (nblk =
+// hbs[frames][bt_hb_stride(capacity)] (each candidate's hb as a byte, 0xFF
+// for none), needs[frames][kBtWords], mask and cnt [bt_hr][nblk] (nblk =
 // bt_blocks(frames)), off[bt_hr nblk + 1] with its scan scratch,
 // list[frames kBtPairs], cinfo[3 bt_hr + 1], sched[1],
-// res[frames kBtPairs bt_nbmax].
+// res[frames kBtPairs bt_nbmax] (bins of half windows with more than
+// kBtPassBins bins; the others' slots are finished inside k_bt_dft).
+__host__ __device__ inline long long bt_hb_stride(int capacity) { return (capacity + 15) / 16 * 16; }
 struct BtBufs {
   unsigned long long* codes;
   double* inv;
+  unsigned char* hbs;
   unsigned long long *needs, *mask;
   int *cnt, *off, *scan_tmp;
   long long scan_len;

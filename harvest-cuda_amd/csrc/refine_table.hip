@@ -10,7 +10,8 @@
 // one h can read form a short range per harmonic: the candidates giving h lie
 // in one narrow f0 interval.  The work is therefore split as
 //   k_bt_codes    one lane per official candidate: its half window h,
-//                 harmonic count and FixF0 bins, packed in one word, and 1/f0;
+//                 harmonic count and FixF0 bins, packed in one word, 1/f0,
+//                 and h again as a byte (for k_bt_dft's slot search);
 //   k_bt_plan     one lane per frame: the half windows its slots use (a
 //                 bitmask per frame) and, per block of 64 frames and per h,
 //                 the frames that use h;
@@ -20,9 +21,13 @@
 //                 h's table by Goertzel recurrences over the window's
 //                 samples, whose window values and recurrence coefficients
 //                 are uniform across the wavefront (scalar operands), with
-//                 the samples staged in LDS by direct loads;
-//   k_bt_finish   one wavefront per frame: FixF0 and GetRefinedF0's gate for
-//                 each slot from the bin results of its pair.
+//                 the samples staged in LDS by direct loads; then FixF0
+//                 and GetRefinedF0's gate of every slot that reads the
+//                 pair, from the bins in LDS (one pass over the window, at
+//                 most kBtPassBins bins) or read back from res (more bins);
+//   k_bt_finish   one wavefront per frame with more than kBtPairs half
+//                 windows (not listed): FixF0 of its slots, bins computed
+//                 one at a time.
 // Each bin is one fixed sequence of operations (samples in order), so the
 // results depend neither on the batch nor on which kernel computed the bin.
 #include <hip/hip_runtime.h>
@@ -128,6 +133,37 @@ __device__ __forceinline__ double2 bin_amp_if(double2 pn, double kfn, double fs)
   return make_double2(sqrt(pn.x), ifr);
 }
 
+// FixF0 (harvest.cpp:507-536) and GetRefinedF0's gate (:610-614) of one
+// slot: (refined f0, score) from its candidate code, f0 and 1 / f0, with
+// bin(q) the (amplitude, IF) of bin q of its half window's table.  The
+// error term's divisions by j + 1 and by f0 (and the frequency's by 2 pi, in
+// bin_amp_if) are products with reciprocals: within two ulps of WORLD's
+// quotients, far below the spectra's own rounding; the score they feed is
+// compared with 2.5, so a slot could change only if its score fell within a
+// few ulps of that threshold.
+template <class Bin>
+__device__ __forceinline__ double2 fix_f0_slot(const Config& c, unsigned long long code,
+                                               double f0, double inv_f0, Bin bin) {
+  const int H = (int)(code >> 9) & 7;
+  double numer = 0.0, den = 0.0, dev = 0.0;
+  for (int hh = 0; hh < H; ++hh) {
+    const double2 pn = bin((int)(code >> (12 + 7 * hh)) & 127);
+    const double amp = pn.x, ifr = pn.y;
+    const double inv_h = hh == 0 ? 1.0 : hh == 1 ? 0.5 : hh == 2 ? 1.0 / 3.0
+                       : hh == 3 ? 0.25 : hh == 4 ? 0.2 : 1.0 / 6.0;
+    numer += amp * ifr;
+    den += amp * (hh + 1.0);
+    dev += fabs((ifr * inv_h - f0) * inv_f0);
+  }
+  double refined = numer / (den + kGuard);
+  double score = 1.0 / (dev / H + kGuard);
+  if (refined < c.f0_floor || refined > c.f0_ceil || score < 2.5) {
+    refined = 0.0;
+    score = 0.0;
+  }
+  return make_double2(refined, score);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -164,12 +200,15 @@ __global__ __launch_bounds__(kThreads) void k_bt_tables(Config c, BinTable t, co
 // (:577-578) and q_j the index, among h's table bins, of the bin k_j of
 // harmonic j (:515; 7 bits each); 0 for a candidate that is not positive or
 // not one of the utterance's count.
-// inv[f][jj] = 1 / f0.  A half window or bin outside the tables sets the
-// fault flag.
+// inv[f][jj] = 1 / f0, hbs[f][jj] = hb (0xFF without a code; the row's
+// padding to bt_hb_stride bytes is 0xFF too).  A half window or bin outside
+// the tables sets the fault flag.  The lane also clears frame f's slots g cnt
+// + jj (g < 7) of rf0 and rscore: slots without a candidate stay 0, the
+// others are written by k_bt_dft or k_bt_finish.
 __global__ __launch_bounds__(kThreads) void k_bt_codes(
     Config c, const long long* f_off, long long total_frames, const double* official,
     const int* count, const int4* harm, unsigned long long* codes, double* inv,
-    unsigned long long* work) {
+    unsigned char* hbs, double* rf0, double* rscore, unsigned long long* work) {
   const long long e = (long long)blockIdx.x * kThreads + threadIdx.x;
   if (e >= total_frames * c.capacity) return;
   const long long gf = e / c.capacity;
@@ -200,6 +239,17 @@ __global__ __launch_bounds__(kThreads) void k_bt_codes(
   }
   codes[e] = code;
   inv[e] = r;
+  const int cnt_u = count[ui];
+  if (jj < cnt_u)
+#pragma unroll
+    for (int g = 0; g < 7; ++g) {
+      rf0[gf * c.slots + g * cnt_u + jj] = 0.0;
+      rscore[gf * c.slots + g * cnt_u + jj] = 0.0;
+    }
+  const long long stride = bt_hb_stride(c.capacity);
+  hbs[gf * stride + jj] = (code & 1ull) ? (unsigned char)((code >> 1) & 0xFF) : 0xFF;
+  if (jj == c.capacity - 1)
+    for (long long b = c.capacity; b < stride; ++b) hbs[gf * stride + b] = 0xFF;
   if (fault && work) atomicOr(&work[kStatFault * kWorkSpread + (blockIdx.x % kWorkSpread)], 1ull);
 }
 
@@ -403,6 +453,9 @@ __device__ __forceinline__ void bt_pass(const double* yu, long long first, long 
   // registers throughout); at each block's first pair the next block's loads
   // are issued into the other buffer.  Block b is in buf0 when b is even.
   constexpr int kPairs = kBtBlock / 2;
+  // The wave's earlier reads of this space (the last pass's samples or bin
+  // results) are done before the first block's loads overwrite it.
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
   issue(0, lds0, buf0);
   const int npairs = (len - 1) / 2;
   for (int p = 0; p < npairs; ++p) {
@@ -450,21 +503,125 @@ __device__ __forceinline__ void bt_chunk(bool clamp, const double* yu, long long
     }
 }
 
+// LDS per wavefront of k_bt_dft (doubles): two staging buffers of kBtBlock
+// samples per lane, later the (amplitude, IF) of up to kBtPassBins bins per lane.
+constexpr int kBtStage = (kBtBlock > kBtPassBins ? kBtBlock : kBtPassBins) * 128;
+
+// Where the slots of a pair find their candidates and put their results.
+struct SlotArgs {
+  const int* count;
+  const unsigned long long* codes;
+  const double* official;
+  const double* inv;
+  const unsigned char* hbs;
+  double* rf0;
+  double* rscore;
+};
+
+// FixF0 of every slot of the lane's pair (frame gf, half window hb): the
+// slots of frame i whose candidate gives h are those of the official
+// candidates of frames i - 3 .. i + 3 (OverlapF0Candidates
+// harvest.cpp:417-429, slot g cnt + jj for candidate jj of source group g)
+// with that half window.  hbs rows hold each candidate's hb as a byte: the
+// seven rows' bytes (16 candidates at a time) are loaded together and
+// turned into a bitmask of matches by zero-byte tests of the rows XOR hb,
+// then one loop takes the matches, so that the wave's rounds are its lanes'
+// largest match count.  bin(q) is the pair's bin q.
+template <class Bin>
+__device__ __forceinline__ void bt_pair_slots(const Config& c, const SlotArgs& a, long long gf,
+                                              long long uf, long long frames, int cnt_u, int hb,
+                                              Bin bin) {
+  const long long i = gf - uf, stride = bt_hb_stride(c.capacity);
+  const unsigned pat = (unsigned)hb * 0x01010101u;
+  double* out_f0 = a.rf0 + gf * c.slots;
+  double* out_sc = a.rscore + gf * c.slots;
+  for (int j0 = 0; j0 < cnt_u; j0 += 16) {
+    const unsigned valid = cnt_u - j0 >= 16 ? 0xFFFFu : (1u << (cnt_u - j0)) - 1u;
+    unsigned long long m[2] = {0ull, 0ull};  // bit 16 (g & 3) + j of m[g >> 2]: candidate j0 + j
+#pragma unroll
+    for (int g = 0; g < 7; ++g) {
+      const long long src = overlap_source(i, g);
+      uint4 w = make_uint4(~0u, ~0u, ~0u, ~0u);
+      if (src >= 0 && src < frames)
+        w = reinterpret_cast<const uint4*>(a.hbs + (uf + src) * stride)[j0 >> 4];
+      const unsigned words[4] = {w.x, w.y, w.z, w.w};
+      unsigned bits = 0;
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        const unsigned x = words[d] ^ pat;
+        // 0x80 in each byte of x that is zero (no carry crosses a byte).
+        const unsigned z = ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x) & 0x80808080u;
+        bits |= (((z >> 7) & 1u) | ((z >> 14) & 2u) | ((z >> 21) & 4u) | ((z >> 28) & 8u))
+                << (4 * d);
+      }
+      m[g >> 2] |= (unsigned long long)(bits & valid) << (16 * (g & 3));
+    }
+    while (m[0] | m[1]) {
+      const int half = m[0] ? 0 : 1;
+      const unsigned long long mh = half ? m[1] : m[0];
+      const int b = __builtin_ctzll(mh);
+      if (half) m[1] = mh & (mh - 1);
+      else m[0] = mh & (mh - 1);
+      const int g = 4 * half + (b >> 4), jj = j0 + (b & 15);
+      const long long e = (uf + overlap_source(i, g)) * c.capacity + jj;
+      const double2 v = fix_f0_slot(c, a.codes[e], a.official[e], a.inv[e], bin);
+      const int sl = g * cnt_u + jj;
+      out_f0[sl] = v.x;
+      out_sc[sl] = v.y;
+    }
+  }
+}
+
+// A pair whose half window has at most kBtPassBins bins: one pass over the
+// window, the bins' (amplitude, IF) to LDS (over the staging space, whose
+// reads are done), then FixF0 of the pair's slots from them.
+template <int NQ>
+__device__ __forceinline__ void bt_chunk_slots(bool clamp, const double* yu, long long first,
+                                               long long last, int len,
+                                               const CONSTANT_AS double* win,
+                                               const CONSTANT_AS double* rec, LDS double* buf0,
+                                               LDS double* buf1, int lane, bool act, double fs,
+                                               const Config& c, const SlotArgs& a, long long gf,
+                                               long long uf, long long frames, int cnt_u, int hb) {
+  static_assert(128 * NQ <= kBtStage, "bin results exceed the staging space");
+  Goertzel<NQ> g;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) g.c2[q] = rec[q * kBtRecord];
+  g.clear();
+  bt_pass<NQ>(yu, first, last, clamp, len, win, buf0, buf1, lane, g);
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const CONSTANT_AS double* r = rec + q * kBtRecord;
+    const double2 v = bin_amp_if(
+        goertzel_result(g.a2[q], g.a1[q], g.b2[q], g.b1[q], r[1], r[2], r[3], r[4]), r[5], fs);
+    buf0[2 * (64 * q + lane)] = v.x;
+    buf0[2 * (64 * q + lane) + 1] = v.y;
+  }
+  if (act)
+    bt_pair_slots(c, a, gf, uf, frames, cnt_u, hb, [&](int q) {
+      const int at = 2 * (64 * q + lane);
+      return make_double2(buf0[at], buf0[at + 1]);
+    });
+}
+
 }  // namespace
 
 // One wavefront per chunk of 64 pairs of one h.  A workgroup takes four
 // consecutive chunks at a time from a counter until none is left (a
 // persistent grid): its waves then mostly share one h, so that they stream
-// the same table rows through the scalar cache together.
-// res[(frame kBtPairs + rank) bt_nbmax + q] receives (amplitude,
-// instantaneous frequency) of the pair's bin q (bin_amp_if).
+// the same table rows through the scalar cache together.  Then the pair's
+// slots are finished here (bt_pair_slots writes their refined f0 and
+// score) from the (amplitude, instantaneous frequency) of its bins
+// (bin_amp_if): in LDS when h has at most kBtPassBins bins (one pass),
+// otherwise in res[(frame kBtPairs + rank) bt_nbmax + q], written by the
+// passes and read back by the same lane.
 __global__ __launch_bounds__(kThreads) void k_bt_dft(
     Config c, const Utt* utt, const double* y, const int2* list, const int* cinfo, int* sched,
-    BinTable t, double2* res, unsigned long long* work) {
+    BinTable t, double2* res, SlotArgs slots, unsigned long long* work) {
   __shared__ int s_info[3 * kBtMaxHr + 1];
   __shared__ int s_next;
-  // Sample staging: two buffers per wave of kBtBlock samples per lane.
-  __shared__ double stage0[kThreads / 64][kBtBlock * 64], stage1[kThreads / 64][kBtBlock * 64];
+  // Per wave: two sample staging buffers, later the bin results.
+  __shared__ double stage[kThreads / 64][kBtStage];
   const int HR = c.bt_hr;
   for (int k = threadIdx.x; k < 3 * HR + 1; k += kThreads) s_info[k] = cinfo[k];
   __syncthreads();
@@ -473,8 +630,8 @@ __global__ __launch_bounds__(kThreads) void k_bt_dft(
   const int* hend = s_info + 2 * HR + 1;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  LDS double* buf0 = (LDS double*)stage0[wave];
-  LDS double* buf1 = (LDS double*)stage1[wave];
+  LDS double* buf0 = (LDS double*)stage[wave];
+  LDS double* buf1 = buf0 + kBtBlock * 64;
   const int total = cpre[HR];
   const int spf = (int)(c.fs_work / 1000.0);  // samples per 1 ms frame (whole)
   unsigned long long wsamp = 0, bsamp = 0, nbins = 0, npairs = 0;
@@ -513,10 +670,26 @@ __global__ __launch_bounds__(kThreads) void k_bt_dft(
     const int nb = __builtin_amdgcn_readfirstlane(info.y);
     const CONSTANT_AS double* win = wins + 2LL * __builtin_amdgcn_readfirstlane(info.z);
     const CONSTANT_AS double* rec0 = bins + (long long)kBtRecord * __builtin_amdgcn_readfirstlane(info.w);
+    static_assert(kBtPassBins <= 12, "the pass switches below have cases for up to 12 bins");
+    if (nb <= kBtPassBins) {
+      const int cnt_u = slots.count[ui];
+      const long long frames = utt[ui].frames;
+#define BT_SLOTS(NQ)                                                                      \
+  case NQ:                                                                               \
+    bt_chunk_slots<NQ>(clamp, yu, first, last, len, win, rec0, buf0, buf1, lane, act,    \
+                       c.fs_work, c, slots, gf, uf, frames, cnt_u, hb);                  \
+    break;
+      switch (nb) {
+        BT_SLOTS(1) BT_SLOTS(2) BT_SLOTS(3) BT_SLOTS(4)
+        BT_SLOTS(5) BT_SLOTS(6) BT_SLOTS(7) BT_SLOTS(8)
+        BT_SLOTS(9) BT_SLOTS(10) BT_SLOTS(11) BT_SLOTS(12)
+        default: break;
+      }
+#undef BT_SLOTS
+    }
     double2* out = res + (gf * kBtPairs + r) * c.bt_nbmax;
     // Passes of at most kBtPassBins bins, split evenly.
-    static_assert(kBtPassBins <= 12, "the pass switch below has cases for up to 12 bins");
-    const int npass = (nb + kBtPassBins - 1) / kBtPassBins;
+    const int npass = nb <= kBtPassBins ? 0 : (nb + kBtPassBins - 1) / kBtPassBins;
     for (int p = 0; p < npass; ++p) {
       const int q0 = p * nb / npass, nq = (p + 1) * nb / npass - q0;
       const CONSTANT_AS double* rec = rec0 + (long long)q0 * kBtRecord;
@@ -533,6 +706,9 @@ __global__ __launch_bounds__(kThreads) void k_bt_dft(
       }
 #undef BT_CHUNK
     }
+    if (npass > 0 && act)
+      bt_pair_slots(c, slots, gf, uf, utt[ui].frames, slots.count[ui], hb,
+                    [&](int q) { return out[q]; });
     const unsigned long long na = (unsigned long long)__popcll(__ballot(act));
     npairs += na;
     wsamp += na * len;
@@ -577,87 +753,46 @@ __device__ double2 bt_bin_direct(const double* yu, long long first, long long la
 
 }  // namespace
 
-// FixF0 (harvest.cpp:507-536) and GetRefinedF0's gate (:610-614) of every
-// slot, one wavefront per frame (lanes over slots), from the slot's
-// candidate code and the bins' amplitudes and frequencies of its (frame, h)
-// pair.  The error term's divisions by j + 1 and by f0 (and the frequency's
-// by 2 pi, in bin_amp_if) are products with reciprocals: within two ulps of
-// WORLD's quotients, far below the spectra's own rounding; the score they
-// feed is compared with 2.5, so a slot could change only if its score fell
-// within a few ulps of that threshold.
+// FixF0 (harvest.cpp:507-536) and GetRefinedF0's gate (:610-614) of the
+// frames k_bt_dft leaves: those with more than kBtPairs half windows, whose
+// pairs are not listed.  One wavefront per frame (lanes over slots); the
+// bins are computed here (bt_bin_direct); other frames return at once.
 __global__ __launch_bounds__(kThreads) void k_bt_finish(
     Config c, const Utt* utt, const long long* f_off, long long total_frames, const double* y,
     const double* official, const unsigned long long* codes, const double* inv,
-    const int* count, const unsigned long long* needs, BinTable t, const double2* res,
-    double* rf0, double* rscore) {
+    const int* count, const unsigned long long* needs, BinTable t, double* rf0,
+    double* rscore) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const long long gf = (long long)blockIdx.x * (kThreads / 64) + wave;
   if (gf >= total_frames) return;
+  int pairs = 0;
+#pragma unroll
+  for (int w = 0; w < kBtWords; ++w) pairs += __popcll(needs[gf * kBtWords + w]);
+  if (pairs <= kBtPairs) return;
   const int ui = find_segment(f_off, c.count, gf);
   const int cnt_u = count[ui];
   const int ncand = 7 * cnt_u;
-  if (ncand == 0) return;
   const long long uf = utt[ui].f_off, frames = utt[ui].frames;
   const long long i = gf - uf;
   const double fs = c.fs_work;
-  unsigned long long nw[kBtWords];
-  int pairs = 0;
-#pragma unroll
-  for (int w = 0; w < kBtWords; ++w) {
-    nw[w] = needs[gf * kBtWords + w];
-    pairs += __popcll(nw[w]);
-  }
-  const bool direct = pairs > kBtPairs;
   double* out_f0 = rf0 + gf * c.slots;
   double* out_sc = rscore + gf * c.slots;
-  const double2* row = res + gf * kBtPairs * c.bt_nbmax;
   const int spf = (int)(fs / 1000.0);
   for (int s = lane; s < ncand; s += 64) {
     const int g = s / cnt_u, jj = s - g * cnt_u;
     const long long src = overlap_source(i, g);
     const long long e = (uf + src) * c.capacity + jj;
     const unsigned long long code = (src >= 0 && src < frames) ? codes[e] : 0ull;
-    double refined = 0.0, score = 0.0;
-    if (code & 1ull) {
-      const double f0 = official[e], inv_f0 = inv[e];
-      const int hb = (int)(code >> 1) & 0xFF, H = (int)(code >> 9) & 7, h = c.bt_hlo + hb;
-      int r = 0;  // rank of h among the frame's half windows
-#pragma unroll
-      for (int w = 0; w < kBtWords; ++w) {
-        const int lo = w * 64;
-        const unsigned long long m =
-            hb >= lo + 64 ? ~0ull : (hb <= lo ? 0ull : ((1ull << (hb - lo)) - 1ull));
-        r += __popcll(nw[w] & m);
-      }
-      const double2* pr = row + (long long)r * c.bt_nbmax;
-      double numer = 0.0, den = 0.0, dev = 0.0;
-      for (int hh = 0; hh < H; ++hh) {
-        const int q = (int)(code >> (12 + 7 * hh)) & 127;
-        double2 pn;
-        if (direct) {
-          const int4 info = t.info[hb];
-          pn = bt_bin_direct(y + utt[ui].y_off, (long long)spf * i - h - 1,
-                             (long long)utt[ui].n - 1, h, t.win + info.z,
-                             t.bins + (long long)(info.w + q) * kBtRecord, fs);
-        } else {
-          pn = pr[q];
-        }
-        const double amp = pn.x, ifr = pn.y;
-        const double inv_h = hh == 0 ? 1.0 : hh == 1 ? 0.5 : hh == 2 ? 1.0 / 3.0
-                           : hh == 3 ? 0.25 : hh == 4 ? 0.2 : 1.0 / 6.0;
-        numer += amp * ifr;
-        den += amp * (hh + 1.0);
-        dev += fabs((ifr * inv_h - f0) * inv_f0);
-      }
-      refined = numer / (den + kGuard);
-      score = 1.0 / (dev / H + kGuard);
-      if (refined < c.f0_floor || refined > c.f0_ceil || score < 2.5) {
-        refined = 0.0;
-        score = 0.0;
-      }
-    }
-    out_f0[s] = refined;
-    out_sc[s] = score;
+    if (!(code & 1ull)) continue;  // cleared by k_bt_codes
+    const int hb = (int)(code >> 1) & 0xFF, h = c.bt_hlo + hb;
+    const int4 info = t.info[hb];
+    const double2 v = fix_f0_slot(c, code, official[e], inv[e], [&](int q) {
+      return bt_bin_direct(y + utt[ui].y_off, (long long)spf * i - h - 1,
+                           (long long)utt[ui].n - 1, h, t.win + info.z,
+                           t.bins + (long long)(info.w + q) * kBtRecord, fs);
+    });
+    out_f0[s] = v.x;
+    out_sc[s] = v.y;
   }
 }
 
@@ -684,7 +819,7 @@ void launch_refine_table(hipStream_t s, const Config& c, const Utt* utt, const l
   const unsigned grid_blk = (unsigned)((nblk + kThreads / 64 - 1) / (kThreads / 64));
   const long long entries = total_frames * c.capacity;
   k_bt_codes<<<(unsigned)((entries + kThreads - 1) / kThreads), kThreads, 0, s>>>(
-      c, f_off, total_frames, official, count, t.harm, b.codes, b.inv, work);
+      c, f_off, total_frames, official, count, t.harm, b.codes, b.inv, b.hbs, rf0, rscore, work);
   check_launch("bt_codes");
   k_bt_plan<<<grid_blk, kThreads, 0, s>>>(c, utt, f_off, total_frames, nblk, b.codes, count,
                                           b.needs, b.mask, b.cnt);
@@ -701,13 +836,13 @@ void launch_refine_table(hipStream_t s, const Config& c, const Utt* utt, const l
   const long long cap = c.bt_groups > 0 ? c.bt_groups : kBtDftGroups;
   groups = groups < 64 ? 64 : (groups > cap ? cap : groups);
   if (dft_begin) record(dft_begin, s);
+  const SlotArgs slots{count, b.codes, official, b.inv, b.hbs, rf0, rscore};
   k_bt_dft<<<(unsigned)groups, kThreads, 0, s>>>(c, utt, y, b.list, b.cinfo, b.sched, t, b.res,
-                                                 work);
+                                                 slots, work);
   check_launch("bt_dft");
   if (dft_end) record(dft_end, s);
   k_bt_finish<<<(unsigned)((total_frames + 3) / 4), kThreads, 0, s>>>(
-      c, utt, f_off, total_frames, y, official, b.codes, b.inv, count, b.needs, t, b.res, rf0,
-      rscore);
+      c, utt, f_off, total_frames, y, official, b.codes, b.inv, count, b.needs, t, rf0, rscore);
   check_launch("bt_finish");
 }
 
