@@ -507,6 +507,9 @@ __device__ __forceinline__ void bt_chunk(bool clamp, const double* yu, long long
 // samples per lane, later the (amplitude, IF) of up to kBtPassBins bins per lane.
 constexpr int kBtStage = (kBtBlock > kBtPassBins ? kBtBlock : kBtPassBins) * 128;
 
+// Matched slots whose candidates one lane loads together (bt_pair_slots).
+constexpr int kBtSlotBatch = 8;
+
 // Where the slots of a pair find their candidates and put their results.
 struct SlotArgs {
   const int* count;
@@ -518,56 +521,168 @@ struct SlotArgs {
   double* rscore;
 };
 
-// FixF0 of every slot of the lane's pair (frame gf, half window hb): the
-// slots of frame i whose candidate gives h are those of the official
-// candidates of frames i - 3 .. i + 3 (OverlapF0Candidates
-// harvest.cpp:417-429, slot g cnt + jj for candidate jj of source group g)
-// with that half window.  hbs rows hold each candidate's hb as a byte: the
-// seven rows' bytes (16 candidates at a time) are loaded together and
-// turned into a bitmask of matches by zero-byte tests of the rows XOR hb,
-// then one loop takes the matches, so that the wave's rounds are its lanes'
-// largest match count.  bin(q) is the pair's bin q.
+// The slots of frame i (utterance frames [uf, uf + frames)) whose candidate
+// gives half window hb are those of the official candidates of frames i - 3
+// .. i + 3 (OverlapF0Candidates harvest.cpp:417-429, slot g cnt + jj for
+// candidate jj of source group g) with that half window.  hbs rows hold
+// each candidate's hb as a byte: the seven rows' bytes of candidates j0 ..
+// j0 + 15 are loaded together and turned into a bitmask of matches by
+// zero-byte tests of the rows XOR hb; bit 16 (g & 3) + j of m[g >> 2] is
+// candidate j0 + j of group g.
+__device__ __forceinline__ void bt_match_mask(const Config& c, const SlotArgs& a, long long i,
+                                              long long uf, long long frames, int cnt_u, int hb,
+                                              int j0, unsigned long long m[2]) {
+  const long long stride = bt_hb_stride(c.capacity);
+  const unsigned pat = (unsigned)hb * 0x01010101u;
+  const unsigned valid = j0 >= cnt_u ? 0u
+                       : cnt_u - j0 >= 16 ? 0xFFFFu : (1u << (cnt_u - j0)) - 1u;
+  m[0] = m[1] = 0ull;
+#pragma unroll
+  for (int g = 0; g < 7; ++g) {
+    const long long src = overlap_source(i, g);
+    uint4 w = make_uint4(~0u, ~0u, ~0u, ~0u);
+    if (valid && src >= 0 && src < frames)
+      w = reinterpret_cast<const uint4*>(a.hbs + (uf + src) * stride)[j0 >> 4];
+    const unsigned words[4] = {w.x, w.y, w.z, w.w};
+    unsigned bits = 0;
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+      const unsigned x = words[d] ^ pat;
+      // 0x80 in each byte of x that is zero (no carry crosses a byte).
+      const unsigned z = ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x) & 0x80808080u;
+      bits |= (((z >> 7) & 1u) | ((z >> 14) & 2u) | ((z >> 21) & 4u) | ((z >> 28) & 8u))
+              << (4 * d);
+    }
+    m[g >> 2] |= (unsigned long long)(bits & valid) << (16 * (g & 3));
+  }
+}
+
+// Position of the n-th (from 0) set bit of w, n < popcount(w).
+__device__ __forceinline__ int select_bit(unsigned long long w, int n) {
+  int pos = 0;
+#pragma unroll
+  for (int width = 32; width > 0; width >>= 1) {
+    const int cnt = __popcll((w >> pos) & ((1ull << width) - 1ull));
+    if (n >= cnt) {
+      n -= cnt;
+      pos += width;
+    }
+  }
+  return pos;
+}
+
+__device__ __forceinline__ unsigned long long shfl_u64(unsigned long long v, int l) {
+  const unsigned lo = __shfl((unsigned)v, l, 64), hi = __shfl((unsigned)(v >> 32), l, 64);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+// FixF0 of every slot of the wave's pairs (lane l: frame gf, half window
+// hb, active when act) whose bins are in LDS, rr[2 (64 q + l) + {0, 1}]:
+// the lanes' matches (bt_match_mask) are numbered across the wave by a
+// prefix sum of their counts, and each round every lane takes one of them
+// (its owner lane by a binary search of the prefix sums, its slot by the
+// owner's mask), so that the wave's rounds are its total match count / 64
+// rather than its largest per-lane count.
+__device__ __forceinline__ void bt_wave_slots(const Config& c, const SlotArgs& a, long long gf,
+                                              long long uf, long long frames, int cnt_u, int hb,
+                                              bool act, int lane, const LDS double* rr) {
+  const long long i = gf - uf;
+  int jmax = act ? cnt_u : 0;
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) jmax = max(jmax, __shfl_xor(jmax, d, 64));
+  for (int j0 = 0; j0 < jmax; j0 += 16) {
+    unsigned long long m[2];
+    bt_match_mask(c, a, i, uf, frames, act ? cnt_u : 0, hb, j0, m);
+    const int cnt = __popcll(m[0]) + __popcll(m[1]);
+    int incl = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int o = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += o;
+    }
+    const int excl = incl - cnt;
+    const int total = __shfl(incl, 63, 64);
+    for (int k0 = 0; k0 < total; k0 += 64) {
+      const int k = k0 + lane;
+      int o = 0;  // owner: the last lane whose first match index is <= k
+#pragma unroll
+      for (int step = 32; step > 0; step >>= 1)
+        if (__shfl(excl, o + step, 64) <= k) o += step;
+      const int n = k - __shfl(excl, o, 64);
+      const unsigned long long m0 = shfl_u64(m[0], o), m1 = shfl_u64(m[1], o);
+      const long long gf_o = (long long)shfl_u64((unsigned long long)gf, o);
+      const long long uf_o = (long long)shfl_u64((unsigned long long)uf, o);
+      const int cnt_o = __shfl(cnt_u, o, 64);
+      if (k >= total) continue;
+      const int c0 = __popcll(m0);
+      const bool upper = n >= c0;
+      const int b = select_bit(upper ? m1 : m0, upper ? n - c0 : n);
+      const int g = 4 * upper + (b >> 4), jj = j0 + (b & 15);
+      const long long e = (uf_o + overlap_source(gf_o - uf_o, g)) * c.capacity + jj;
+      const double2 v = fix_f0_slot(c, a.codes[e], a.official[e], a.inv[e], [&](int q) {
+        const int at = 2 * (64 * q + o);
+        return make_double2(rr[at], rr[at + 1]);
+      });
+      const long long sl = gf_o * c.slots + g * cnt_o + jj;
+      a.rf0[sl] = v.x;
+      a.rscore[sl] = v.y;
+    }
+  }
+}
+
+// FixF0 of every slot of the lane's pair (frame gf, half window hb), one
+// lane alone (bt_match_mask, then the matches in order); bin(q) is the
+// pair's bin q.
 template <class Bin>
 __device__ __forceinline__ void bt_pair_slots(const Config& c, const SlotArgs& a, long long gf,
                                               long long uf, long long frames, int cnt_u, int hb,
                                               Bin bin) {
-  const long long i = gf - uf, stride = bt_hb_stride(c.capacity);
-  const unsigned pat = (unsigned)hb * 0x01010101u;
+  const long long i = gf - uf;
   double* out_f0 = a.rf0 + gf * c.slots;
   double* out_sc = a.rscore + gf * c.slots;
   for (int j0 = 0; j0 < cnt_u; j0 += 16) {
-    const unsigned valid = cnt_u - j0 >= 16 ? 0xFFFFu : (1u << (cnt_u - j0)) - 1u;
-    unsigned long long m[2] = {0ull, 0ull};  // bit 16 (g & 3) + j of m[g >> 2]: candidate j0 + j
-#pragma unroll
-    for (int g = 0; g < 7; ++g) {
-      const long long src = overlap_source(i, g);
-      uint4 w = make_uint4(~0u, ~0u, ~0u, ~0u);
-      if (src >= 0 && src < frames)
-        w = reinterpret_cast<const uint4*>(a.hbs + (uf + src) * stride)[j0 >> 4];
-      const unsigned words[4] = {w.x, w.y, w.z, w.w};
-      unsigned bits = 0;
-#pragma unroll
-      for (int d = 0; d < 4; ++d) {
-        const unsigned x = words[d] ^ pat;
-        // 0x80 in each byte of x that is zero (no carry crosses a byte).
-        const unsigned z = ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x) & 0x80808080u;
-        bits |= (((z >> 7) & 1u) | ((z >> 14) & 2u) | ((z >> 21) & 4u) | ((z >> 28) & 8u))
-                << (4 * d);
-      }
-      m[g >> 2] |= (unsigned long long)(bits & valid) << (16 * (g & 3));
-    }
-    while (m[0] | m[1]) {
+    unsigned long long m[2];
+    bt_match_mask(c, a, i, uf, frames, cnt_u, hb, j0, m);
+    // The matches in order (slot g cnt + jj), kBtSlotBatch at a time: the
+    // batch's candidates are all loaded before any FixF0, so that one load
+    // latency serves the batch.
+    auto take = [&](int& sl, long long& e) -> bool {
+      if (!(m[0] | m[1])) return false;
       const int half = m[0] ? 0 : 1;
       const unsigned long long mh = half ? m[1] : m[0];
       const int b = __builtin_ctzll(mh);
       if (half) m[1] = mh & (mh - 1);
       else m[0] = mh & (mh - 1);
       const int g = 4 * half + (b >> 4), jj = j0 + (b & 15);
-      const long long e = (uf + overlap_source(i, g)) * c.capacity + jj;
-      const double2 v = fix_f0_slot(c, a.codes[e], a.official[e], a.inv[e], bin);
-      const int sl = g * cnt_u + jj;
-      out_f0[sl] = v.x;
-      out_sc[sl] = v.y;
+      sl = g * cnt_u + jj;
+      e = (uf + overlap_source(i, g)) * c.capacity + jj;
+      return true;
+    };
+    while (m[0] | m[1]) {
+      int sl[kBtSlotBatch];
+      bool ok[kBtSlotBatch];
+      unsigned long long code[kBtSlotBatch];
+      double f0[kBtSlotBatch], inv_f0[kBtSlotBatch];
+#pragma unroll
+      for (int k = 0; k < kBtSlotBatch; ++k) {
+        long long e = 0;
+        sl[k] = 0;
+        ok[k] = take(sl[k], e);
+        code[k] = 0;
+        f0[k] = inv_f0[k] = 0.0;
+        if (ok[k]) {
+          code[k] = a.codes[e];
+          f0[k] = a.official[e];
+          inv_f0[k] = a.inv[e];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kBtSlotBatch; ++k)
+        if (ok[k]) {
+          const double2 v = fix_f0_slot(c, code[k], f0[k], inv_f0[k], bin);
+          out_f0[sl[k]] = v.x;
+          out_sc[sl[k]] = v.y;
+        }
     }
   }
 }
@@ -597,11 +712,7 @@ __device__ __forceinline__ void bt_chunk_slots(bool clamp, const double* yu, lon
     buf0[2 * (64 * q + lane)] = v.x;
     buf0[2 * (64 * q + lane) + 1] = v.y;
   }
-  if (act)
-    bt_pair_slots(c, a, gf, uf, frames, cnt_u, hb, [&](int q) {
-      const int at = 2 * (64 * q + lane);
-      return make_double2(buf0[at], buf0[at + 1]);
-    });
+  bt_wave_slots(c, a, gf, uf, frames, cnt_u, hb, act, lane, buf0);
 }
 
 }  // namespace
