@@ -147,7 +147,8 @@ def roofline(kernels_ms, work, launches, workload, lengths, fs):
     recurrence over the main- and derivative-windowed samples: 6 FLOP per
     bin-sample (an FMA and a subtraction per signal, an FMA counted as 2)
     plus 2 per window sample (the two window products), counted on the
-    device; it is timed alone.  Other rates run k_refine_waves: 8 FLOP per
+    device; it is timed alone (its time includes FixF0 of the slots that
+    read each pair, whose FLOPs are not counted).  Other rates run k_refine_waves: 8 FLOP per
     bin-sample plus 17 per window sample, timed with its small set-up
     kernels.  The filterbank's FLOPs are filterbank_flops().  `launches` is
     the analyses per step (one per bucket): a launch here is one analysis'

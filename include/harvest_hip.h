@@ -45,8 +45,8 @@ typedef enum hv_status {
  *   [3] decimation scan kernels      [4] filterbank + crossing kernel
  *   [5] tile scan + interpolation    [6] refinement kernels
  *   [7] prune, best, scan, compaction
- *   [8] bin-table DFT kernel alone (part of [6]; 0 when the per-frame
- *       refinement ran) */
+ *   [8] bin-table DFT kernel alone, its bins and the slots' FixF0 (part
+ *       of [6]; 0 when the per-frame refinement ran) */
 #define HV_TIMING_SLOTS 9
 
 /* Work counters of the last analysis call (hv_last_stats):
